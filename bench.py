@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Benchmark: batched reverse-time migration, 64 shots, on one GPU.
+
+Workload (one "step" = one full RTM of the 64-shot batch, inputs resident in
+HBM): grid 301 x 1001 at 10 m with a 150-cell random-boundary pad
+(601 x 1301 cells), nt = 4001 at dt = 0.8 ms (+ tlag), Ricker 10 Hz peak
+(max_frequency 30 Hz), 64 shots along the surface, 1001 receivers at 10 m,
+six seeded dipping layers 1500-4000 m/s, migration velocity = the true model
+smoothed (sigma 20 cells, slowness).  Observed data are modeled from the true
+model (sponge boundary) and direct-wave muted before the timed region.
+
+Per step: random-boundary generation, one forward pass and one reverse pass
+(source reversal + receiver back-propagation + imaging) over all 64 shots.
+Cell-updates = shots x nzz x nxx x T per pass, 2 per cell in the reverse pass.
+
+Prints one JSON line (the metric is Gcell-updates/s).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_FWD = 16.0      # read u_t, u_{t-1}, v; write u_{t+1}  (FP32)
+BYTES_REV = 22.0      # per cell-update of the reverse pass (44 B per cell-step)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--shots", type=int, default=64)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--vz", type=int, default=0)
+    p.add_argument("--xw", type=int, default=0)
+    p.add_argument("--graph", type=int, default=-1)
+    return p.parse_args()
+
+
+def cpu_baseline(s, vmig, xs, zs, xr, zr, w, tlag, obs0, threads):
+    """FP32 CPU statement of the reference stencil (OpenMP), one shot of the
+    same workload: random-boundary forward + reverse imaging pass."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    nb = s.nb
+    vpad = ob.expand(vmig, nb)
+    gi = lambda x: ob.grid_index(float(x), np.float32(s.dh), nb)
+    riz = np.array([gi(z) for z in zr], np.int32)
+    rix = np.array([gi(x) for x in xr], np.int32)
+    one = np.float32(1.0)
+    rel = np.float32(s.varVp_rel)
+    vlo = vmig.min() * (one - rel) - np.float32(s.varVp)
+    vhi = vmig.max() * (one + rel) + np.float32(s.varVp)
+    vrb = ob.rb_velocity(vpad, nb, np.float32(s.varVp), rel, vlo, vhi, s.seed, 0)
+    T = s.nt + tlag
+    nzz, nxx = s.nz + 2 * nb, s.nx + 2 * nb
+    t0 = time.perf_counter()
+    _, P, Q = ob.forward(vrb, s.dh, s.dt, s.nt, tlag, w, gi(zs[0]), gi(xs[0]), riz, rix,
+                         precision="f32", nthreads=threads)
+    ob.reverse(vrb, s.dh, s.dt, s.nt, tlag, P, Q, riz, rix, obs0, 0, precision="f32",
+               nthreads=threads)
+    el = time.perf_counter() - t0
+    cells = nzz * nxx * T * 3.0
+    return {"value": round(cells / el / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 of {s.nsrc} shots, full RTM (RB forward + reverse imaging), "
+                      f"{nzz}x{nxx} cells x {T} steps, FP32, OpenMP, {el:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the shot axis "
+              f"would shard across GPUs, but this build measures one GPU's share.",
+              file=sys.stderr)
+        sys.exit(2)
+    import fwi2d_amd as F
+    from fwi2d_amd import configs as C
+
+    s = C.CONFIG2
+    nshots = a.shots
+    true_v, mig_v = C.config2_models()
+    xs, zs, xr, zr = s.acquisition()
+    if nshots != s.nsrc:
+        sel = np.linspace(0, s.nsrc - 1, nshots).round().astype(int)
+        xs, zs = xs[sel], zs[sel]
+    w, tlag = F.ricker_wavelet(s.nt, s.dt, s.fmax)
+    T = s.nt + tlag
+
+    eng = F.Engine(s.nz, s.nx, s.dh, s.nt, s.dt, abc_nb=s.abc_nb, abc_factor=s.abc_factor,
+                   rbc_nb=s.nb, rbc_varVp=s.varVp, rbc_varVp_rel=s.varVp_rel, rbc_seed=s.seed,
+                   max_batch=nshots)
+    eng.set_tuning(a.vz, a.xw, a.graph)
+    eng.set_wavelet(w, tlag)
+    eng.set_geometry(xs, zs, xr, zr)
+    # observed data: true model, sponge boundary, direct-wave mute (setup, untimed)
+    eng.set_model(true_v)
+    obs = eng.modeling(F.ABC)
+    vtop = float(true_v[:, 0].min())
+    for k in range(nshots):
+        obs[k] = C.mute_direct_wave(obs[k], xs[k], zs[k], xr, zr, s.dt, vtop,
+                                    delay=tlag * s.dt + 0.05)
+    eng.set_observed(obs)
+    eng.set_model(mig_v)
+    nzz, nxx, _ = eng.grid_shape(F.RBC)
+
+    for _ in range(a.warmup):
+        eng.reset(F.RESET_IMAGE | F.RESET_ILLUM)
+        eng.migration()
+    eng.synchronize()
+    eng.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        eng.reset(F.RESET_IMAGE | F.RESET_ILLUM)
+        eng.migration()
+    eng.synchronize()
+    el = time.perf_counter() - t0
+    fwd_ms, fwd_n, fwd_cells = eng.profile_read(0)
+    rev_ms, rev_n, rev_cells = eng.profile_read(1)
+    eng.profile_enable(False)
+    img, _, _ = eng.get_image()
+    if not np.all(np.isfinite(img)):
+        raise RuntimeError("non-finite image")
+
+    cells_step = nshots * nzz * nxx * T * 3.0
+    value = cells_step * a.steps / el / 1e9
+
+    def roof(ms, n, cells, bpc):
+        avg_s = ms / n * 1e-3
+        per_launch = cells / n
+        gbs = per_launch * bpc / avg_s / 1e9
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel_avg_us": round(avg_s * 1e6, 2),
+                "gcell_per_s": round(per_launch / avg_s / 1e9, 2)}
+
+    rev_roof = roof(rev_ms, rev_n, rev_cells, BYTES_REV)
+    fwd_roof = roof(fwd_ms, fwd_n, fwd_cells, BYTES_FWD)
+    rev_roof["kernel"] = "rev_step_kernel (fused reverse + imaging)"
+    fwd_roof["kernel"] = "fwd_step_kernel (random boundary)"
+    vz, xw, g = eng.get_tuning()
+    out = {
+        "metric": "Gcell-updates/s", "value": round(value, 2), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(el / a.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded dipping-layer model, modeled + muted observed data)",
+        "config": {"workload": "batched RTM, 64 shots, 301x1001 grid + 150-cell random "
+                               "boundary (601x1301), nt=4001+%d, dt=0.8 ms" % tlag,
+                   "shots": nshots, "nzz": nzz, "nxx": nxx, "steps_per_pass": T,
+                   "tuning": {"vz": vz, "xw": xw, "graph": g}},
+        "roofline": rev_roof,
+        "roofline_forward": fwd_roof,
+    }
+    eng.close()
+    if not a.no_cpu_baseline:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(s, mig_v, xs, zs, xr, zr, w, tlag,
+                                           obs[0].astype(np.float32), threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

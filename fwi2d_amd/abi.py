@@ -1,0 +1,245 @@
+"""ctypes binding of the engine's C ABI (include/fwi2d.h, lib/libfwi2d.so).
+
+Used by the Python tests and the benchmark; the product host side is the
+C++ executables under ``fwi2d_amd/host``.  Loading fails loudly when the
+library has not been built: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libfwi2d.so")
+
+FWI_OK = 0
+FWI_ERR_ARG = 1
+FWI_ERR_STATE = 2
+FWI_ERR_HIP = 3
+FWI_ERR_NOMEM = 4
+FWI_ERR_NUMERIC = 5
+
+ABC = 0
+RBC = 1
+
+RESET_IMAGE = 1
+RESET_GRADIENT = 2
+RESET_ILLUM = 4
+
+
+class FwiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[fwi2d error {code}] {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("nz", C.c_int32), ("nx", C.c_int32), ("dh", C.c_float),
+        ("nt", C.c_int32), ("dt", C.c_float),
+        ("abc_nb", C.c_int32), ("abc_factor", C.c_float),
+        ("rbc_nb", C.c_int32), ("rbc_varVp", C.c_float), ("rbc_varVp_rel", C.c_float),
+        ("rbc_seed", C.c_uint64), ("max_batch", C.c_int32),
+    ]
+
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_ctx = C.c_void_p
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ("fwi_last_error", C.c_char_p, []),
+    ("fwi_version", C.c_char_p, []),
+    ("fwi_create", C.c_int, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    ("fwi_destroy", C.c_int, [_ctx]),
+    ("fwi_ricker_wavelet", C.c_int, [C.c_int32, C.c_float, C.c_float, _f32p,
+                                     C.POINTER(C.c_int32)]),
+    ("fwi_set_wavelet", C.c_int, [_ctx, _f32p, C.c_int32]),
+    ("fwi_set_geometry", C.c_int, [_ctx, C.c_int32, _f32p, _f32p, C.c_int32, _f32p, _f32p]),
+    ("fwi_set_model", C.c_int, [_ctx, _f32p]),
+    ("fwi_set_observed", C.c_int, [_ctx, C.c_int32, C.c_int32, _f32p]),
+    ("fwi_modeling", C.c_int, [_ctx, C.c_int32, C.c_int32, C.c_int32, _vp]),
+    ("fwi_migration", C.c_int, [_ctx, C.c_int32, C.c_int32]),
+    ("fwi_gradient", C.c_int, [_ctx, C.c_int32, C.c_int32, C.c_uint32,
+                               C.POINTER(C.c_double)]),
+    ("fwi_misfit", C.c_int, [_ctx, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    ("fwi_reset", C.c_int, [_ctx, C.c_int32]),
+    ("fwi_get_image", C.c_int, [_ctx, _vp, _vp, _vp]),
+    ("fwi_get_gradient", C.c_int, [_ctx, C.c_float, _vp, _vp, _vp, C.POINTER(C.c_double)]),
+    ("fwi_parabola_step", C.c_int, [_f64p, _f64p, C.POINTER(C.c_double)]),
+    ("fwi_rb_velocity", C.c_int, [_ctx, C.c_int32, C.c_uint32, _f32p]),
+    ("fwi_grid_shape", C.c_int, [_ctx, C.c_int32, C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("fwi_synchronize", C.c_int, [_ctx]),
+    ("fwi_set_tuning", C.c_int, [_ctx, C.c_int32, C.c_int32, C.c_int32]),
+    ("fwi_get_tuning", C.c_int, [_ctx, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32)]),
+    ("fwi_profile_enable", C.c_int, [_ctx, C.c_int32]),
+    ("fwi_profile_read", C.c_int, [_ctx, C.c_int32, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+]
+
+_lib = None
+
+
+def lib():
+    """Loads lib/libfwi2d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                f"g.build()'` or `make -C fwi2d_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != FWI_OK:
+        raise FwiError(rc, lib().fwi_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def ricker_wavelet(nt, dt, fmax):
+    """Host wavelet (FP64 DFT rounded to FP32) and its tlag."""
+    w = np.zeros(nt, np.float32)
+    tl = C.c_int32(0)
+    _check(lib().fwi_ricker_wavelet(nt, dt, fmax, w, C.byref(tl)))
+    return w, tl.value
+
+
+def parabola_step(alpha, f):
+    s = C.c_double(0.0)
+    _check(lib().fwi_parabola_step(np.ascontiguousarray(alpha, np.float64),
+                                   np.ascontiguousarray(f, np.float64), C.byref(s)))
+    return s.value
+
+
+class Engine:
+    """One survey context on the GPU (thin owner of an ``fwi_ctx``)."""
+
+    def __init__(self, nz, nx, dh, nt, dt, abc_nb=0, abc_factor=0.0, rbc_nb=0,
+                 rbc_varVp=0.0, rbc_varVp_rel=0.0, rbc_seed=0, max_batch=1):
+        cfg = Config(nz, nx, dh, nt, dt, abc_nb, abc_factor, rbc_nb, rbc_varVp,
+                     rbc_varVp_rel, rbc_seed, max_batch)
+        h = C.c_void_p()
+        _check(lib().fwi_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.nz, self.nx, self.nt = nz, nx, nt
+        self.nsrc = 0
+        self.nrec = 0
+
+    def close(self):
+        if self._h:
+            lib().fwi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_wavelet(self, w, tlag):
+        _check(lib().fwi_set_wavelet(self._h, np.ascontiguousarray(w, np.float32), tlag))
+
+    def set_geometry(self, xsrc, zsrc, xrec, zrec):
+        f = lambda a: np.ascontiguousarray(a, np.float32)
+        xs, zs, xr, zr = f(xsrc), f(zsrc), f(xrec), f(zrec)
+        _check(lib().fwi_set_geometry(self._h, xs.size, xs, zs, xr.size, xr, zr))
+        self.nsrc, self.nrec = xs.size, xr.size
+
+    def set_model(self, vp):
+        """vp: [nx, nz] float32 (z fastest)."""
+        _check(lib().fwi_set_model(self._h, np.ascontiguousarray(vp, np.float32)))
+
+    def set_observed(self, data, shot0=0):
+        data = np.ascontiguousarray(data, np.float32)
+        n = data.shape[0]
+        _check(lib().fwi_set_observed(self._h, shot0, n, data))
+
+    def modeling(self, boundary, shot0=0, nshots=None, record=True):
+        nshots = self.nsrc - shot0 if nshots is None else nshots
+        out = np.zeros((nshots, self.nrec, self.nt), np.float32) if record else None
+        _check(lib().fwi_modeling(self._h, boundary, shot0, nshots, _p(out)))
+        return out
+
+    def migration(self, shot0=0, nshots=None):
+        nshots = self.nsrc - shot0 if nshots is None else nshots
+        _check(lib().fwi_migration(self._h, shot0, nshots))
+
+    def gradient(self, shot0=0, nshots=None, iteration=0):
+        nshots = self.nsrc - shot0 if nshots is None else nshots
+        s = C.c_double(0.0)
+        _check(lib().fwi_gradient(self._h, shot0, nshots, iteration, C.byref(s)))
+        return s.value
+
+    def misfit(self, shot0=0, nshots=None):
+        nshots = self.nsrc - shot0 if nshots is None else nshots
+        s = C.c_double(0.0)
+        _check(lib().fwi_misfit(self._h, shot0, nshots, C.byref(s)))
+        return s.value
+
+    def reset(self, what=RESET_IMAGE | RESET_GRADIENT | RESET_ILLUM):
+        _check(lib().fwi_reset(self._h, what))
+
+    def get_image(self):
+        """Returns (image, raw_image, raw_illum), each [nx, nz]."""
+        shp = (self.nx, self.nz)
+        img, raw, ill = (np.zeros(shp, np.float32) for _ in range(3))
+        _check(lib().fwi_get_image(self._h, _p(img), _p(raw), _p(ill)))
+        return img, raw, ill
+
+    def get_gradient(self, zmask):
+        """Returns (direction, raw_grad, raw_illum, gmax)."""
+        shp = (self.nx, self.nz)
+        g, raw, ill = (np.zeros(shp, np.float32) for _ in range(3))
+        gm = C.c_double(0.0)
+        _check(lib().fwi_get_gradient(self._h, zmask, _p(g), _p(raw), _p(ill), C.byref(gm)))
+        return g, raw, ill, gm.value
+
+    def rb_velocity(self, shot, iteration=0):
+        nzz, nxx, _ = self.grid_shape(RBC)
+        out = np.zeros((nxx, nzz), np.float32)
+        _check(lib().fwi_rb_velocity(self._h, shot, iteration, out))
+        return out
+
+    def grid_shape(self, boundary):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().fwi_grid_shape(self._h, boundary, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def synchronize(self):
+        _check(lib().fwi_synchronize(self._h))
+
+    def set_tuning(self, vz=0, xw=0, use_graph=-1):
+        _check(lib().fwi_set_tuning(self._h, vz, xw, use_graph))
+
+    def get_tuning(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().fwi_get_tuning(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def profile_enable(self, on=True):
+        _check(lib().fwi_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self, kind):
+        ms, n, cells = C.c_double(), C.c_int64(), C.c_double()
+        _check(lib().fwi_profile_read(self._h, kind, C.byref(ms), C.byref(n), C.byref(cells)))
+        return ms.value, n.value, cells.value

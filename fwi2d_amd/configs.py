@@ -1,0 +1,216 @@
+"""Seeded survey generators and data preparation for the benchmark configs.
+
+Restates the reference's Python harness (run/generate_setup.py:23-53 --
+acquisition files and the smoothed-slowness starting model;
+run/generate_input_data.py:22-46 -- direct-wave mute) and adds the seeded
+model families the benchmark configurations name (flat / dipping layers,
+Gaussian lenses, salt-like polygons).  Every model is [nx, nz] float32 with z
+fastest, the reference's file layout.
+"""
+import dataclasses
+import os
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Survey:
+    name: str
+    nz: int
+    nx: int
+    dh: float
+    nt: int
+    dt: float
+    fmax: float                   # the reference's max_frequency = 3 x Ricker peak
+    nb: int                       # pad cells per side (4-sided)
+    nsrc: int
+    nrec: int
+    zsrc: float
+    zrec: float
+    varVp_rel: float = 0.4        # random boundary: +-40 % ramp
+    varVp: float = 0.0
+    abc_nb: int = 100
+    abc_factor: float = 0.0035
+    seed: int = 2024
+
+    def acquisition(self):
+        """Sources evenly spaced along the surface, one receiver per column
+        band (run/generate_setup.py:31-36 spaces them with linspace)."""
+        xmax = (self.nx - 1) * self.dh
+        if self.nsrc == 1:
+            xs = np.array([xmax / 2.0])
+        else:
+            xs = np.linspace(0.02 * xmax, 0.98 * xmax, self.nsrc)
+        xr = np.linspace(0.0, xmax, self.nrec)
+        return (xs.astype(np.float32), np.full(self.nsrc, self.zsrc, np.float32),
+                xr.astype(np.float32), np.full(self.nrec, self.zrec, np.float32))
+
+
+# The five benchmark configurations (sizes as stated; peaks -> fmax = 3 x peak).
+CONFIG1 = Survey("forward-1shot", 121, 401, 10.0, 2001, 1e-3, 45.0, 100, 1, 401, 20.0, 10.0)
+CONFIG2 = Survey("rtm-64shot", 301, 1001, 10.0, 4001, 8e-4, 30.0, 150, 64, 1001, 20.0, 10.0)
+CONFIG3 = Survey("fwi-128shot", 301, 1001, 10.0, 4001, 8e-4, 24.0, 150, 128, 1001, 20.0, 10.0)
+CONFIG5 = Survey("fwi-256shot", 401, 1601, 10.0, 5001, 6e-4, 36.0, 200, 256, 1601, 20.0, 10.0)
+
+
+def gaussian_smooth(a, sigma, truncate=4.0):
+    """Separable Gaussian filter with reflected edges, the semantics of
+    scipy.ndimage.gaussian_filter(mode='reflect', truncate=4.0) that
+    run/generate_setup.py:49 uses."""
+    a = np.asarray(a, np.float64)
+    r = int(truncate * sigma + 0.5)
+    x = np.arange(-r, r + 1, dtype=np.float64)
+    k = np.exp(-0.5 * (x / sigma) ** 2)
+    k /= k.sum()
+    out = a
+    for axis in range(a.ndim):
+        pad = [(0, 0)] * a.ndim
+        pad[axis] = (r, r)
+        p = np.pad(out, pad, mode="symmetric")
+        acc = np.zeros_like(out)
+        for i, w in enumerate(k):
+            sl = [slice(None)] * a.ndim
+            sl[axis] = slice(i, i + out.shape[axis])
+            acc += w * p[tuple(sl)]
+        out = acc
+    return out
+
+
+def smooth_slowness(v, sigma, keep_rows=0):
+    """1 / gaussian(1/v, sigma) with the top `keep_rows` rows copied from v
+    (run/generate_setup.py:49-51)."""
+    out = 1.0 / gaussian_smooth(1.0 / np.asarray(v, np.float64), sigma)
+    if keep_rows > 0:
+        out[:, :keep_rows] = v[:, :keep_rows]
+    return out.astype(np.float32)
+
+
+def flat_layers(nz, nx, speeds):
+    v = np.zeros((nx, nz), np.float32)
+    n = len(speeds)
+    for k, s in enumerate(speeds):
+        v[:, k * nz // n:] = s
+    return v
+
+
+def gaussian_lens(v, zc, xc, sigma, amp):
+    nx, nz = v.shape
+    z = np.arange(nz)[None, :]
+    x = np.arange(nx)[:, None]
+    v += (amp * np.exp(-0.5 * ((z - zc) ** 2 + (x - xc) ** 2) / sigma ** 2)).astype(np.float32)
+    return v
+
+
+def dipping_layers(nz, nx, nlayers, vmin, vmax, seed):
+    """nlayers seeded dipping interfaces, speeds increasing with depth."""
+    rng = np.random.default_rng(seed)
+    speeds = np.linspace(vmin, vmax, nlayers)
+    base = np.sort(rng.uniform(0.1, 0.95, nlayers - 1)) * nz
+    dips = rng.uniform(-0.12, 0.12, nlayers - 1)
+    x = np.arange(nx)
+    v = np.full((nx, nz), speeds[0], np.float32)
+    z = np.arange(nz)[None, :]
+    for k in range(nlayers - 1):
+        depth = base[k] + dips[k] * (x - nx / 2.0)
+        v[z >= depth[:, None]] = speeds[k + 1]
+    return v
+
+
+def salt_polygon(v, zc, xc, hz, hx, speed, seed, nvert=12):
+    """Seeded random star-shaped polygon of half-sizes (hz, hx) cells."""
+    rng = np.random.default_rng(seed)
+    ang = np.sort(rng.uniform(0, 2 * np.pi, nvert))
+    rad = rng.uniform(0.6, 1.0, nvert)
+    nx, nz = v.shape
+    z = np.arange(nz)[None, :] - zc
+    x = np.arange(nx)[:, None] - xc
+    theta = np.arctan2(z / hz, x / hx) % (2 * np.pi)
+    r = np.sqrt((z / hz) ** 2 + (x / hx) ** 2)
+    # boundary radius by linear interpolation between vertices
+    a2 = np.concatenate([ang - 2 * np.pi, ang, ang + 2 * np.pi])
+    r2 = np.concatenate([rad, rad, rad])
+    rb = np.interp(theta, a2, r2)
+    v[r <= rb] = speed
+    return v
+
+
+def config1_model(seed=1):
+    v = flat_layers(121, 401, [1500, 2000, 2500, 3000])
+    rng = np.random.default_rng(seed)
+    return gaussian_lens(v, rng.uniform(40, 90), rng.uniform(120, 280), 15.0, 250.0)
+
+
+def config2_models(seed=2):
+    true = dipping_layers(301, 1001, 6, 1500.0, 4000.0, seed)
+    return true, smooth_slowness(true, 20.0)
+
+
+def config3_models(seed=3):
+    rng = np.random.default_rng(seed)
+    v = flat_layers(301, 1001, list(np.linspace(1500, 3500, 6)))
+    for k in range(3):
+        gaussian_lens(v, rng.uniform(80, 250), rng.uniform(100, 900), 25.0,
+                      300.0 if k % 2 == 0 else -300.0)
+    salt_polygon(v, rng.uniform(150, 220), rng.uniform(350, 650), 30, 100, 4500.0, seed + 7)
+    return v, smooth_slowness(v, 30.0)
+
+
+def mute_direct_wave(data, xs, zs, xr, zr, dt, velocity, delay, gstd=30.0, normalise=True):
+    """Direct-wave mute of one shot (run/generate_input_data.py:22-46):
+    data[nrec, nt] (t fastest) is divided by its max |.| and the samples
+    before t_r = (distance/velocity + delay)/dt get the Gaussian taper
+    exp(-0.5 ((t - t_r)/gstd)^2)."""
+    d = np.array(data, np.float32, copy=True)
+    if normalise:
+        m = np.max(np.abs(d))
+        if m > 0:
+            d *= 1.0 / m
+    nt = d.shape[1]
+    t = np.arange(nt)
+    dist = np.sqrt((xr - xs) ** 2 + (zr - zs) ** 2)
+    tid = ((dist / velocity + delay) / dt).astype(int)
+    for r in range(d.shape[0]):
+        k = min(max(tid[r], 0), nt)
+        d[r, :k] *= np.exp(-0.5 * ((t[:k] - tid[r]) / gstd) ** 2)
+    return d
+
+
+def write_parameters(path, s, model_file, sps, rps, folders, max_iteration=5, depth_mask=0.0,
+                     mig_prefix="", inv_prefix="", extra=None):
+    """parameters.txt in the reference's key = value format
+    (run/parameters.txt keys; Appendix of the key list in the docs)."""
+    lines = [
+        f"x_samples = {s.nx}", f"z_samples = {s.nz}", f"model_spacing = {s.dh}",
+        f"model_file = {model_file}", f"SPS = {sps}", f"RPS = {rps}",
+        f"time_samples = {s.nt}", f"time_spacing = {s.dt}", f"max_frequency = {s.fmax}",
+        f"abc_length = {(s.abc_nb - 1) * s.dh + 0.5 * s.dh}", f"abc_factor = {s.abc_factor}",
+        f"mod_output_folder = {folders['data']}",
+        f"max_iteration = {max_iteration}", f"depth_mask = {depth_mask}",
+        f"inv_rbc_ratio = 200", f"inv_rbc_varVp = {s.varVp}",
+        f"inv_rbc_varVp_rel = {s.varVp_rel}",
+        f"inv_rbc_length = {(s.nb - 1) * s.dh + 0.5 * s.dh}",
+        f"inv_input_folder = {folders['data']}", f"inv_input_prefix = {inv_prefix}",
+        f"inv_output_folder = {folders['models']}", f"inv_residuo_folder = {folders['residuo']}",
+        f"mig_rbc_ratio = 200", f"mig_rbc_varVp = {s.varVp}",
+        f"mig_rbc_varVp_rel = {s.varVp_rel}",
+        f"mig_rbc_length = {(s.nb - 1) * s.dh + 0.5 * s.dh}",
+        f"mig_input_folder = {folders['data']}", f"mig_input_prefix = {mig_prefix}",
+        f"mig_output_folder = {folders['seismic']}", f"rbc_seed = {s.seed}",
+    ]
+    for k, v in (extra or {}).items():
+        lines.append(f"{k} = {v}")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_geometry(sps_path, rps_path, xs, zs, xr, zr):
+    """SPS / RPS text files, "x,z" per line with %.2f (generate_setup.py:35-36)."""
+    np.savetxt(sps_path, np.stack([xs, zs], 1), fmt="%.2f", delimiter=",")
+    np.savetxt(rps_path, np.stack([xr, zr], 1), fmt="%.2f", delimiter=",")
+
+
+def ensure_dirs(root):
+    folders = {k: os.path.join(root, k) + "/" for k in ("data", "models", "residuo", "seismic")}
+    for p in folders.values():
+        os.makedirs(p, exist_ok=True)
+    return folders

@@ -1,0 +1,796 @@
+// engine.hip -- host side of the C ABI declared in include/fwi2d.h.
+//
+// A context owns one HIP stream and every device buffer of a survey.  Shots
+// run in batches of up to max_batch resident slots; each pass over a batch is
+// one launch per time step covering every slot (all shots of the batch in
+// one grid), and the T-step loop is captured once into a hipGraph per
+// (pass kind, batch size) and replayed.  The first global shot of the batch
+// reaches the kernels through a device scalar, so one captured graph serves
+// every batch of the same size.
+#include "fwi2d.h"
+#include "kernels.h"
+#include "dsp.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(FWI_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                            \
+    } while (0)
+
+#define TRY(expr)                    \
+    do {                             \
+        int r_ = (expr);             \
+        if (r_ != FWI_OK) return r_; \
+    } while (0)
+
+constexpr float kCflLimit = 0.5546f;   // 2-D leapfrog limit of the 8th-order stencil
+
+struct GridBuf {
+    fwi::Grid g{};
+    bool on = false;
+    float *vpad = nullptr;       // padded base model (slot layout)
+    int *src_iz = nullptr, *src_ix = nullptr;
+    int *col_ptr = nullptr, *rec_iz = nullptr, *rec_id = nullptr;
+};
+
+enum PassKind { PASS_FWD_ABC_REC = 0, PASS_FWD_RBC = 1, PASS_FWD_RBC_REC = 2,
+                PASS_REV_IMAGE = 3, PASS_REV_GRAD = 4, PASS_FWD_ABC = 5 };
+
+}  // namespace
+
+struct fwi_ctx {
+    fwi_config cfg{};
+    hipStream_t stream = nullptr;
+    GridBuf grid[2];
+    long fstride_max = 0;
+    float kc = 0.f;            // dt^2 / dh^2
+    float idh2 = 0.f;
+
+    float *d_vp = nullptr;     // interior model
+    float *d_cabc = nullptr;   // shared coefficient, abc grid
+    float *d_crb = nullptr;    // per-slot coefficient, rbc grid
+    float *d_b1d = nullptr;
+    float *d_wav = nullptr;
+    int *d_shot0 = nullptr;
+    float *d_f[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *d_acc_slot = nullptr, *d_ill_slot = nullptr;
+    float *d_image = nullptr, *d_grad = nullptr, *d_illum = nullptr;
+    float *d_obs = nullptr;    // [nsrc][nrec][nt]
+    float *d_seis = nullptr;   // [max_batch][nrec][nt]
+    double *d_partials = nullptr, *d_sum = nullptr;
+    float *d_tmp = nullptr;    // nz*nx scratch for crops
+    int npartials = 1024;
+
+    int nsrc = 0, nrec = 0, tlag = 0;
+    bool have_wavelet = false, have_geometry = false, have_model = false;
+    float vmin = 0.f, vmax = 0.f;
+
+    int vz = 4, xw = 32, use_graph = 1;
+    std::map<long, hipGraphExec_t> graphs;
+
+    bool profile = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double prof_ms[2] = {0, 0};
+    int64_t prof_launches[2] = {0, 0};
+    double prof_cells[2] = {0, 0};
+};
+
+namespace {
+
+int T_of(const fwi_ctx *c) { return c->cfg.nt + c->tlag; }
+
+fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g)
+{
+    fwi::Tiling t;
+    t.vz = c->vz;
+    t.xw = c->xw;
+    t.nzseg = (g.ldz + 64 * t.vz - 1) / (64 * t.vz);
+    t.nxchunk = (g.nxx - 8 + t.xw - 1) / t.xw;
+    return t;
+}
+
+void drop_graphs(fwi_ctx *c)
+{
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+}
+
+template <typename T>
+int dalloc(T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess)
+        return fail(FWI_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                    hipGetErrorString(e));
+    e = hipMemset(*p, 0, n * sizeof(T));
+    if (e != hipSuccess) return fail(FWI_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    return FWI_OK;
+}
+
+fwi::FwdStep fwd_args(fwi_ctx *c, int kind, bool record)
+{
+    const GridBuf &gb = c->grid[kind];
+    fwi::FwdStep a{};
+    a.g = gb.g;
+    a.fa = c->d_f[0];
+    a.fb = c->d_f[1];
+    a.c = (kind == FWI_BOUNDARY_ABC) ? c->d_cabc : c->d_crb;
+    a.cstride = (kind == FWI_BOUNDARY_ABC) ? 0 : gb.g.fstride;
+    a.b1d = c->d_b1d;
+    a.abc = (kind == FWI_BOUNDARY_ABC) ? 1 : 0;
+    a.src_iz = gb.src_iz;
+    a.src_ix = gb.src_ix;
+    a.shot0 = c->d_shot0;
+    a.wav = c->d_wav;
+    a.nt = c->cfg.nt;
+    a.tlag = c->tlag;
+    a.T = T_of(c);
+    a.rec = {gb.col_ptr, gb.rec_iz, gb.rec_id, c->nrec};
+    a.seis = record ? c->d_seis : nullptr;
+    return a;
+}
+
+// Reverse-pass arguments: the forward pass left u_T and u_{T-1} in d_f[0..1].
+fwi::RevStep rev_args(fwi_ctx *c, int mode)
+{
+    const GridBuf &gb = c->grid[FWI_BOUNDARY_RBC];
+    const int T = T_of(c);
+    fwi::RevStep a{};
+    a.g = gb.g;
+    // forward step t writes into (t&1 ? fa : fb): u_T sits in ((T-1)&1 ? f0 : f1)
+    float *uT = ((T - 1) & 1) ? c->d_f[0] : c->d_f[1];
+    float *uTm1 = ((T - 1) & 1) ? c->d_f[1] : c->d_f[0];
+    a.sa = uT;      // reverse step 0: cur = sb = u_{T-1}, old = sa = u_T
+    a.sb = uTm1;
+    a.ra = c->d_f[2];
+    a.rb = c->d_f[3];
+    a.c = c->d_crb;
+    a.cstride = gb.g.fstride;
+    a.shot0 = c->d_shot0;
+    a.data = (mode == 0) ? c->d_obs : c->d_seis;
+    a.data_by_shot = (mode == 0) ? 1 : 0;
+    a.idh2 = c->idh2;
+    a.nt = c->cfg.nt;
+    a.T = T;
+    a.rec = {gb.col_ptr, gb.rec_iz, gb.rec_id, c->nrec};
+    a.acc = c->d_acc_slot;
+    a.ill = c->d_ill_slot;
+    a.mode = mode;
+    return a;
+}
+
+// Enqueues the T step launches of one pass (or replays its captured graph).
+int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fwi::RevStep *ra)
+{
+    const int T = T_of(c);
+    const fwi::Grid &g = fa ? fa->g : ra->g;
+    const fwi::Tiling tl = tiling_of(c, g);
+    auto enqueue = [&]() -> hipError_t {
+        for (int t = 0; t < T; ++t) {
+            hipError_t e = fa ? fwi::launch_fwd_step(*fa, tl, nslots, t, c->stream)
+                              : fwi::launch_rev_step(*ra, tl, nslots, t, c->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    const int kind = fa ? 0 : 1;
+    if (c->profile) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    if (c->use_graph) {
+        const long key = ((long)pass << 40) | ((long)nslots << 20) | ((long)c->vz << 16) | c->xw;
+        auto it = c->graphs.find(key);
+        if (it == c->graphs.end()) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = enqueue();
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if (e != hipSuccess) return fail(FWI_ERR_HIP, "kernel launch during capture: %s",
+                                             hipGetErrorString(e));
+            if (e2 != hipSuccess) return fail(FWI_ERR_HIP, "hipStreamEndCapture: %s",
+                                              hipGetErrorString(e2));
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(FWI_ERR_HIP, "hipGraphInstantiate: %s",
+                                             hipGetErrorString(e));
+            it = c->graphs.emplace(key, exec).first;
+        }
+        HIP_TRY(hipGraphLaunch(it->second, c->stream));
+    } else {
+        hipError_t e = enqueue();
+        if (e != hipSuccess) return fail(FWI_ERR_HIP, "step launch: %s", hipGetErrorString(e));
+    }
+    if (c->profile) {
+        HIP_TRY(hipEventRecord(c->ev1, c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->prof_ms[kind] += ms;
+        c->prof_launches[kind] += T;
+        c->prof_cells[kind] += (double)nslots * g.nzz * g.nxx * T * (kind ? 2.0 : 1.0);
+    }
+    return FWI_OK;
+}
+
+int check_ready(fwi_ctx *c, bool need_obs = false)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    if (!c->have_wavelet) return fail(FWI_ERR_STATE, "wavelet not set (fwi_set_wavelet)");
+    if (!c->have_geometry) return fail(FWI_ERR_STATE, "geometry not set (fwi_set_geometry)");
+    if (!c->have_model) return fail(FWI_ERR_STATE, "model not set (fwi_set_model)");
+    (void)need_obs;
+    return FWI_OK;
+}
+
+int check_shots(fwi_ctx *c, int shot0, int nshots)
+{
+    if (shot0 < 0 || nshots < 0 || shot0 + nshots > c->nsrc)
+        return fail(FWI_ERR_ARG, "shot range [%d, %d) outside [0, %d)", shot0, shot0 + nshots,
+                    c->nsrc);
+    return FWI_OK;
+}
+
+int set_shot0(fwi_ctx *c, int shot0)
+{
+    HIP_TRY(fwi::launch_set_int(c->d_shot0, shot0, c->stream));
+    return FWI_OK;
+}
+
+// Forward pass of one batch; fields end in d_f[0..1] (u_T, u_{T-1}).
+int forward_batch(fwi_ctx *c, int kind, int shot0, int nslots, bool record, uint32_t iteration)
+{
+    GridBuf &gb = c->grid[kind];
+    TRY(set_shot0(c, shot0));
+    const long nf = (long)nslots * gb.g.fstride;
+    if (kind == FWI_BOUNDARY_RBC) {
+        const fwi_config &k = c->cfg;
+        float vlo = c->vmin * (1.0f - k.rbc_varVp_rel) - k.rbc_varVp;
+        float vhi = c->vmax * (1.0f + k.rbc_varVp_rel) + k.rbc_varVp;
+        const float vcfl = 0.999f * kCflLimit * k.dh / k.dt;
+        if (vhi > vcfl) vhi = vcfl;
+        HIP_TRY(fwi::launch_rb(gb.vpad, gb.g, nslots, c->d_shot0, iteration, k.rbc_seed,
+                               k.rbc_varVp, k.rbc_varVp_rel, vlo, vhi, c->kc, c->d_crb, nullptr,
+                               c->stream));
+    }
+    HIP_TRY(fwi::launch_fill(c->d_f[0], nf, 0.f, c->stream));
+    HIP_TRY(fwi::launch_fill(c->d_f[1], nf, 0.f, c->stream));
+    fwi::FwdStep a = fwd_args(c, kind, record);
+    HIP_TRY(fwi::launch_fwd_init(a, nslots, c->stream));
+    if (record)
+        HIP_TRY(fwi::launch_zero_sample0(c->d_seis, c->nrec, c->cfg.nt, nslots, c->stream));
+    int pass = (kind == FWI_BOUNDARY_ABC) ? (record ? PASS_FWD_ABC_REC : PASS_FWD_ABC)
+                                          : (record ? PASS_FWD_RBC_REC : PASS_FWD_RBC);
+    return run_steps(c, pass, nslots, &a, nullptr);
+}
+
+// Reverse pass of one batch after an rbc forward pass; mode 0 image, 1 gradient.
+int reverse_batch(fwi_ctx *c, int shot0, int nslots, int mode)
+{
+    GridBuf &gb = c->grid[FWI_BOUNDARY_RBC];
+    TRY(set_shot0(c, shot0));
+    const long nf = (long)nslots * gb.g.fstride;
+    HIP_TRY(fwi::launch_fill(c->d_f[2], nf, 0.f, c->stream));
+    HIP_TRY(fwi::launch_fill(c->d_f[3], nf, 0.f, c->stream));
+    fwi::RevStep a = rev_args(c, mode);
+    HIP_TRY(fwi::launch_rev_init(a, nslots, c->stream));
+    TRY(run_steps(c, mode == 0 ? PASS_REV_IMAGE : PASS_REV_GRAD, nslots, nullptr, &a));
+    float *dst = (mode == 0) ? c->d_image : c->d_grad;
+    HIP_TRY(fwi::launch_reduce_slots(c->d_acc_slot, gb.g.fstride, gb.g.fstride, nslots, dst,
+                                     c->stream));
+    HIP_TRY(fwi::launch_reduce_slots(c->d_ill_slot, gb.g.fstride, gb.g.fstride, nslots,
+                                     c->d_illum, c->stream));
+    return FWI_OK;
+}
+
+// residual = obs - cal in d_seis (after an abc recorded pass); adds sum res^2.
+int residual_batch(fwi_ctx *c, int shot0, int nslots, double *sum)
+{
+    const long per_shot = (long)c->nrec * c->cfg.nt;
+    HIP_TRY(fwi::launch_residual(c->d_obs, c->d_seis, c->d_shot0, per_shot, nslots,
+                                 c->d_partials, c->npartials, c->stream));
+    HIP_TRY(fwi::launch_sum_partials(c->d_partials, c->npartials, c->d_sum, c->stream));
+    double h = 0.0;
+    HIP_TRY(hipMemcpyAsync(&h, c->d_sum, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *sum += h;
+    (void)shot0;
+    return FWI_OK;
+}
+
+int setup_grid(fwi_ctx *c, int kind, int nb)
+{
+    GridBuf &gb = c->grid[kind];
+    gb.on = nb > 0;
+    if (!gb.on) return FWI_OK;
+    if (nb < 4) return fail(FWI_ERR_ARG, "pad width %d < 4 cells (stencil rim)", nb);
+    gb.g.nb = nb;
+    gb.g.nzz = c->cfg.nz + 2 * nb;
+    gb.g.nxx = c->cfg.nx + 2 * nb;
+    gb.g.ldz = (gb.g.nzz + 3) & ~3;
+    long fs = (long)gb.g.ldz * gb.g.nxx;
+    fs = (fs + 63) & ~63L;     // 256 B aligned slots
+    gb.g.fstride = fs;
+    c->fstride_max = std::max(c->fstride_max, fs);
+    return FWI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fwi_last_error(void) { return g_last_error.c_str(); }
+
+const char *fwi_version(void) { return "fwi2d-mi355x 0.1 (gfx950)"; }
+
+int fwi_create(const fwi_config *cfg, fwi_ctx **out)
+{
+    if (!cfg || !out) return fail(FWI_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->nz < 1 || cfg->nx < 1 || !(cfg->dh > 0.f) || cfg->nt < 2 || !(cfg->dt > 0.f))
+        return fail(FWI_ERR_ARG, "bad grid: nz=%d nx=%d dh=%g nt=%d dt=%g", cfg->nz, cfg->nx,
+                    cfg->dh, cfg->nt, cfg->dt);
+    if (cfg->max_batch < 1) return fail(FWI_ERR_ARG, "max_batch must be >= 1");
+    if (cfg->abc_nb <= 0 && cfg->rbc_nb <= 0)
+        return fail(FWI_ERR_ARG, "neither an abc nor an rbc grid is configured");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(FWI_ERR_HIP, "no HIP device available");
+
+    fwi_ctx *c = new fwi_ctx();
+    c->cfg = *cfg;
+    c->kc = (float)((double)cfg->dt * cfg->dt / ((double)cfg->dh * cfg->dh));
+    c->idh2 = (float)(1.0 / ((double)cfg->dh * cfg->dh));
+    int r = setup_grid(c, FWI_BOUNDARY_ABC, cfg->abc_nb);
+    if (r == FWI_OK) r = setup_grid(c, FWI_BOUNDARY_RBC, cfg->rbc_nb);
+    if (r != FWI_OK) { delete c; return r; }
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(FWI_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    (void)hipEventCreate(&c->ev0);
+    (void)hipEventCreate(&c->ev1);
+
+    const long S = cfg->max_batch;
+    const long n_int = (long)cfg->nz * cfg->nx;
+    r = FWI_OK;
+    if (r == FWI_OK) r = dalloc(&c->d_vp, n_int);
+    if (r == FWI_OK) r = dalloc(&c->d_tmp, n_int);
+    if (r == FWI_OK) r = dalloc(&c->d_shot0, 1);
+    if (r == FWI_OK) r = dalloc(&c->d_wav, cfg->nt);
+    for (int k = 0; k < 4 && r == FWI_OK; ++k) r = dalloc(&c->d_f[k], S * c->fstride_max);
+    for (int kind = 0; kind < 2 && r == FWI_OK; ++kind)
+        if (c->grid[kind].on) r = dalloc(&c->grid[kind].vpad, c->grid[kind].g.fstride);
+    if (r == FWI_OK && c->grid[0].on) {
+        std::vector<float> b1d;
+        fwi::cerjan_table(cfg->abc_nb, cfg->abc_factor, b1d);
+        r = dalloc(&c->d_b1d, b1d.size());
+        if (r == FWI_OK)
+            r = hipMemcpy(c->d_b1d, b1d.data(), b1d.size() * 4, hipMemcpyHostToDevice) == hipSuccess
+                    ? FWI_OK : fail(FWI_ERR_HIP, "b1d upload");
+        if (r == FWI_OK) r = dalloc(&c->d_cabc, c->grid[0].g.fstride);
+    }
+    if (r == FWI_OK && c->grid[1].on) {
+        const long fs = c->grid[1].g.fstride;
+        r = dalloc(&c->d_crb, S * fs);
+        if (r == FWI_OK) r = dalloc(&c->d_acc_slot, S * fs);
+        if (r == FWI_OK) r = dalloc(&c->d_ill_slot, S * fs);
+        if (r == FWI_OK) r = dalloc(&c->d_image, fs);
+        if (r == FWI_OK) r = dalloc(&c->d_grad, fs);
+        if (r == FWI_OK) r = dalloc(&c->d_illum, fs);
+    }
+    if (r == FWI_OK) r = dalloc(&c->d_partials, c->npartials);
+    if (r == FWI_OK) r = dalloc(&c->d_sum, 1);
+    if (r != FWI_OK) { fwi_destroy(c); return r; }
+    // default launch shape: 4 rows per lane on tall grids, 32-column chunks
+    c->vz = (c->fstride_max / std::max(1, cfg->nx) > 192) ? 4 : 2;
+    c->xw = 32;
+    *out = c;
+    return FWI_OK;
+}
+
+int fwi_destroy(fwi_ctx *c)
+{
+    if (!c) return FWI_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    void *bufs[] = {c->d_vp, c->d_cabc, c->d_crb, c->d_b1d, c->d_wav, c->d_shot0, c->d_f[0],
+                    c->d_f[1], c->d_f[2], c->d_f[3], c->d_acc_slot, c->d_ill_slot, c->d_image,
+                    c->d_grad, c->d_illum, c->d_obs, c->d_seis, c->d_partials, c->d_sum, c->d_tmp};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    for (auto &gb : c->grid) {
+        void *gp[] = {gb.vpad, gb.src_iz, gb.src_ix, gb.col_ptr, gb.rec_iz, gb.rec_id};
+        for (void *p : gp)
+            if (p) (void)hipFree(p);
+    }
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FWI_OK;
+}
+
+int fwi_ricker_wavelet(int32_t nt, float dt, float fmax, float *w_out, int32_t *tlag_out)
+{
+    if (nt < 2 || !(dt > 0.f) || !(fmax > 0.f) || !w_out)
+        return fail(FWI_ERR_ARG, "bad wavelet arguments");
+    std::vector<double> w;
+    const int tlag = fwi::ricker_half_derivative(nt, dt, fmax, w);
+    for (int k = 0; k < nt; ++k) w_out[k] = (float)w[k];
+    if (tlag_out) *tlag_out = tlag;
+    return FWI_OK;
+}
+
+int fwi_set_wavelet(fwi_ctx *c, const float *w, int32_t tlag)
+{
+    if (!c || !w) return fail(FWI_ERR_ARG, "null argument");
+    if (tlag < 1) return fail(FWI_ERR_ARG, "tlag must be >= 1");
+    std::vector<float> h(c->cfg.nt);
+    for (int k = 0; k < c->cfg.nt; ++k) h[k] = c->idh2 * w[k];   // idh2 * w[t] (modeling.cu:421)
+    HIP_TRY(hipMemcpy(c->d_wav, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    if (tlag != c->tlag) drop_graphs(c);
+    c->tlag = tlag;
+    c->have_wavelet = true;
+    return FWI_OK;
+}
+
+int fwi_set_geometry(fwi_ctx *c, int32_t nsrc, const float *xsrc, const float *zsrc,
+                     int32_t nrec, const float *xrec, const float *zrec)
+{
+    if (!c || nsrc < 1 || nrec < 1 || !xsrc || !zsrc || !xrec || !zrec)
+        return fail(FWI_ERR_ARG, "bad geometry arguments");
+    const float dh = c->cfg.dh;
+    for (int kind = 0; kind < 2; ++kind) {
+        GridBuf &gb = c->grid[kind];
+        if (!gb.on) continue;
+        const int nb = gb.g.nb;
+        auto index = [&](float x) { return (int)((x + 0.5f * dh) / dh) + nb; };
+        std::vector<int> siz(nsrc), six(nsrc);
+        for (int s = 0; s < nsrc; ++s) {
+            siz[s] = index(zsrc[s]);
+            six[s] = index(xsrc[s]);
+            if (siz[s] < 4 || siz[s] >= gb.g.nzz - 4 || six[s] < 4 || six[s] >= gb.g.nxx - 4)
+                return fail(FWI_ERR_ARG, "source %d at (x=%g, z=%g) m falls outside the grid",
+                            s, xsrc[s], zsrc[s]);
+        }
+        std::vector<int> riz(nrec), rix(nrec), order(nrec);
+        for (int r = 0; r < nrec; ++r) {
+            riz[r] = index(zrec[r]);
+            rix[r] = index(xrec[r]);
+            if (riz[r] < 4 || riz[r] >= gb.g.nzz - 4 || rix[r] < 4 || rix[r] >= gb.g.nxx - 4)
+                return fail(FWI_ERR_ARG, "receiver %d at (x=%g, z=%g) m falls outside the grid",
+                            r, xrec[r], zrec[r]);
+            order[r] = r;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return rix[a] != rix[b] ? rix[a] < rix[b] : riz[a] < riz[b];
+        });
+        std::vector<int> col_ptr(gb.g.nxx + 1, 0), s_iz(nrec), s_id(nrec);
+        for (int r = 0; r < nrec; ++r) col_ptr[rix[r] + 1]++;
+        for (int j = 0; j < gb.g.nxx; ++j) col_ptr[j + 1] += col_ptr[j];
+        for (int k = 0; k < nrec; ++k) {
+            s_iz[k] = riz[order[k]];
+            s_id[k] = order[k];
+        }
+        void *old[] = {gb.src_iz, gb.src_ix, gb.col_ptr, gb.rec_iz, gb.rec_id};
+        for (void *p : old)
+            if (p) (void)hipFree(p);
+        gb.src_iz = gb.src_ix = gb.col_ptr = gb.rec_iz = gb.rec_id = nullptr;
+        TRY(dalloc(&gb.src_iz, nsrc));
+        TRY(dalloc(&gb.src_ix, nsrc));
+        TRY(dalloc(&gb.col_ptr, gb.g.nxx + 1));
+        TRY(dalloc(&gb.rec_iz, nrec));
+        TRY(dalloc(&gb.rec_id, nrec));
+        HIP_TRY(hipMemcpy(gb.src_iz, siz.data(), nsrc * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gb.src_ix, six.data(), nsrc * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gb.col_ptr, col_ptr.data(), col_ptr.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gb.rec_iz, s_iz.data(), nrec * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gb.rec_id, s_id.data(), nrec * 4, hipMemcpyHostToDevice));
+    }
+    drop_graphs(c);
+    if (c->d_obs) { (void)hipFree(c->d_obs); c->d_obs = nullptr; }
+    if (c->d_seis) { (void)hipFree(c->d_seis); c->d_seis = nullptr; }
+    const long per_shot = (long)nrec * c->cfg.nt;
+    TRY(dalloc(&c->d_obs, (long)nsrc * per_shot));
+    TRY(dalloc(&c->d_seis, (long)c->cfg.max_batch * per_shot));
+    c->nsrc = nsrc;
+    c->nrec = nrec;
+    c->have_geometry = true;
+    return FWI_OK;
+}
+
+int fwi_set_model(fwi_ctx *c, const float *vp)
+{
+    if (!c || !vp) return fail(FWI_ERR_ARG, "null argument");
+    const long n = (long)c->cfg.nz * c->cfg.nx;
+    float vmin = 1e30f, vmax = -1e30f;
+    for (long k = 0; k < n; ++k) {
+        if (!(vp[k] > 0.f) || !std::isfinite(vp[k]))
+            return fail(FWI_ERR_NUMERIC, "velocity %g at cell %ld is not positive and finite",
+                        vp[k], k);
+        vmin = std::min(vmin, vp[k]);
+        vmax = std::max(vmax, vp[k]);
+    }
+    const float courant = vmax * c->cfg.dt / c->cfg.dh;
+    if (courant > kCflLimit)
+        return fail(FWI_ERR_NUMERIC, "CFL violated: vmax*dt/dh = %g > %g", courant, kCflLimit);
+    HIP_TRY(hipMemcpyAsync(c->d_vp, vp, n * 4, hipMemcpyHostToDevice, c->stream));
+    for (int kind = 0; kind < 2; ++kind) {
+        GridBuf &gb = c->grid[kind];
+        if (!gb.on) continue;
+        HIP_TRY(fwi::launch_pad_model(c->d_vp, c->cfg.nz, c->cfg.nx, gb.g, gb.vpad, c->stream));
+    }
+    if (c->grid[0].on)
+        HIP_TRY(fwi::launch_coef(c->grid[0].vpad, c->grid[0].g, c->kc, c->d_cabc, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->vmin = vmin;
+    c->vmax = vmax;
+    c->have_model = true;
+    return FWI_OK;
+}
+
+int fwi_set_observed(fwi_ctx *c, int32_t shot0, int32_t nshots, const float *data)
+{
+    if (!c || !data) return fail(FWI_ERR_ARG, "null argument");
+    if (!c->have_geometry) return fail(FWI_ERR_STATE, "geometry not set (fwi_set_geometry)");
+    TRY(check_shots(c, shot0, nshots));
+    const long per_shot = (long)c->nrec * c->cfg.nt;
+    HIP_TRY(hipMemcpy(c->d_obs + (long)shot0 * per_shot, data, (size_t)nshots * per_shot * 4,
+                      hipMemcpyHostToDevice));
+    return FWI_OK;
+}
+
+int fwi_modeling(fwi_ctx *c, int32_t boundary, int32_t shot0, int32_t nshots, float *seis_out)
+{
+    TRY(check_ready(c));
+    TRY(check_shots(c, shot0, nshots));
+    if (boundary != FWI_BOUNDARY_ABC && boundary != FWI_BOUNDARY_RBC)
+        return fail(FWI_ERR_ARG, "unknown boundary %d", boundary);
+    if (!c->grid[boundary].on) return fail(FWI_ERR_ARG, "boundary %d grid not configured", boundary);
+    const long per_shot = (long)c->nrec * c->cfg.nt;
+    for (int b0 = shot0; b0 < shot0 + nshots; b0 += c->cfg.max_batch) {
+        const int ns = std::min(c->cfg.max_batch, shot0 + nshots - b0);
+        TRY(forward_batch(c, boundary, b0, ns, seis_out != nullptr, 0));
+        if (seis_out)
+            HIP_TRY(hipMemcpyAsync(seis_out + (long)(b0 - shot0) * per_shot, c->d_seis,
+                                   (size_t)ns * per_shot * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FWI_OK;
+}
+
+int fwi_migration(fwi_ctx *c, int32_t shot0, int32_t nshots)
+{
+    TRY(check_ready(c));
+    TRY(check_shots(c, shot0, nshots));
+    if (!c->grid[1].on) return fail(FWI_ERR_ARG, "random-boundary grid not configured");
+    for (int b0 = shot0; b0 < shot0 + nshots; b0 += c->cfg.max_batch) {
+        const int ns = std::min(c->cfg.max_batch, shot0 + nshots - b0);
+        TRY(forward_batch(c, FWI_BOUNDARY_RBC, b0, ns, false, 0));
+        TRY(reverse_batch(c, b0, ns, 0));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FWI_OK;
+}
+
+int fwi_gradient(fwi_ctx *c, int32_t shot0, int32_t nshots, uint32_t iteration, double *sum_res2)
+{
+    TRY(check_ready(c));
+    TRY(check_shots(c, shot0, nshots));
+    if (!c->grid[0].on || !c->grid[1].on)
+        return fail(FWI_ERR_ARG, "the gradient needs both the abc and the rbc grid");
+    double acc = 0.0;
+    for (int b0 = shot0; b0 < shot0 + nshots; b0 += c->cfg.max_batch) {
+        const int ns = std::min(c->cfg.max_batch, shot0 + nshots - b0);
+        TRY(forward_batch(c, FWI_BOUNDARY_ABC, b0, ns, true, 0));
+        TRY(residual_batch(c, b0, ns, &acc));
+        TRY(forward_batch(c, FWI_BOUNDARY_RBC, b0, ns, false, iteration));
+        TRY(reverse_batch(c, b0, ns, 1));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (sum_res2) *sum_res2 += acc;
+    return FWI_OK;
+}
+
+int fwi_misfit(fwi_ctx *c, int32_t shot0, int32_t nshots, double *sum_res2)
+{
+    TRY(check_ready(c));
+    TRY(check_shots(c, shot0, nshots));
+    if (!c->grid[0].on) return fail(FWI_ERR_ARG, "the misfit needs the abc grid");
+    double acc = 0.0;
+    for (int b0 = shot0; b0 < shot0 + nshots; b0 += c->cfg.max_batch) {
+        const int ns = std::min(c->cfg.max_batch, shot0 + nshots - b0);
+        TRY(forward_batch(c, FWI_BOUNDARY_ABC, b0, ns, true, 0));
+        TRY(residual_batch(c, b0, ns, &acc));
+    }
+    if (sum_res2) *sum_res2 += acc;
+    return FWI_OK;
+}
+
+int fwi_reset(fwi_ctx *c, int32_t what)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    if (!c->grid[1].on) return FWI_OK;
+    const long fs = c->grid[1].g.fstride;
+    if (what & FWI_RESET_IMAGE) HIP_TRY(fwi::launch_fill(c->d_image, fs, 0.f, c->stream));
+    if (what & FWI_RESET_GRADIENT) HIP_TRY(fwi::launch_fill(c->d_grad, fs, 0.f, c->stream));
+    if (what & FWI_RESET_ILLUM) HIP_TRY(fwi::launch_fill(c->d_illum, fs, 0.f, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FWI_OK;
+}
+
+static int crop_to_host(fwi_ctx *c, const float *src, float *dst)
+{
+    const long n = (long)c->cfg.nz * c->cfg.nx;
+    HIP_TRY(fwi::launch_crop(src, c->grid[1].g, c->cfg.nz, c->cfg.nx, c->d_tmp, c->stream));
+    HIP_TRY(hipMemcpyAsync(dst, c->d_tmp, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FWI_OK;
+}
+
+int fwi_get_image(fwi_ctx *c, float *image_out, float *raw_image, float *raw_illum)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    if (!c->grid[1].on) return fail(FWI_ERR_ARG, "random-boundary grid not configured");
+    const int nz = c->cfg.nz, nx = c->cfg.nx;
+    const long n = (long)nz * nx;
+    std::vector<float> im(n), il(n);
+    TRY(crop_to_host(c, c->d_image, im.data()));
+    TRY(crop_to_host(c, c->d_illum, il.data()));
+    if (raw_image) std::memcpy(raw_image, im.data(), n * 4);
+    if (raw_illum) std::memcpy(raw_illum, il.data(), n * 4);
+    if (image_out) {
+        std::vector<double> out;
+        fwi::second_derivative_z(im.data(), il.data(), nz, nx, c->cfg.dh, 0, out);
+        for (long k = 0; k < n; ++k) image_out[k] = (float)out[k];
+    }
+    return FWI_OK;
+}
+
+int fwi_get_gradient(fwi_ctx *c, float zmask, float *grad_out, float *raw_grad,
+                     float *raw_illum, double *gmax_out)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    if (!c->grid[1].on) return fail(FWI_ERR_ARG, "random-boundary grid not configured");
+    const int nz = c->cfg.nz, nx = c->cfg.nx;
+    const long n = (long)nz * nx;
+    std::vector<float> gr(n), il(n);
+    TRY(crop_to_host(c, c->d_grad, gr.data()));
+    TRY(crop_to_host(c, c->d_illum, il.data()));
+    // the kernels accumulate R * c * lap(S) = dt * (dt R v^2 lap(S)); undo one dt
+    const float inv_dt = 1.0f / c->cfg.dt;
+    for (long k = 0; k < n; ++k) gr[k] *= inv_dt;
+    if (raw_grad) std::memcpy(raw_grad, gr.data(), n * 4);
+    if (raw_illum) std::memcpy(raw_illum, il.data(), n * 4);
+    std::vector<double> out;
+    const int izm = (int)(zmask / c->cfg.dh);
+    fwi::second_derivative_z(gr.data(), il.data(), nz, nx, c->cfg.dh, izm, out);
+    double gmax = 0.0;
+    for (long k = 0; k < n; ++k) gmax = std::max(gmax, std::fabs(out[k]));
+    if (gmax_out) *gmax_out = gmax;
+    if (grad_out) {
+        const double s = gmax > 0.0 ? 1.0 / gmax : 0.0;
+        for (long k = 0; k < n; ++k) grad_out[k] = (float)(out[k] * s);
+    }
+    return FWI_OK;
+}
+
+int fwi_parabola_step(const double *alpha, const double *f, double *step)
+{
+    if (!alpha || !f || !step) return fail(FWI_ERR_ARG, "null argument");
+    if (!fwi::parabola_vertex(alpha, f, step))
+        return fail(FWI_ERR_NUMERIC, "degenerate parabola (collinear or non-finite misfits)");
+    return FWI_OK;
+}
+
+int fwi_rb_velocity(fwi_ctx *c, int32_t shot, uint32_t iteration, float *out)
+{
+    TRY(check_ready(c));
+    TRY(check_shots(c, shot, 1));
+    GridBuf &gb = c->grid[1];
+    if (!gb.on) return fail(FWI_ERR_ARG, "random-boundary grid not configured");
+    const long ncell = (long)gb.g.nzz * gb.g.nxx;
+    TRY(set_shot0(c, shot));
+    const fwi_config &k = c->cfg;
+    float vlo = c->vmin * (1.0f - k.rbc_varVp_rel) - k.rbc_varVp;
+    float vhi = c->vmax * (1.0f + k.rbc_varVp_rel) + k.rbc_varVp;
+    const float vcfl = 0.999f * kCflLimit * k.dh / k.dt;
+    if (vhi > vcfl) vhi = vcfl;
+    float *d_out = nullptr;
+    TRY(dalloc(&d_out, ncell));
+    hipError_t e = fwi::launch_rb(gb.vpad, gb.g, 1, c->d_shot0, iteration, k.rbc_seed,
+                                  k.rbc_varVp, k.rbc_varVp_rel, vlo, vhi, c->kc, nullptr, d_out,
+                                  c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, ncell * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(FWI_ERR_HIP, "rb velocity: %s", hipGetErrorString(e));
+    return FWI_OK;
+}
+
+int fwi_grid_shape(fwi_ctx *c, int32_t boundary, int32_t *nzz, int32_t *nxx, int32_t *nb)
+{
+    if (!c || boundary < 0 || boundary > 1) return fail(FWI_ERR_ARG, "bad argument");
+    const GridBuf &gb = c->grid[boundary];
+    if (nzz) *nzz = gb.on ? gb.g.nzz : 0;
+    if (nxx) *nxx = gb.on ? gb.g.nxx : 0;
+    if (nb) *nb = gb.on ? gb.g.nb : 0;
+    return FWI_OK;
+}
+
+int fwi_synchronize(fwi_ctx *c)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FWI_OK;
+}
+
+int fwi_set_tuning(fwi_ctx *c, int32_t vz, int32_t xw, int32_t use_graph)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    if (vz > 0) {
+        if (vz != 1 && vz != 2 && vz != 4) return fail(FWI_ERR_ARG, "vz must be 1, 2 or 4");
+        c->vz = vz;
+    }
+    if (xw > 0) c->xw = xw;
+    if (use_graph >= 0) c->use_graph = use_graph ? 1 : 0;
+    return FWI_OK;
+}
+
+int fwi_get_tuning(fwi_ctx *c, int32_t *vz, int32_t *xw, int32_t *use_graph)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    if (vz) *vz = c->vz;
+    if (xw) *xw = c->xw;
+    if (use_graph) *use_graph = c->use_graph;
+    return FWI_OK;
+}
+
+int fwi_profile_enable(fwi_ctx *c, int32_t enable)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    c->profile = enable != 0;
+    for (int k = 0; k < 2; ++k) {
+        c->prof_ms[k] = 0.0;
+        c->prof_launches[k] = 0;
+        c->prof_cells[k] = 0.0;
+    }
+    return FWI_OK;
+}
+
+int fwi_profile_read(fwi_ctx *c, int32_t kind, double *ms, int64_t *launches, double *cells)
+{
+    if (!c || kind < 0 || kind > 1) return fail(FWI_ERR_ARG, "bad argument");
+    if (ms) *ms = c->prof_ms[kind];
+    if (launches) *launches = c->prof_launches[kind];
+    if (cells) *cells = c->prof_cells[kind];
+    return FWI_OK;
+}
+
+}  // extern "C"

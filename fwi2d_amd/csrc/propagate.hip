@@ -1,0 +1,685 @@
+// propagate.hip -- CDNA4 (gfx950) kernels of the 2-D acoustic engine.
+//
+// Time stepping is the reference's 8th-order-space / 2nd-order-time
+// leapfrog (src/modeling/modeling.cu:405-449, src/migration/migration.cu:177-226,
+// src/inversion/inversion.cu:395-444), re-laid out for the hardware:
+//
+//  * z (the contiguous axis of every reference array) maps to lanes: a
+//    64-lane wave owns 64*VZ consecutive rows, VZ = 1, 2 or 4 per lane, so
+//    each column load is one coalesced 256 B .. 1 KiB access per wave;
+//  * the wave marches along x over xw columns and keeps the 9 columns the
+//    x-derivative needs in a register queue, so the current field is read
+//    from memory about once per step (plus an 8-column warm-up per chunk);
+//  * the +-4 z-neighbours come from the neighbouring lanes by cross-lane
+//    shuffles; only the edge lanes read the 4-row halo from memory;
+//  * source injection, the Cerjan sponge, receiver sampling (forward) and
+//    data injection, imaging / gradient and illumination (reverse) are fused
+//    into the same pass, so one launch per time step does all the work;
+//  * blocks are renumbered so that each XCD receives a contiguous run of
+//    tiles: neighbouring chunks share their halo columns in that XCD's L2.
+//
+// FP32 operation order: the Laplacian is summed as differences from the
+// centre cell and the update as (p + (p - old)) + c*lap, with c = dt^2 v^2/dh^2.
+// This is the reference's recurrence with less rounding error (about 7x less
+// than the reference's order against an FP64 evaluation of the same
+// recurrence, over 2,000+ steps).
+//
+// Source injection is folded into the write of the field it enters:
+// the field written at step t receives idh2*w[t+1], which is what the
+// reference's step t+1 adds before reading it (modeling.cu:420-421).  With
+// the sponge on, the reference damps the current field in place
+// (modeling.cu:441-447); here the stored field is undamped and the factor is
+// applied when it is read back as the previous field one step later:
+//   u_{t+1} = d * ((u_t + (u_t - d*u_{t-1})) + c*lap u_t)  [+ idh2 w[t+1]].
+#include "kernels.h"
+
+namespace fwi {
+
+namespace {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ULL;
+
+template <int VZ>
+__device__ __forceinline__ void vload(float (&r)[VZ], const float *p)
+{
+    if constexpr (VZ == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        r[0] = v.x; r[1] = v.y; r[2] = v.z; r[3] = v.w;
+    } else if constexpr (VZ == 2) {
+        const float2 v = *reinterpret_cast<const float2 *>(p);
+        r[0] = v.x; r[1] = v.y;
+    } else {
+        r[0] = *p;
+    }
+}
+
+template <int VZ>
+__device__ __forceinline__ void vstore(float *p, const float (&r)[VZ])
+{
+    if constexpr (VZ == 4) {
+        *reinterpret_cast<float4 *>(p) = make_float4(r[0], r[1], r[2], r[3]);
+    } else if constexpr (VZ == 2) {
+        *reinterpret_cast<float2 *>(p) = make_float2(r[0], r[1]);
+    } else {
+        *p = r[0];
+    }
+}
+
+template <int VZ>
+__device__ __forceinline__ void vzero(float (&r)[VZ])
+{
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) r[k] = 0.f;
+}
+
+// ext[0 .. VZ+7] = rows zb-4 .. zb+VZ+3 of one column: the lane's own VZ
+// rows, 4 above and 4 below taken from neighbouring lanes by shuffles; lanes
+// whose neighbour is outside the wave read the row from memory (0 outside
+// the array).
+template <int VZ>
+__device__ __forceinline__ void zext(float (&ext)[VZ + 8], const float (&own)[VZ],
+                                     const float *col, int zb, int lane, int ldz)
+{
+#pragma unroll
+    for (int r = 0; r < VZ; ++r) ext[4 + r] = own[r];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k = kk - 4;                       // row offset -4 .. -1
+        const int o = -((-k + VZ - 1) / VZ);        // source lane offset (< 0)
+        const int e = k - o * VZ;                   // element of that lane
+        float v = __shfl(own[e], lane + o, 64);
+        if (lane + o < 0) {
+            const int row = zb + k;
+            v = (row >= 0) ? col[row] : 0.f;
+        }
+        ext[kk] = v;
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k = VZ + kk;                      // row offset VZ .. VZ+3
+        const int o = k / VZ;                       // source lane offset (> 0)
+        const int e = k - o * VZ;
+        float v = __shfl(own[e], lane + o, 64);
+        if (lane + o > 63) {
+            const int row = zb + k;
+            v = (row < ldz) ? col[row] : 0.f;
+        }
+        ext[VZ + 4 + kk] = v;
+    }
+}
+
+// Laplacian (times dh^2) at row r of the lane from the x queue q[0..8]
+// (columns j-4..j+4) and the z extension ext, summed as centre differences.
+template <int VZ>
+__device__ __forceinline__ float lap8(const float (&q)[9][VZ], const float (&ext)[VZ + 8],
+                                      int r)
+{
+    const float p = q[4][r];
+    const float s1 = ((q[3][r] - p) + (q[5][r] - p)) + ((ext[r + 3] - p) + (ext[r + 5] - p));
+    const float s2 = ((q[2][r] - p) + (q[6][r] - p)) + ((ext[r + 2] - p) + (ext[r + 6] - p));
+    const float s3 = ((q[1][r] - p) + (q[7][r] - p)) + ((ext[r + 1] - p) + (ext[r + 7] - p));
+    const float s4 = ((q[0][r] - p) + (q[8][r] - p)) + ((ext[r + 0] - p) + (ext[r + 8] - p));
+    return kC4 * s1 - kC3 * s2 + kC2 * s3 - kC1 * s4;
+}
+
+// Cerjan factor of cell (i,j): 1 inside, b1d on the edges, b1d[a]+b1d[b]-1
+// in the corners (modeling.cu:459-506).  i is clamped into the grid.
+__device__ __forceinline__ float damper(const float *b1d, int i, int j, const Grid &g)
+{
+    i = min(max(i, 0), g.nzz - 1);
+    const int nb = g.nb;
+    const bool inz = (i >= nb) && (i < g.nzz - nb);
+    const bool inx = (j >= nb) && (j < g.nxx - nb);
+    if (inz && inx) return 1.f;
+    const int a = (i < nb) ? i : (g.nzz - 1 - i);
+    const int b = (j < nb) ? j : (g.nxx - 1 - j);
+    if (inx) return b1d[a];
+    if (inz) return b1d[b];
+    return (b1d[a] + b1d[b]) - 1.f;
+}
+
+// Tile of a 1-D launch after the XCD renumbering: each of the 8 XCDs gets a
+// contiguous run of per = ceil(ntiles/8) logical tiles.  Returns -1 for the
+// blocks past the last tile.
+__device__ __forceinline__ int xcd_tile(int ntiles)
+{
+    const int id = blockIdx.x;
+    const int per = (ntiles + 7) >> 3;
+    const int logical = (id & 7) * per + (id >> 3);
+    return logical < ntiles ? logical : -1;
+}
+
+// ---------------------------------------------------------------- forward
+
+template <int VZ, bool ABC>
+__global__ __launch_bounds__(64) void fwd_step_kernel(FwdStep a, int nzseg, int nxchunk,
+                                                      int xw, int nslots, int t)
+{
+    const int tile = xcd_tile(nzseg * nxchunk * nslots);
+    if (tile < 0) return;
+    const int zseg = tile % nzseg;
+    const int xc = (tile / nzseg) % nxchunk;
+    const int slot = tile / (nzseg * nxchunk);
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int zb = zseg * 64 * VZ + lane * VZ;
+    const bool active = zb < g.ldz;
+    const int j0 = 4 + xc * xw;
+    const int j1 = min(j0 + xw, g.nxx - 4);
+
+    const long soff = (long)slot * g.fstride;
+    const float *cur = ((t & 1) ? a.fb : a.fa) + soff;
+    float *old = ((t & 1) ? a.fa : a.fb) + soff;
+    const float *cc = a.c + (long)slot * a.cstride;
+    const int gshot = *a.shot0 + slot;
+    const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
+    const float wsrc = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
+    const bool rec_on = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    const int tsamp = t + 1 - a.tlag;
+    float *seis = rec_on ? a.seis + (long)slot * a.rec.nrec * a.nt : nullptr;
+    const int ldz = g.ldz;
+
+    float q[9][VZ];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        if (active) vload<VZ>(q[k], cur + (long)(j0 - 4 + k) * ldz + zb);
+        else vzero<VZ>(q[k]);
+    }
+    float on[VZ], cn[VZ], pn[VZ];
+    vzero<VZ>(on); vzero<VZ>(cn); vzero<VZ>(pn);
+    if (active) {
+        vload<VZ>(on, old + (long)j0 * ldz + zb);
+        vload<VZ>(cn, cc + (long)j0 * ldz + zb);
+    }
+
+    for (int j = j0; j < j1; ++j) {
+        float o[VZ], c[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { o[r] = on[r]; c[r] = cn[r]; }
+        if (active && j + 1 < j1) {           // prefetch the next column
+            vload<VZ>(pn, cur + (long)(j + 5) * ldz + zb);
+            vload<VZ>(on, old + (long)(j + 1) * ldz + zb);
+            vload<VZ>(cn, cc + (long)(j + 1) * ldz + zb);
+        }
+        float ext[VZ + 8];
+        zext<VZ>(ext, q[4], cur + (long)j * ldz, zb, lane, ldz);
+
+        float nv[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) {
+            const int i = zb + r;
+            const float p = q[4][r];
+            const float lap = lap8<VZ>(q, ext, r);
+            float val;
+            if constexpr (ABC) {
+                const float d = damper(a.b1d, i, j, g);
+                val = d * ((p + (p - d * o[r])) + c[r] * lap);
+            } else {
+                val = (p + (p - o[r])) + c[r] * lap;
+            }
+            if (i == siz && j == six) val += wsrc;
+            nv[r] = (i >= 4 && i < g.nzz - 4) ? val : o[r];
+        }
+        if (active) vstore<VZ>(old + (long)j * ldz + zb, nv);
+
+        if (rec_on) {
+            const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
+            for (int k = k0; k < k1; ++k) {
+                const int rr = a.rec.iz[k] - zb;
+#pragma unroll
+                for (int r = 0; r < VZ; ++r) {
+                    if (rr == r) {
+                        float val = nv[r];
+                        if constexpr (ABC) val *= damper(a.b1d, zb + r, j, g);
+                        seis[(long)a.rec.id[k] * a.nt + tsamp] = val;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) q[k][r] = q[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) q[8][r] = pn[r];
+    }
+}
+
+// ---------------------------------------------------------------- reverse
+
+template <int VZ, int MODE>
+__global__ __launch_bounds__(64) void rev_step_kernel(RevStep a, int nzseg, int nxchunk,
+                                                      int xw, int nslots, int k)
+{
+    const int tile = xcd_tile(nzseg * nxchunk * nslots);
+    if (tile < 0) return;
+    const int zseg = tile % nzseg;
+    const int xc = (tile / nzseg) % nxchunk;
+    const int slot = tile / (nzseg * nxchunk);
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int zb = zseg * 64 * VZ + lane * VZ;
+    const bool active = zb < g.ldz;
+    const int j0 = 4 + xc * xw;
+    const int j1 = min(j0 + xw, g.nxx - 4);
+    const int ldz = g.ldz;
+
+    const long soff = (long)slot * g.fstride;
+    const float *scur = ((k & 1) ? a.sa : a.sb) + soff;
+    float *sold = ((k & 1) ? a.sb : a.sa) + soff;
+    const float *rcur = ((k & 1) ? a.rb : a.ra) + soff;
+    float *rold = ((k & 1) ? a.ra : a.rb) + soff;
+    const float *cc = a.c + (long)slot * a.cstride;
+    float *acc = a.acc + soff;
+    float *ill = a.ill + soff;
+    const bool inj_on = (k + 1 < a.nt);
+    const int trow = a.nt - 2 - k;        // data sample entering R_{k+1}
+    const int drow = a.data_by_shot ? (*a.shot0 + slot) : slot;
+    const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
+
+    float qs[9][VZ], qr[9][VZ];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        if (active) {
+            vload<VZ>(qs[m], scur + (long)(j0 - 4 + m) * ldz + zb);
+            vload<VZ>(qr[m], rcur + (long)(j0 - 4 + m) * ldz + zb);
+        } else {
+            vzero<VZ>(qs[m]); vzero<VZ>(qr[m]);
+        }
+    }
+    float osn[VZ], orn[VZ], cn[VZ], an[VZ], in[VZ], psn[VZ], prn[VZ];
+    vzero<VZ>(osn); vzero<VZ>(orn); vzero<VZ>(cn); vzero<VZ>(an); vzero<VZ>(in);
+    vzero<VZ>(psn); vzero<VZ>(prn);
+    if (active) {
+        const long off = (long)j0 * ldz + zb;
+        vload<VZ>(osn, sold + off); vload<VZ>(orn, rold + off); vload<VZ>(cn, cc + off);
+        vload<VZ>(an, acc + off); vload<VZ>(in, ill + off);
+    }
+
+    for (int j = j0; j < j1; ++j) {
+        float os[VZ], orr[VZ], c[VZ], ac[VZ], il[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) {
+            os[r] = osn[r]; orr[r] = orn[r]; c[r] = cn[r]; ac[r] = an[r]; il[r] = in[r];
+        }
+        if (active && j + 1 < j1) {
+            const long offp = (long)(j + 5) * ldz + zb;
+            const long offn = (long)(j + 1) * ldz + zb;
+            vload<VZ>(psn, scur + offp); vload<VZ>(prn, rcur + offp);
+            vload<VZ>(osn, sold + offn); vload<VZ>(orn, rold + offn);
+            vload<VZ>(cn, cc + offn); vload<VZ>(an, acc + offn); vload<VZ>(in, ill + offn);
+        }
+        float es[VZ + 8], er[VZ + 8];
+        zext<VZ>(es, qs[4], scur + (long)j * ldz, zb, lane, ldz);
+        zext<VZ>(er, qr[4], rcur + (long)j * ldz, zb, lane, ldz);
+
+        float ns[VZ], nr[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) {
+            const int i = zb + r;
+            const bool inside = (i >= 4) && (i < g.nzz - 4);
+            const float ps = qs[4][r];
+            const float pr = qr[4][r];
+            const float lsc = c[r] * lap8<VZ>(qs, es, r);
+            const float sn = (ps + (ps - os[r])) + lsc;
+            const float rn = (pr + (pr - orr[r])) + c[r] * lap8<VZ>(qr, er, r);
+            ns[r] = inside ? sn : os[r];
+            nr[r] = inside ? rn : orr[r];
+            if (inside) {
+                il[r] += sn * sn;
+                if constexpr (MODE == 0) ac[r] += sn * pr;
+                else ac[r] += pr * lsc;
+            }
+        }
+        if (inj_on) {
+            const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
+            for (int m = k0; m < k1; ++m) {
+                const int rr = a.rec.iz[m] - zb;
+#pragma unroll
+                for (int r = 0; r < VZ; ++r)
+                    if (rr == r) nr[r] += a.idh2 * dat[(long)a.rec.id[m] * a.nt + trow];
+            }
+        }
+        if (active) {
+            const long off = (long)j * ldz + zb;
+            vstore<VZ>(sold + off, ns); vstore<VZ>(rold + off, nr);
+            vstore<VZ>(acc + off, ac); vstore<VZ>(ill + off, il);
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) { qs[m][r] = qs[m + 1][r]; qr[m][r] = qr[m + 1][r]; }
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { qs[8][r] = psn[r]; qr[8][r] = prn[r]; }
+    }
+}
+
+// ---------------------------------------------------------------- helpers
+
+__global__ void fwd_init_kernel(FwdStep a, int nslots)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nslots || a.nt < 1) return;
+    const int gshot = *a.shot0 + slot;
+    const long idx = (long)slot * a.g.fstride + (long)a.src_ix[gshot] * a.g.ldz + a.src_iz[gshot];
+    a.fa[idx] += a.wav[0];
+}
+
+__global__ void rev_init_kernel(RevStep a, int nslots)
+{
+    // one thread per slot: receivers sharing a cell are summed in order
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nslots || a.nt < 1) return;
+    const int drow = a.data_by_shot ? (*a.shot0 + slot) : slot;
+    const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
+    float *ra = a.ra + (long)slot * a.g.fstride;
+    for (int j = 0; j < a.g.nxx; ++j)
+        for (int m = a.rec.col_ptr[j]; m < a.rec.col_ptr[j + 1]; ++m)
+            ra[(long)j * a.g.ldz + a.rec.iz[m]] += a.idh2 * dat[(long)a.rec.id[m] * a.nt + a.nt - 1];
+}
+
+__global__ void fill_kernel(float *p, long n, float v)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void set_int_kernel(int *p, int v) { *p = v; }
+
+__global__ void pad_model_kernel(const float *vp, int nz, int nx, Grid g, float *out)
+{
+    const long n = (long)g.ldz * g.nxx;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % g.ldz), j = (int)(idx / g.ldz);
+        float v = 0.f;
+        if (i < g.nzz) {
+            const int ii = min(max(i - g.nb, 0), nz - 1);
+            const int jj = min(max(j - g.nb, 0), nx - 1);
+            v = vp[(long)jj * nz + ii];
+        }
+        out[idx] = v;
+    }
+}
+
+__global__ void coef_kernel(const float *v, long n, float k, float *c)
+{
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long)gridDim.x * blockDim.x) {
+        const float x = v[idx];
+        c[idx] = (x * x) * k;
+    }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ULL;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return z;
+}
+
+__device__ __forceinline__ float rb_draw(uint64_t key, uint64_t cell, int k)
+{
+    const uint64_t h = mix64(key + (2 * cell + (uint64_t)k + 1) * kGolden);
+    const int32_t s = (int32_t)(h >> 40) - 8388608;
+    return __fmul_rn((float)s, 1.0f / 8388608.0f);
+}
+
+// Seeded random boundary: pad cells take the edge value plus a ramped
+// uniform perturbation, once for the x pad and once for the z pad, clamped to
+// [vlo, vhi] (the structure of random_boundary_bg, modeling.cu:591-628, with
+// a counter-based draw).  Every float operation is an explicit
+// round-to-nearest intrinsic so the result is bit-identical to the
+// sequential C statement of the same generator.
+__global__ void rb_kernel(const float *vpad, Grid g, int nslots, const int *shot0,
+                          uint32_t iteration, uint64_t seed, float amp_abs, float amp_rel,
+                          float vlo, float vhi, float kc, float *c, float *v_out)
+{
+    const long ncell = (long)g.nzz * g.nxx;
+    const int slot = blockIdx.y;
+    const uint64_t stream = ((uint64_t)iteration << 32) | (uint32_t)(*shot0 + slot);
+    const uint64_t key = mix64(seed ^ mix64(stream + kGolden));
+    const float fnb = (float)g.nb;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < ncell;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % g.nzz), j = (int)(idx / g.nzz);
+        const float base = vpad[(long)j * g.ldz + i];
+        const bool xpad = (j < g.nb) || (j >= g.nxx - g.nb);
+        const bool zpad = (i < g.nb) || (i >= g.nzz - g.nb);
+        float val = base;
+        if (xpad || zpad) {
+            const float A = __fadd_rn(amp_abs, __fmul_rn(amp_rel, base));
+            const uint64_t cell = (uint64_t)idx;
+            if (xpad) {
+                const int dj = (j < g.nb) ? j : (g.nxx - 1 - j);
+                const float f = __fsub_rn(1.0f, __fdiv_rn((float)dj, fnb));
+                val = __fadd_rn(val, __fmul_rn(__fmul_rn(f, A), rb_draw(key, cell, 0)));
+            }
+            if (zpad) {
+                const int di = (i < g.nb) ? i : (g.nzz - 1 - i);
+                const float f = __fsub_rn(1.0f, __fdiv_rn((float)di, fnb));
+                val = __fadd_rn(val, __fmul_rn(__fmul_rn(f, A), rb_draw(key, cell, 1)));
+            }
+            val = fminf(fmaxf(val, vlo), vhi);
+        }
+        if (c) c[(long)slot * g.fstride + (long)j * g.ldz + i] = __fmul_rn(__fmul_rn(val, val), kc);
+        if (v_out) v_out[(long)slot * ncell + idx] = val;
+    }
+}
+
+__global__ __launch_bounds__(256) void residual_kernel(const float *obs, float *cal,
+                                                       const int *shot0, long per_shot,
+                                                       long n, double *partials)
+{
+    __shared__ double red[256];
+    const long base_obs = (long)(*shot0) * per_shot;
+    double s = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x) {
+        const float r = obs[base_obs + e] - cal[e];
+        cal[e] = r;
+        s += (double)r * (double)r;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void sum_partials_kernel(const double *p, int n, double *out)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int e = threadIdx.x; e < n; e += 256) s += p[e];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void reduce_slots_kernel(float *src, long fstride, long ncell, int nslots, float *dst)
+{
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < ncell;
+         idx += (long)gridDim.x * blockDim.x) {
+        float s = dst[idx];
+        for (int k = 0; k < nslots; ++k) {
+            s += src[(long)k * fstride + idx];
+            src[(long)k * fstride + idx] = 0.f;
+        }
+        dst[idx] = s;
+    }
+}
+
+__global__ void crop_kernel(const float *src, Grid g, int nz, int nx, float *dst)
+{
+    const long n = (long)nz * nx;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % nz), j = (int)(idx / nz);
+        dst[idx] = src[(long)(j + g.nb) * g.ldz + (i + g.nb)];
+    }
+}
+
+__global__ void zero_sample0_kernel(float *seis, int nrec, int nt, long n)
+{
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long)gridDim.x * blockDim.x)
+        seis[idx * nt] = 0.f;
+}
+
+inline int grid1d(long n, int block, int cap = 8192)
+{
+    long b = (n + block - 1) / block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+template <int VZ>
+hipError_t fwd_dispatch(const FwdStep &a, const Tiling &tl, int nslots, int t, hipStream_t s)
+{
+    const int ntiles = tl.nzseg * tl.nxchunk * nslots;
+    const int nblocks = ((ntiles + 7) >> 3) << 3;
+    if (a.abc)
+        hipLaunchKernelGGL((fwd_step_kernel<VZ, true>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    else
+        hipLaunchKernelGGL((fwd_step_kernel<VZ, false>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    return hipGetLastError();
+}
+
+template <int VZ>
+hipError_t rev_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, hipStream_t s)
+{
+    const int ntiles = tl.nzseg * tl.nxchunk * nslots;
+    const int nblocks = ((ntiles + 7) >> 3) << 3;
+    if (a.mode == 0)
+        hipLaunchKernelGGL((rev_step_kernel<VZ, 0>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+    else
+        hipLaunchKernelGGL((rev_step_kernel<VZ, 1>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t, hipStream_t s)
+{
+    switch (tl.vz) {
+    case 1: return fwd_dispatch<1>(a, tl, nslots, t, s);
+    case 2: return fwd_dispatch<2>(a, tl, nslots, t, s);
+    case 4: return fwd_dispatch<4>(a, tl, nslots, t, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k, hipStream_t s)
+{
+    switch (tl.vz) {
+    case 1: return rev_dispatch<1>(a, tl, nslots, k, s);
+    case 2: return rev_dispatch<2>(a, tl, nslots, k, s);
+    case 4: return rev_dispatch<4>(a, tl, nslots, k, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fwd_init(const FwdStep &a, int nslots, hipStream_t s)
+{
+    hipLaunchKernelGGL(fwd_init_kernel, dim3((nslots + 63) / 64), dim3(64), 0, s, a, nslots);
+    return hipGetLastError();
+}
+
+hipError_t launch_rev_init(const RevStep &a, int nslots, hipStream_t s)
+{
+    hipLaunchKernelGGL(rev_init_kernel, dim3((nslots + 63) / 64), dim3(64), 0, s, a, nslots);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float *p, long n, float v, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_int(int *p, int v, hipStream_t s)
+{
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_model(const float *vp, int nz, int nx, const Grid &g, float *out,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(pad_model_kernel, dim3(grid1d((long)g.ldz * g.nxx, 256)), dim3(256), 0,
+                       s, vp, nz, nx, g, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_coef(const float *v, const Grid &g, float k, float *c, hipStream_t s)
+{
+    const long n = (long)g.ldz * g.nxx;
+    hipLaunchKernelGGL(coef_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, v, n, k, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_rb(const float *vpad, const Grid &g, int nslots, const int *shot0,
+                     uint32_t iteration, uint64_t seed, float amp_abs, float amp_rel,
+                     float vlo, float vhi, float k, float *c, float *v_out, hipStream_t s)
+{
+    const long n = (long)g.nzz * g.nxx;
+    hipLaunchKernelGGL(rb_kernel, dim3(grid1d(n, 256, 2048), nslots), dim3(256), 0, s, vpad, g,
+                       nslots, shot0, iteration, seed, amp_abs, amp_rel, vlo, vhi, k, c, v_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const float *obs, float *cal, const int *shot0, long per_shot,
+                           int nslots, double *partials, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(residual_kernel, dim3(nblocks), dim3(256), 0, s, obs, cal, shot0,
+                       per_shot, per_shot * nslots, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const double *partials, int n, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, partials, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_slots(float *src, long fstride, long ncell, int nslots, float *dst,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_slots_kernel, dim3(grid1d(ncell, 256)), dim3(256), 0, s, src,
+                       fstride, ncell, nslots, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_crop(const float *src, const Grid &g, int nz, int nx, float *dst,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(crop_kernel, dim3(grid1d((long)nz * nx, 256)), dim3(256), 0, s, src, g,
+                       nz, nx, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_sample0(float *seis, int nrec, int nt, int nslots, hipStream_t s)
+{
+    const long n = (long)nrec * nslots;
+    hipLaunchKernelGGL(zero_sample0_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, seis, nrec,
+                       nt, n);
+    return hipGetLastError();
+}
+
+}  // namespace fwi

@@ -1,0 +1,288 @@
+// survey.cpp -- see survey.hpp.
+#include "survey.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+
+namespace fwi {
+
+void check(int rc, const char *what)
+{
+    if (rc != FWI_OK)
+        throw std::runtime_error(std::string(what) + ": " + fwi_last_error());
+}
+
+namespace {
+
+std::string fname_suffix(float fmax, int nz, int nx, float dh)
+{
+    return std::to_string(int(fmax)) + "Hz_" + std::to_string(nz) + "x" + std::to_string(nx) +
+           "_" + std::to_string((int)(dh)) + "m.bin";
+}
+
+}  // namespace
+
+Modeling::~Modeling()
+{
+    if (ctx) fwi_destroy(ctx);
+}
+
+void Modeling::set_main_parameters()
+{
+    nx = param_int("x_samples", parameters);
+    nz = param_int("z_samples", parameters);
+    dh = param_float("model_spacing", parameters);
+    nt = param_int("time_samples", parameters);
+    dt = param_float("time_spacing", parameters);
+    fmax = param_float("max_frequency", parameters);
+}
+
+void Modeling::set_wavelet()
+{
+    wavelet.assign(nt, 0.f);
+    int32_t tl = 0;
+    check(fwi_ricker_wavelet(nt, dt, fmax, wavelet.data(), &tl), "wavelet");
+    tlag = tl;
+}
+
+void Modeling::set_properties()
+{
+    vp.assign((size_t)nx * nz, 0.f);
+    import_binary_float(catch_parameter("model_file", parameters), vp.data(), (long)nx * nz);
+    vmin = *std::min_element(vp.begin(), vp.end());
+    vmax = *std::max_element(vp.begin(), vp.end());
+}
+
+void Modeling::create_engine(int rbc_nb, float varVp, float varVp_rel, uint64_t seed)
+{
+    geometry.set_parameters(parameters);
+    max_batch = param_int("gpu_max_batch", parameters, std::min(geometry.nsrc, 256));
+    fwi_config cfg{};
+    cfg.nz = nz;
+    cfg.nx = nx;
+    cfg.dh = dh;
+    cfg.nt = nt;
+    cfg.dt = dt;
+    cfg.abc_nb = abc_nb;
+    cfg.abc_factor = abc_factor;
+    cfg.rbc_nb = rbc_nb;
+    cfg.rbc_varVp = varVp;
+    cfg.rbc_varVp_rel = varVp_rel;
+    cfg.rbc_seed = seed;
+    cfg.max_batch = std::max(1, max_batch);
+    check(fwi_create(&cfg, &ctx), "create");
+    check(fwi_set_wavelet(ctx, wavelet.data(), tlag), "set_wavelet");
+    check(fwi_set_geometry(ctx, geometry.nsrc, geometry.xsrc.data(), geometry.zsrc.data(),
+                           geometry.nrec, geometry.xrec.data(), geometry.zrec.data()),
+          "set_geometry");
+    check(fwi_set_model(ctx, vp.data()), "set_model");
+}
+
+void Modeling::set_parameters()
+{
+    title = "Seismic Modeling";
+    set_main_parameters();
+    const float abc_length = param_float("abc_length", parameters);
+    abc_factor = param_float("abc_factor", parameters);
+    abc_nb = (int)(abc_length / dh) + 1;
+    set_wavelet();
+    set_properties();
+    data_folder = catch_parameter("mod_output_folder", parameters, "");
+    create_engine(0, 0.f, 0.f, 0);
+}
+
+void Modeling::show_information() const
+{
+    std::cout << std::string(80, '-') << "\n" << title << "\n" << std::string(80, '-') << "\n";
+    std::cout << "Model dimensions: (z = " << (nz - 1) * dh << ", x = " << (nx - 1) * dh
+              << ") m\n";
+    std::cout << "Shots: " << geometry.nsrc << ", receivers: " << geometry.nrec
+              << ", batch: " << max_batch << "\n";
+}
+
+void Modeling::forward_solver()
+{
+    seismogram.assign((size_t)geometry.nsrc * geometry.nrec * nt, 0.f);
+    check(fwi_modeling(ctx, FWI_BOUNDARY_ABC, 0, geometry.nsrc, seismogram.data()), "modeling");
+}
+
+void Modeling::get_seismogram()
+{
+    const long per = (long)geometry.nrec * nt;
+    for (int s = 0; s < geometry.nsrc; ++s) {
+        std::string file = data_folder + "seismogram_nt" + std::to_string(nt) + "_nr" +
+                           std::to_string(geometry.nrec) + "_" + std::to_string(int(1e6f * dt)) +
+                           "us_shot_" + std::to_string(s + 1) + ".bin";
+        export_binary_float(file, seismogram.data() + s * per, per);
+    }
+}
+
+// ------------------------------------------------------------------ Migration
+
+void Migration::set_parameters()
+{
+    title = "Reverse Time Migration";
+    set_main_parameters();
+    rbc_varVp = param_float("mig_rbc_varVp", parameters);
+    rbc_varVp_rel = param_float("mig_rbc_varVp_rel", parameters, 0.f);
+    const float rbc_length = param_float("mig_rbc_length", parameters);
+    rbc_nb = (int)(rbc_length / dh) + 1;
+    const uint64_t seed = param_u64("rbc_seed", parameters, 20240u);
+    set_wavelet();
+    set_properties();
+    input_folder = catch_parameter("mig_input_folder", parameters, "");
+    input_prefix = catch_parameter("mig_input_prefix", parameters);
+    output_folder = catch_parameter("mig_output_folder", parameters, "");
+    abc_nb = 0;
+    create_engine(rbc_nb, rbc_varVp, rbc_varVp_rel, seed);
+    const long per = (long)geometry.nrec * nt;
+    std::vector<float> data(per);
+    for (int s = 0; s < geometry.nsrc; ++s) {   // migration.cu:129-134, once per shot
+        import_binary_float(input_folder + input_prefix + std::to_string(s + 1) + ".bin",
+                            data.data(), per);
+        check(fwi_set_observed(ctx, s, 1, data.data()), "set_observed");
+    }
+    check(fwi_reset(ctx, FWI_RESET_IMAGE | FWI_RESET_ILLUM), "reset");
+}
+
+void Migration::run_migration()
+{
+    check(fwi_migration(ctx, 0, geometry.nsrc), "migration");
+}
+
+void Migration::export_seismic()
+{
+    std::vector<float> image((size_t)nx * nz);
+    check(fwi_get_image(ctx, image.data(), nullptr, nullptr), "get_image");
+    export_binary_float(output_folder + "RTM_section_" + fname_suffix(fmax, nz, nx, dh),
+                        image.data(), (long)nx * nz);
+}
+
+// ------------------------------------------------------------------ Inversion
+
+void Inversion::set_parameters()
+{
+    title = "Full Waveform Inversion";
+    set_main_parameters();
+    const float abc_length = param_float("abc_length", parameters);
+    abc_factor = param_float("abc_factor", parameters);
+    abc_nb = (int)(abc_length / dh) + 1;
+    rbc_varVp = param_float("inv_rbc_varVp", parameters);
+    rbc_varVp_rel = param_float("inv_rbc_varVp_rel", parameters, 0.f);
+    const float rbc_length = param_float("inv_rbc_length", parameters);
+    rbc_nb = (int)(rbc_length / dh) + 1;
+    const uint64_t seed = param_u64("rbc_seed", parameters, 20240u);
+    zmask = param_float("depth_mask", parameters);
+    max_iteration = param_int("max_iteration", parameters);
+    input_folder = catch_parameter("inv_input_folder", parameters, "");
+    input_prefix = catch_parameter("inv_input_prefix", parameters);
+    output_folder = catch_parameter("inv_output_folder", parameters, "");
+    residuo_folder = catch_parameter("inv_residuo_folder", parameters, "");
+    set_wavelet();
+    set_properties();
+    vmin0 = vmin;
+    vmax0 = vmax;
+    create_engine(rbc_nb, rbc_varVp, rbc_varVp_rel, seed);
+    // observed data stay resident on the device (read once, not every pass)
+    const long per = (long)geometry.nrec * nt;
+    std::vector<float> data(per);
+    for (int s = 0; s < geometry.nsrc; ++s) {
+        import_binary_float(input_folder + input_prefix + std::to_string(s + 1) + ".bin",
+                            data.data(), per);
+        check(fwi_set_observed(ctx, s, 1, data.data()), "set_observed");
+    }
+    model.resize(vp.size());
+    for (size_t k = 0; k < vp.size(); ++k) model[k] = 1.0f / (vp[k] * vp[k]);
+    gradient.assign(vp.size(), 0.f);
+    // the illumination accumulates over every gradient pass (inversion.cu:51, 441)
+    check(fwi_reset(ctx, FWI_RESET_GRADIENT | FWI_RESET_ILLUM), "reset");
+}
+
+void Inversion::compute_gradient()
+{
+    sum_res = 0.0;
+    check(fwi_reset(ctx, FWI_RESET_GRADIENT), "reset");
+    if (iteration < max_iteration) {
+        check(fwi_gradient(ctx, 0, geometry.nsrc, (uint32_t)iteration, &sum_res), "gradient");
+        double gmax = 0.0;
+        check(fwi_get_gradient(ctx, zmask, gradient.data(), nullptr, nullptr, &gmax),
+              "get_gradient");
+        std::cout << "iteration " << iteration << ": gradient pass done, max|g| = " << gmax
+                  << "\n";
+    } else {
+        // the final pass only measures the residual (inversion.cu:117-136)
+        check(fwi_misfit(ctx, 0, geometry.nsrc, &sum_res), "misfit");
+    }
+}
+
+void Inversion::check_convergence()
+{
+    ++iteration;
+    residuo.push_back(std::sqrt(sum_res));
+    converged = iteration > max_iteration;
+    std::cout << "residuo[" << residuo.size() - 1 << "] = " << residuo.back() << "\n";
+}
+
+double Inversion::linesearch(float alpha)
+{
+    std::vector<float> trial(model.size());
+    for (size_t k = 0; k < model.size(); ++k) {
+        const float m = model[k] - alpha * gradient[k];
+        if (!(m > 0.f))
+            throw std::runtime_error("line search: non-positive slowness^2 at a trial step");
+        trial[k] = 1.0f / std::sqrt(m);
+    }
+    check(fwi_set_model(ctx, trial.data()), "set_model (trial)");
+    double s = 0.0;
+    check(fwi_misfit(ctx, 0, geometry.nsrc, &s), "misfit");
+    return std::sqrt(s);
+}
+
+void Inversion::optimization()
+{
+    const double dm = 1.0 / ((double)vmin0 * vmin0) - 1.0 / ((double)vmax0 * vmax0);
+    const double a[3] = {0.0, 0.02 * dm, 0.05 * dm};
+    double f[3];
+    f[0] = residuo.back();
+    f[1] = linesearch((float)a[1]);
+    f[2] = linesearch((float)a[2]);
+    check(fwi_parabola_step(a, f, &step), "parabola");
+}
+
+void Inversion::update_model()
+{
+    std::vector<float> v(model.size());
+    for (size_t k = 0; k < model.size(); ++k) {
+        model[k] = model[k] - (float)step * gradient[k];
+        if (!(model[k] > 0.f))
+            throw std::runtime_error("model update produced a non-positive slowness^2");
+        v[k] = 1.0f / std::sqrt(model[k]);
+    }
+    vp = v;
+    // the accepted model is the one the next gradient uses
+    check(fwi_set_model(ctx, vp.data()), "set_model");
+    export_binary_float(output_folder + "model_FWI_iteration_" + std::to_string(iteration) + "_" +
+                            fname_suffix(fmax, nz, nx, dh),
+                        vp.data(), (long)vp.size());
+}
+
+void Inversion::export_convergence()
+{
+    const std::string path =
+        residuo_folder + "convergence_" + std::to_string(max_iteration) + "_iterations.txt";
+    std::ofstream out(path);
+    if (!out.is_open()) throw std::runtime_error("Error: " + path + " could not be opened!");
+    for (double r : residuo) out << (float)r << "\n";
+    std::cout << "Text file " << path << " was successfully written." << std::endl;
+}
+
+void Inversion::export_final_model()
+{
+    export_binary_float(output_folder + "final_model_FWI_" + fname_suffix(fmax, nz, nx, dh),
+                        vp.data(), (long)vp.size());
+}
+
+}  // namespace fwi

@@ -1,0 +1,87 @@
+// survey.hpp -- the workflow classes of the drop-in executables, with the
+// reference's method names (src/modeling/modeling.cuh:18-87,
+// src/migration/migration.cuh:6-38, src/inversion/inversion.cuh:6-83).
+// All wavefield work runs on the GPU engine behind the C ABI (fwi2d.h); every
+// shot of a pass runs as one batch instead of one shot at a time.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "admin.hpp"
+#include "fwi2d.h"
+
+namespace fwi {
+
+// Throws std::runtime_error with the engine's message when rc != FWI_OK.
+void check(int rc, const char *what);
+
+class Modeling {
+public:
+    virtual ~Modeling();
+    std::string parameters;
+    Geometry geometry;
+
+    void set_parameters();        // modeling.cu:107-119
+    void forward_solver();        // all shots; modeling.cu:390-403
+    void get_seismogram();        // one file per shot; modeling.cu:383-388
+    void show_information() const;
+
+protected:
+    int nx = 0, nz = 0, nt = 0, tlag = 0, max_batch = 1;
+    float dh = 0, dt = 0, fmax = 0;
+    int abc_nb = 0;
+    float abc_factor = 0;
+    float vmin = 0, vmax = 0;
+    std::vector<float> vp;          // interior model [nx][nz]
+    std::vector<float> wavelet;
+    std::vector<float> seismogram;  // [nsrc][nrec][nt]
+    std::string data_folder, title;
+    fwi_ctx *ctx = nullptr;
+
+    void set_main_parameters();     // modeling.cu:121-134
+    void set_wavelet();             // modeling.cu:136-194
+    void set_properties();          // modeling.cu:256-292
+    void create_engine(int rbc_nb, float varVp, float varVp_rel, uint64_t seed);
+};
+
+class Migration : public Modeling {
+public:
+    void set_parameters();          // migration.cu:49-82
+    void run_migration();           // forward_propagation + backward_propagation, all shots
+    void export_seismic();          // migration.cu:136-161
+
+private:
+    float rbc_varVp = 0, rbc_varVp_rel = 0;
+    int rbc_nb = 0;
+    std::string input_folder, input_prefix, output_folder;
+};
+
+class Inversion : public Modeling {
+public:
+    bool converged = false;
+    void set_parameters();          // inversion.cu:3-53
+    void compute_gradient();        // inversion.cu:111-166
+    void check_convergence();       // inversion.cu:275-284
+    void optimization();            // inversion.cu:286-308
+    void update_model();            // inversion.cu:344-358
+    void export_convergence();      // inversion.cu:360-372
+    void export_final_model();      // inversion.cu:374-379
+
+    int iteration = 0;
+    int max_iteration = 0;
+    std::vector<double> residuo;
+    std::vector<float> model;       // m = 1/v^2 (inversion.cu:101-109)
+    std::vector<float> gradient;    // search direction
+
+private:
+    float zmask = 0;
+    float rbc_varVp = 0, rbc_varVp_rel = 0;
+    int rbc_nb = 0;
+    double sum_res = 0, step = 0;
+    float vmin0 = 0, vmax0 = 0;     // extremes of the initial model (dm of the line search)
+    std::string input_folder, input_prefix, output_folder, residuo_folder;
+    double linesearch(float alpha);
+};
+
+}  // namespace fwi

@@ -1,0 +1,173 @@
+/*
+ * fwi2d.h -- C ABI of the MI355X-native acoustic modeling / RTM / FWI engine.
+ *
+ * One opaque context per survey (grid, model, wavelet, geometry, observed
+ * data, accumulators) owning its device buffers and one HIP stream.  All
+ * pointers in these signatures are HOST pointers owned by the caller and
+ * only read or written during the call.  Every entry point returns 0 on
+ * success or a positive FWI_ERR_* code; fwi_last_error() returns the message
+ * of the last failure on the calling thread.  A context is used by one host
+ * thread at a time.
+ *
+ * The reference (phbastosa/FWI2D) has no plugin or FFI layer: its operator
+ * surface is the Modeling / Migration / Inversion class methods and their
+ * CUDA kernels.  Each entry point below cites the methods it replaces.
+ * Arrays follow the reference's file layouts: models and images [nx][nz]
+ * with z fastest; seismograms [shot][nrec][nt] with t fastest.
+ */
+#ifndef FWI2D_H
+#define FWI2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FWI_OK          0
+#define FWI_ERR_ARG     1   /* invalid argument or geometry             */
+#define FWI_ERR_STATE   2   /* call order: a required set_* is missing  */
+#define FWI_ERR_HIP     3   /* HIP runtime failure (message has detail) */
+#define FWI_ERR_NOMEM   4   /* device or host allocation failed         */
+#define FWI_ERR_NUMERIC 5   /* CFL violated, non-positive model, ...    */
+
+#define FWI_BOUNDARY_ABC 0  /* Cerjan sponge on the abc_nb-padded grid  */
+#define FWI_BOUNDARY_RBC 1  /* seeded random boundary on the rbc_nb grid */
+
+#define FWI_RESET_IMAGE    1
+#define FWI_RESET_GRADIENT 2
+#define FWI_RESET_ILLUM    4
+
+typedef struct fwi_ctx fwi_ctx;
+
+typedef struct fwi_config {
+    int32_t  nz, nx;        /* z_samples, x_samples (model samples)            */
+    float    dh;            /* model_spacing [m]                               */
+    int32_t  nt;            /* time_samples                                    */
+    float    dt;            /* time_spacing [s]                                */
+    int32_t  abc_nb;        /* sponge pad per side, (int)(abc_length/dh)+1     */
+    float    abc_factor;    /* Cerjan factor                                   */
+    int32_t  rbc_nb;        /* random-boundary pad per side (cells)            */
+    float    rbc_varVp;     /* random-boundary amplitude [m/s]                 */
+    float    rbc_varVp_rel; /* extra amplitude as a fraction of the edge speed */
+    uint64_t rbc_seed;      /* seed of the random boundary                     */
+    int32_t  max_batch;     /* shots resident on the device per pass (>= 1)    */
+} fwi_config;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *fwi_last_error(void);
+/* Library version string. */
+const char *fwi_version(void);
+
+/* Creates a context and its stream.  Replaces the set_parameters chains of
+ * Modeling (src/modeling/modeling.cu:107-119), Migration
+ * (src/migration/migration.cu:49-82) and Inversion (src/inversion/inversion.cu:3-53)
+ * for everything that is not read from files. */
+int fwi_create(const fwi_config *cfg, fwi_ctx **out);
+int fwi_destroy(fwi_ctx *ctx);
+
+/* Host helper: Ricker + half-derivative wavelet, FP64 DFT, rounded to FP32.
+ * Replaces Modeling::set_wavelet (modeling.cu:136-194); returns tlag. */
+int fwi_ricker_wavelet(int32_t nt, float dt, float fmax, float *w_out, int32_t *tlag_out);
+
+/* Uploads the source wavelet w[nt]; T = nt + tlag steps per pass.
+ * Replaces the cudaMemcpy of Modeling::set_wavelet (modeling.cu:186-188). */
+int fwi_set_wavelet(fwi_ctx *ctx, const float *w, int32_t tlag);
+
+/* Source / receiver coordinates in metres.  Grid indices follow
+ * (int)((x + 0.5 dh)/dh) + nb per padded grid.  Replaces
+ * Geometry::set_parameters (src/geometry/geometry.cpp:3-43) and
+ * Modeling::initialization (modeling.cu:356-381). */
+int fwi_set_geometry(fwi_ctx *ctx, int32_t nsrc, const float *xsrc, const float *zsrc,
+                     int32_t nrec, const float *xrec, const float *zrec);
+
+/* Interior velocity model vp[nx][nz] (m/s).  Padded on the device by edge
+ * replication for both grids.  Replaces Modeling::set_properties
+ * (modeling.cu:256-292) and the model uploads of Inversion::linesearch /
+ * update_model (inversion.cu:310-321, 344-354). */
+int fwi_set_model(fwi_ctx *ctx, const float *vp);
+
+/* Observed (or migration input) data of shots [shot0, shot0+nshots):
+ * data[nshots][nrec][nt], kept resident on the device.  Replaces
+ * Migration::set_seismic_source (migration.cu:129-134) and
+ * Inversion::set_obs_data (inversion.cu:168-172). */
+int fwi_set_observed(fwi_ctx *ctx, int32_t shot0, int32_t nshots, const float *data);
+
+/* Forward modeling of shots [shot0, shot0+nshots) on the chosen boundary;
+ * if seis_out != NULL it receives [nshots][nrec][nt] (sample 0 = 0).
+ * Replaces Modeling::forward_solver + compute_pressure + compute_seismogram
+ * + get_seismogram (modeling.cu:383-457). */
+int fwi_modeling(fwi_ctx *ctx, int32_t boundary, int32_t shot0, int32_t nshots,
+                 float *seis_out);
+
+/* RTM of shots [shot0, shot0+nshots) with the observed data as input:
+ * random-boundary forward, time reversal and receiver back-propagation,
+ * image += S*R and illum += S^2 accumulated on the device.  Replaces
+ * Migration::forward_propagation / backward_propagation, inject_seismogram
+ * and cross_correlation (migration.cu:95-127, 163-226). */
+int fwi_migration(fwi_ctx *ctx, int32_t shot0, int32_t nshots);
+
+/* FWI gradient of shots [shot0, shot0+nshots): sponge-boundary modeling,
+ * residual obs - cal with its FP64 sum of squares added to *sum_res2,
+ * random-boundary forward (stream (iteration, shot)), adjoint back-propagation
+ * and grad += dt R v^2 lap(S), illum += S^2.  Replaces the shot loop body of
+ * Inversion::compute_gradient (inversion.cu:119-134) with get_cal_data,
+ * set_adjoint_source, update_RBC, forward/backward_propagation,
+ * inject_adjoint and build_gradient (:174-248, 381-444). */
+int fwi_gradient(fwi_ctx *ctx, int32_t shot0, int32_t nshots, uint32_t iteration,
+                 double *sum_res2);
+
+/* FP64 sum of squared residuals of sponge-boundary modeling against the
+ * observed data, added to *sum_res2.  Replaces the shot loop of
+ * Inversion::linesearch (inversion.cu:323-341). */
+int fwi_misfit(fwi_ctx *ctx, int32_t shot0, int32_t nshots, double *sum_res2);
+
+/* Zeroes accumulators (FWI_RESET_* bit mask).  The illumination is cumulative
+ * over every migration / gradient call until reset. */
+int fwi_reset(fwi_ctx *ctx, int32_t what);
+
+/* RTM output: image_out[nx][nz] = -d2/dz2 (image / illum), zero on the first
+ * and last rows; raw_image / raw_illum (cropped, may be NULL).  Replaces
+ * Migration::export_seismic (migration.cu:136-161). */
+int fwi_get_image(fwi_ctx *ctx, float *image_out, float *raw_image, float *raw_illum);
+
+/* FWI search direction: g = -d2/dz2 (grad / illum) below zmask, divided by
+ * max|g| (returned in *gmax).  raw_grad already carries the dt factor.
+ * Replaces the post-processing of Inversion::compute_gradient
+ * (inversion.cu:138-165). */
+int fwi_get_gradient(fwi_ctx *ctx, float zmask, float *grad_out, float *raw_grad,
+                     float *raw_illum, double *gmax);
+
+/* Parabola vertex through (alpha_k, f_k), k = 0..2, in FP64.  Replaces the
+ * step formula of Inversion::optimization (inversion.cu:304-307).  Returns
+ * FWI_ERR_NUMERIC for a degenerate parabola. */
+int fwi_parabola_step(const double *alpha, const double *f, double *step);
+
+/* Test hook: the random-boundary velocity [nxx][nzz] (rbc grid) that a pass
+ * of shot `shot` in FWI iteration `iteration` uses (migration: iteration 0). */
+int fwi_rb_velocity(fwi_ctx *ctx, int32_t shot, uint32_t iteration, float *vp_padded_out);
+
+/* Padded grid of a boundary kind: nzz, nxx, nb. */
+int fwi_grid_shape(fwi_ctx *ctx, int32_t boundary, int32_t *nzz, int32_t *nxx, int32_t *nb);
+
+/* Blocks until the context's stream is idle. */
+int fwi_synchronize(fwi_ctx *ctx);
+
+/* Launch tuning: cells per lane along z (1, 2 or 4), columns per wave and
+ * hipGraph capture of the time loop (1 on, 0 off).  0 / negative keeps the
+ * current value. */
+int fwi_set_tuning(fwi_ctx *ctx, int32_t vz, int32_t xw, int32_t use_graph);
+int fwi_get_tuning(fwi_ctx *ctx, int32_t *vz, int32_t *xw, int32_t *use_graph);
+
+/* Per-pass timing with HIP events recorded on the context stream around each
+ * time loop.  kind: 0 forward step, 1 reverse step.  Returns the summed GPU
+ * milliseconds, the number of step launches and the cell-updates counted
+ * (shots x nzz x nxx x steps, a reverse step counting 2 per cell). */
+int fwi_profile_enable(fwi_ctx *ctx, int32_t enable);
+int fwi_profile_read(fwi_ctx *ctx, int32_t kind, double *ms, int64_t *launches,
+                     double *cell_updates);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 CPU oracle.
+
+Tolerances (relative L2 against the FP64 oracle on identical inputs):
+  seismograms 1e-5, images / gradients / illumination 1e-4,
+  random-boundary velocity: bit-exact, launch-shape variants: bit-exact.
+"""
+import numpy as np
+import pytest
+
+import fwi2d_amd as F
+import oracle_bind as ob
+from cases import layered, overthrust, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL = 1e-5
+IMG_TOL = 1e-4
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+class Survey:
+    """A small survey run through both the engine and the oracle."""
+
+    def __init__(self, v, dh, nt, dt, fmax, xsrc, zsrc, xrec, zrec, abc_nb=0, abc_factor=0.0,
+                 rbc_nb=0, varVp=0.0, varVp_rel=0.0, seed=0, max_batch=1):
+        self.v = np.ascontiguousarray(v, np.float32)
+        self.nx, self.nz = v.shape
+        self.dh, self.dt, self.nt = f32(dh), f32(dt), nt
+        self.w, self.tlag = F.ricker_wavelet(nt, dt, fmax)
+        self.xsrc, self.zsrc = np.float32(xsrc), np.float32(zsrc)
+        self.xrec, self.zrec = np.float32(xrec), np.float32(zrec)
+        self.abc_nb, self.abc_factor, self.rbc_nb = abc_nb, abc_factor, rbc_nb
+        self.varVp, self.varVp_rel, self.seed = varVp, varVp_rel, seed
+        self.eng = F.Engine(self.nz, self.nx, dh, nt, dt, abc_nb, abc_factor, rbc_nb, varVp,
+                            varVp_rel, seed, max_batch)
+        self.eng.set_wavelet(self.w, self.tlag)
+        self.eng.set_geometry(self.xsrc, self.zsrc, self.xrec, self.zrec)
+        self.eng.set_model(self.v)
+
+    # ---- oracle side
+    def idx(self, nb):
+        gi = lambda x: ob.grid_index(float(x), self.dh, nb)
+        siz = np.array([gi(z) for z in self.zsrc], np.int32)
+        six = np.array([gi(x) for x in self.xsrc], np.int32)
+        riz = np.array([gi(z) for z in self.zrec], np.int32)
+        rix = np.array([gi(x) for x in self.xrec], np.int32)
+        return siz, six, riz, rix
+
+    def rb_oracle(self, shot, iteration=0):
+        vpad = ob.expand(self.v, self.rbc_nb)
+        one = np.float32(1.0)
+        vmin, vmax = self.v.min(), self.v.max()
+        rel, ab = np.float32(self.varVp_rel), np.float32(self.varVp)
+        vlo = vmin * (one - rel) - ab
+        vhi = vmax * (one + rel) + ab
+        vcfl = (np.float32(0.999) * np.float32(0.5546) * np.float32(self.dh)) / np.float32(self.dt)
+        vhi = min(vhi, vcfl)
+        stream = (int(iteration) << 32) | int(shot)
+        return ob.rb_velocity(vpad, self.rbc_nb, ab, rel, vlo, vhi, self.seed, stream)
+
+    def oracle_forward(self, shot, boundary, vpad=None, model=None):
+        nb = self.abc_nb if boundary == F.ABC else self.rbc_nb
+        siz, six, riz, rix = self.idx(nb)
+        if vpad is None:
+            vpad = ob.expand(self.v if model is None else model, nb)
+        b1d = ob.abc_b1d(nb, self.abc_factor) if boundary == F.ABC else None
+        return ob.forward(vpad, self.dh, self.dt, self.nt, self.tlag, self.w.astype(np.float64),
+                          int(siz[shot]), int(six[shot]), riz, rix, abc=(boundary == F.ABC),
+                          b1d=b1d, nb=nb)
+
+
+def overthrust_survey(nt=1001, nsrc=3, **kw):
+    v = overthrust()
+    xs = np.linspace(1000, 4000, nsrc)
+    xr = np.linspace(0, 5000, 51)
+    return Survey(v, 25.0, nt, 1e-3, 30.0, xs, np.full(nsrc, 100.0), xr, np.zeros(51), **kw)
+
+
+def test_forward_abc_matches_oracle():
+    s = overthrust_survey(abc_nb=51, abc_factor=0.004, max_batch=2)
+    seis = s.eng.modeling(F.ABC)
+    for k in range(3):
+        ref, _, _ = s.oracle_forward(k, F.ABC)
+        err = rel_l2(seis[k], ref)
+        print(f"shot {k} relL2 {err:.3e}")
+        assert err <= SEIS_TOL
+        assert np.all(seis[k][:, 0] == 0.0)
+
+
+def test_rb_velocity_bit_exact():
+    s = overthrust_survey(rbc_nb=41, varVp=1000.0, seed=1234)
+    for shot, it in [(0, 0), (2, 0), (1, 7)]:
+        gpu = s.eng.rb_velocity(shot, it)
+        ref = s.rb_oracle(shot, it)
+        assert gpu.shape == ref.shape
+        assert np.array_equal(gpu.view(np.uint32), ref.view(np.uint32)), (shot, it)
+    # different streams differ, interior untouched
+    a, b = s.eng.rb_velocity(0, 0), s.eng.rb_velocity(0, 1)
+    nb = 41
+    assert not np.array_equal(a, b)
+    assert np.array_equal(a[nb:-nb, nb:-nb], s.v)
+
+
+def test_forward_rbc_matches_oracle():
+    s = overthrust_survey(rbc_nb=41, varVp=1000.0, seed=99, max_batch=3)
+    seis = s.eng.modeling(F.RBC)
+    for k in range(3):
+        ref, _, _ = s.oracle_forward(k, F.RBC, vpad=s.rb_oracle(k))
+        err = rel_l2(seis[k], ref)
+        print(f"shot {k} relL2 {err:.3e}")
+        assert err <= SEIS_TOL
+
+
+def test_launch_shapes_bit_identical():
+    s = overthrust_survey(abc_nb=51, abc_factor=0.004, max_batch=3)
+    base = s.eng.modeling(F.ABC)
+    for vz, xw, g in [(1, 8, 0), (2, 17, 1), (4, 64, 1), (4, 1000, 0), (1, 3, 1)]:
+        s.eng.set_tuning(vz, xw, g)
+        out = s.eng.modeling(F.ABC)
+        assert np.array_equal(out, base), (vz, xw, g)
+
+
+def test_batching_bit_identical():
+    a = overthrust_survey(nsrc=5, rbc_nb=41, varVp=800.0, seed=5, max_batch=5)
+    b = overthrust_survey(nsrc=5, rbc_nb=41, varVp=800.0, seed=5, max_batch=2)
+    sa = a.eng.modeling(F.RBC)
+    sb = b.eng.modeling(F.RBC)
+    assert np.array_equal(sa, sb)
+    s3 = b.eng.modeling(F.RBC, shot0=3, nshots=1)
+    assert np.array_equal(s3[0], sa[3])
+
+
+def migration_oracle(s, obs):
+    nb = s.rbc_nb
+    siz, six, riz, rix = s.idx(nb)
+    nxx, nzz = s.nx + 2 * nb, s.nz + 2 * nb
+    acc = np.zeros((nxx, nzz))
+    ill = np.zeros((nxx, nzz))
+    for k in range(len(s.xsrc)):
+        vrb = s.rb_oracle(k)
+        _, P, Q = s.oracle_forward(k, F.RBC, vpad=vrb)
+        ob.reverse(vrb, s.dh, s.dt, s.nt, s.tlag, P, Q, riz, rix,
+                   obs[k].astype(np.float64), 0, acc=acc, illum=ill)
+    crop = lambda a: np.ascontiguousarray(a[nb:nb + s.nx, nb:nb + s.nz])
+    img, il = crop(acc), crop(ill)
+    return ob.image_post(img, il, s.dh), img, il
+
+
+def test_migration_matches_oracle():
+    s = overthrust_survey(nsrc=2, abc_nb=51, abc_factor=0.004, rbc_nb=41, varVp=1000.0,
+                          seed=3, max_batch=2)
+    obs = np.stack([s.oracle_forward(k, F.ABC)[0] for k in range(2)]).astype(np.float32)
+    s.eng.set_observed(obs)
+    s.eng.reset()
+    s.eng.migration()
+    img, raw, ill = s.eng.get_image()
+    ref, rraw, rill = migration_oracle(s, obs)
+    e1, e2, e3 = rel_l2(raw, rraw), rel_l2(ill, rill), rel_l2(img, ref)
+    print(f"raw image {e1:.3e} illum {e2:.3e} image {e3:.3e}")
+    assert e1 <= IMG_TOL and e2 <= IMG_TOL and e3 <= IMG_TOL
+
+
+def gradient_oracle(s, obs, model, iteration, zmask):
+    nb = s.rbc_nb
+    siz, six, riz, rix = s.idx(nb)
+    nxx, nzz = s.nx + 2 * nb, s.nz + 2 * nb
+    acc = np.zeros((nxx, nzz))
+    ill = np.zeros((nxx, nzz))
+    sumsq = 0.0
+    for k in range(len(s.xsrc)):
+        cal, _, _ = s.oracle_forward(k, F.ABC, model=model)
+        res = obs[k].astype(np.float64) - cal
+        sumsq += float(np.sum(res * res))
+        vrb = s.rb_oracle(k, iteration)
+        _, P, Q = s.oracle_forward(k, F.RBC, vpad=vrb)
+        ob.reverse(vrb, s.dh, s.dt, s.nt, s.tlag, P, Q, riz, rix, res, 1, acc=acc, illum=ill)
+    crop = lambda a: np.ascontiguousarray(a[nb:nb + s.nx, nb:nb + s.nz])
+    g, il = crop(acc), crop(ill)
+    return ob.gradient_post(g, il, s.dh, zmask), g, il, sumsq
+
+
+def test_gradient_and_misfit_match_oracle():
+    s = overthrust_survey(nsrc=2, abc_nb=51, abc_factor=0.004, rbc_nb=41, varVp=1000.0,
+                          seed=11, max_batch=2)
+    true_v = s.v.copy()
+    obs = np.stack([s.oracle_forward(k, F.ABC)[0] for k in range(2)]).astype(np.float32)
+    init = (true_v * 0.97).astype(np.float32)          # a perturbed starting model
+    s.v = init
+    s.eng.set_model(init)
+    s.eng.set_observed(obs)
+    s.eng.reset()
+    sumsq = s.eng.gradient(iteration=4)
+    g, raw, ill, gmax = s.eng.get_gradient(400.0)
+    rg, rraw, rill, rsum = gradient_oracle(s, obs, init, 4, 400.0)
+    e0 = abs(sumsq - rsum) / rsum
+    e1, e2, e3 = rel_l2(raw, rraw), rel_l2(ill, rill), rel_l2(g, rg)
+    print(f"misfit {e0:.3e} raw grad {e1:.3e} illum {e2:.3e} direction {e3:.3e}")
+    assert e0 <= 1e-5
+    assert e1 <= IMG_TOL and e2 <= IMG_TOL and e3 <= IMG_TOL
+    # misfit alone agrees with the gradient pass's misfit
+    assert abs(s.eng.misfit() - sumsq) <= 1e-12 * sumsq
+
+
+def test_config1_forward_full_size():
+    """121 x 401 at 10 m, 100-cell pad, nt=2001, dt=1 ms, 15 Hz peak (fmax 45)."""
+    v = layered(121, 401, [1500, 2000, 2500, 3000], anomaly=(15, 250.0), seed=1)
+    xr = np.arange(401) * 10.0
+    for kw, bnd in [(dict(abc_nb=100, abc_factor=0.004), F.ABC),
+                    (dict(rbc_nb=100, varVp_rel=0.4, seed=2024), F.RBC)]:
+        s = Survey(v, 10.0, 2001, 1e-3, 45.0, [2000.0], [20.0], xr, np.full(401, 10.0), **kw)
+        seis = s.eng.modeling(bnd)
+        vpad = s.rb_oracle(0) if bnd == F.RBC else None
+        ref, _, _ = s.oracle_forward(0, bnd, vpad=vpad)
+        err = rel_l2(seis[0], ref)
+        print(f"boundary {bnd} relL2 {err:.3e}")
+        assert err <= SEIS_TOL
+
+
+def test_errors_are_reported():
+    v = overthrust()
+    eng = F.Engine(81, 201, 25.0, 101, 1e-3, abc_nb=51, abc_factor=0.004, max_batch=1)
+    with pytest.raises(F.FwiError) as e:
+        eng.modeling(F.ABC)
+    assert e.value.code == 2          # state: nothing set yet
+    w, tl = F.ricker_wavelet(101, 1e-3, 30.0)
+    eng.set_wavelet(w, tl)
+    with pytest.raises(F.FwiError):   # receiver outside the padded grid
+        eng.set_geometry([100.0], [100.0], [1e6], [0.0])
+    with pytest.raises(F.FwiError) as e:   # CFL: 20 km/s at dt/dh = 1/25 ms/m
+        eng.set_model(np.full((201, 81), 20000.0, np.float32))
+    assert e.value.code == 5
+    with pytest.raises(F.FwiError) as e:
+        eng.set_model(np.full((201, 81), -1.0, np.float32))
+    assert e.value.code == 5
