@@ -39,15 +39,18 @@ def parse():
     p.add_argument("--shots", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
-    p.add_argument("--vz", type=int, default=0)
-    p.add_argument("--xw", type=int, default=0)
+    p.add_argument("--cpu-shots", type=int, default=8)
+    p.add_argument("--nt", type=int, default=0,
+                   help="override time_samples (short profiling runs; same per-step work)")
+    p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--graph", type=int, default=-1)
     return p.parse_args()
 
 
-def cpu_baseline(s, vmig, xs, zs, xr, zr, w, tlag, obs0, threads):
-    """FP32 CPU statement of the reference stencil (OpenMP), one shot of the
-    same workload: random-boundary forward + reverse imaging pass."""
+def cpu_baseline(s, vmig, xs, zs, xr, zr, w, tlag, obs, threads, nshots):
+    """FP32 CPU statement of the reference stencil (OpenMP) on the first
+    `nshots` shots of the same workload: random-boundary forward + reverse
+    imaging pass per shot."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     nb = s.nb
@@ -59,20 +62,27 @@ def cpu_baseline(s, vmig, xs, zs, xr, zr, w, tlag, obs0, threads):
     rel = np.float32(s.varVp_rel)
     vlo = vmig.min() * (one - rel) - np.float32(s.varVp)
     vhi = vmig.max() * (one + rel) + np.float32(s.varVp)
-    vrb = ob.rb_velocity(vpad, nb, np.float32(s.varVp), rel, vlo, vhi, s.seed, 0)
+    vcfl = np.float32(np.float32(0.999) * np.float32(0.5546) * np.float32(s.dh)) / np.float32(s.dt)
+    vhi = min(vhi, vcfl)
     T = s.nt + tlag
     nzz, nxx = s.nz + 2 * nb, s.nx + 2 * nb
-    t0 = time.perf_counter()
-    _, P, Q = ob.forward(vrb, s.dh, s.dt, s.nt, tlag, w, gi(zs[0]), gi(xs[0]), riz, rix,
-                         precision="f32", nthreads=threads)
-    ob.reverse(vrb, s.dh, s.dt, s.nt, tlag, P, Q, riz, rix, obs0, 0, precision="f32",
-               nthreads=threads)
-    el = time.perf_counter() - t0
-    cells = nzz * nxx * T * 3.0
+    acc = np.zeros((nxx, nzz), np.float32)
+    ill = np.zeros((nxx, nzz), np.float32)
+    el = 0.0
+    for k in range(nshots):
+        vrb = ob.rb_velocity(vpad, nb, np.float32(s.varVp), rel, vlo, vhi, s.seed, k)
+        t0 = time.perf_counter()
+        _, P, Q = ob.forward(vrb, s.dh, s.dt, s.nt, tlag, w, gi(zs[k]), gi(xs[k]), riz, rix,
+                             precision="f32", nthreads=threads)
+        ob.reverse(vrb, s.dh, s.dt, s.nt, tlag, P, Q, riz, rix, obs[k], 0, acc=acc, illum=ill,
+                   precision="f32", nthreads=threads)
+        el += time.perf_counter() - t0
+    cells = nshots * nzz * nxx * T * 3.0
     return {"value": round(cells / el / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"1 of {s.nsrc} shots, full RTM (RB forward + reverse imaging), "
-                      f"{nzz}x{nxx} cells x {T} steps, FP32, OpenMP, {el:.1f} s"}
+            "sample": f"{nshots} of {s.nsrc} shots, full RTM each (RB forward + reverse "
+                      f"imaging), {nzz}x{nxx} cells x {T} steps, FP32 OpenMP, "
+                      f"{el:.1f} s of CPU time"}
 
 
 def main():
@@ -86,6 +96,9 @@ def main():
     from fwi2d_amd import configs as C
 
     s = C.CONFIG2
+    if a.nt:
+        import dataclasses
+        s = dataclasses.replace(s, nt=a.nt)
     nshots = a.shots
     true_v, mig_v = C.config2_models()
     xs, zs, xr, zr = s.acquisition()
@@ -95,10 +108,11 @@ def main():
     w, tlag = F.ricker_wavelet(s.nt, s.dt, s.fmax)
     T = s.nt + tlag
 
+    copy_gbs = F.abi.stream_copy(1 << 30, reps=20)    # practical HBM ceiling (1 GiB copy)
     eng = F.Engine(s.nz, s.nx, s.dh, s.nt, s.dt, abc_nb=s.abc_nb, abc_factor=s.abc_factor,
                    rbc_nb=s.nb, rbc_varVp=s.varVp, rbc_varVp_rel=s.varVp_rel, rbc_seed=s.seed,
                    max_batch=nshots)
-    eng.set_tuning(a.vz, a.xw, a.graph)
+    eng.set_tuning(use_graph=a.graph)
     eng.set_wavelet(w, tlag)
     eng.set_geometry(xs, zs, xr, zr)
     # observed data: true model, sponge boundary, direct-wave mute (setup, untimed)
@@ -111,6 +125,8 @@ def main():
     eng.set_observed(obs)
     eng.set_model(mig_v)
     nzz, nxx, _ = eng.grid_shape(F.RBC)
+    if not a.no_autotune:
+        eng.autotune(nshots, reps=8)
 
     for _ in range(a.warmup):
         eng.reset(F.RESET_IMAGE | F.RESET_ILLUM)
@@ -139,14 +155,17 @@ def main():
         gbs = per_launch * bpc / avg_s / 1e9
         return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": per_launch * bpc,
                 "kernel_avg_us": round(avg_s * 1e6, 2),
-                "gcell_per_s": round(per_launch / avg_s / 1e9, 2)}
+                "gcell_per_s": round(per_launch / avg_s / 1e9, 2),
+                "frac_of_measured_copy": round(gbs / copy_gbs, 4)}
 
     rev_roof = roof(rev_ms, rev_n, rev_cells, BYTES_REV)
     fwd_roof = roof(fwd_ms, fwd_n, fwd_cells, BYTES_FWD)
     rev_roof["kernel"] = "rev_step_kernel (fused reverse + imaging)"
     fwd_roof["kernel"] = "fwd_step_kernel (random boundary)"
-    vz, xw, g = eng.get_tuning()
+    stf, vzf, xwf, g = eng.get_tuning(0)
+    str_, vzr, xwr, _ = eng.get_tuning(1)
     out = {
         "metric": "Gcell-updates/s", "value": round(value, 2), "unit": "Gcell-updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -156,15 +175,18 @@ def main():
         "config": {"workload": "batched RTM, 64 shots, 301x1001 grid + 150-cell random "
                                "boundary (601x1301), nt=4001+%d, dt=0.8 ms" % tlag,
                    "shots": nshots, "nzz": nzz, "nxx": nxx, "steps_per_pass": T,
-                   "tuning": {"vz": vz, "xw": xw, "graph": g}},
+                   "tuning": {"forward": {"steps_per_launch": stf, "vz": vzf, "xw": xwf},
+                              "reverse": {"steps_per_launch": str_, "vz": vzr, "xw": xwr},
+                              "graph": g},
+                   "measured_copy_GBs": round(copy_gbs, 1)},
         "roofline": rev_roof,
         "roofline_forward": fwd_roof,
     }
     eng.close()
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(s, mig_v, xs, zs, xr, zr, w, tlag,
-                                           obs[0].astype(np.float32), threads)
+        out["cpu_baseline"] = cpu_baseline(s, mig_v, xs, zs, xr, zr, w, tlag, obs, threads,
+                                           min(a.cpu_shots, nshots))
     print(json.dumps(out), flush=True)
 
 

@@ -73,9 +73,12 @@ SIGNATURES = [
     ("fwi_grid_shape", C.c_int, [_ctx, C.c_int32, C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("fwi_synchronize", C.c_int, [_ctx]),
-    ("fwi_set_tuning", C.c_int, [_ctx, C.c_int32, C.c_int32, C.c_int32]),
-    ("fwi_get_tuning", C.c_int, [_ctx, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                 C.POINTER(C.c_int32)]),
+    ("fwi_set_tuning", C.c_int, [_ctx, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    ("fwi_get_tuning", C.c_int, [_ctx, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("fwi_autotune", C.c_int, [_ctx, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)]),
+    ("fwi_stream_copy", C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     ("fwi_profile_enable", C.c_int, [_ctx, C.c_int32]),
     ("fwi_profile_read", C.c_int, [_ctx, C.c_int32, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
@@ -116,6 +119,13 @@ def ricker_wavelet(nt, dt, fmax):
     tl = C.c_int32(0)
     _check(lib().fwi_ricker_wavelet(nt, dt, fmax, w, C.byref(tl)))
     return w, tl.value
+
+
+def stream_copy(nbytes, reps=20):
+    """Device copy bandwidth in GB/s (read + write) of a plain float4 copy."""
+    g = C.c_double()
+    _check(lib().fwi_stream_copy(int(nbytes), reps, C.byref(g)))
+    return g.value
 
 
 def parabola_step(alpha, f):
@@ -228,13 +238,20 @@ class Engine:
     def synchronize(self):
         _check(lib().fwi_synchronize(self._h))
 
-    def set_tuning(self, vz=0, xw=0, use_graph=-1):
-        _check(lib().fwi_set_tuning(self._h, vz, xw, use_graph))
+    def set_tuning(self, vz=0, xw=0, use_graph=-1, kernel=-1, steps=0):
+        _check(lib().fwi_set_tuning(self._h, kernel, steps, vz, xw, use_graph))
 
-    def get_tuning(self):
-        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
-        _check(lib().fwi_get_tuning(self._h, C.byref(a), C.byref(b), C.byref(c)))
-        return a.value, b.value, c.value
+    def get_tuning(self, kernel=0):
+        """(steps per launch, rows per lane, chunk width, graph on)."""
+        st, a, b, c = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().fwi_get_tuning(self._h, kernel, C.byref(st), C.byref(a), C.byref(b),
+                                    C.byref(c)))
+        return st.value, a.value, b.value, c.value
+
+    def autotune(self, nslots, reps=10):
+        f, r = C.c_double(), C.c_double()
+        _check(lib().fwi_autotune(self._h, nslots, reps, C.byref(f), C.byref(r)))
+        return f.value, r.value
 
     def profile_enable(self, on=True):
         _check(lib().fwi_profile_enable(self._h, 1 if on else 0))

@@ -61,6 +61,24 @@ struct GridBuf {
     int *col_ptr = nullptr, *rec_iz = nullptr, *rec_id = nullptr;
 };
 
+// Four buffers of one wavefield: `cur` holds the latest step, `old` the one
+// before.  A single step writes over old; a double step writes the two free
+// buffers in turn.
+struct Quad {
+    float *b[4];
+    int cur, old;
+    void free_pair(int *f1, int *f2) const
+    {
+        int n = 0, f[2] = {0, 0};
+        for (int k = 0; k < 4; ++k)
+            if (k != cur && k != old) f[n++] = k;
+        *f1 = f[0];
+        *f2 = f[1];
+    }
+    void after_single() { std::swap(cur, old); }
+    void after_double(int f1, int f2) { old = f1; cur = f2; }
+};
+
 enum PassKind { PASS_FWD_ABC_REC = 0, PASS_FWD_RBC = 1, PASS_FWD_RBC_REC = 2,
                 PASS_REV_IMAGE = 3, PASS_REV_GRAD = 4, PASS_FWD_ABC = 5 };
 
@@ -80,7 +98,7 @@ struct fwi_ctx {
     float *d_b1d = nullptr;
     float *d_wav = nullptr;
     int *d_shot0 = nullptr;
-    float *d_f[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *d_f[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float *d_acc_slot = nullptr, *d_ill_slot = nullptr;
     float *d_image = nullptr, *d_grad = nullptr, *d_illum = nullptr;
     float *d_obs = nullptr;    // [nsrc][nrec][nt]
@@ -92,8 +110,11 @@ struct fwi_ctx {
     int nsrc = 0, nrec = 0, tlag = 0;
     bool have_wavelet = false, have_geometry = false, have_model = false;
     float vmin = 0.f, vmax = 0.f;
+    Quad fwd{};                // final (u_T, u_{T-1}) of the last forward pass
 
-    int vz = 4, xw = 32, use_graph = 1;
+    int vz[2] = {4, 2}, xw[2] = {32, 32};   // [0] forward kernel, [1] reverse kernel
+    int nsteps[2] = {2, 2};                 // time steps per launch
+    int use_graph = 1;
     std::map<long, hipGraphExec_t> graphs;
 
     bool profile = false;
@@ -107,12 +128,12 @@ namespace {
 
 int T_of(const fwi_ctx *c) { return c->cfg.nt + c->tlag; }
 
-fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g)
+fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g, int kind)
 {
     fwi::Tiling t;
-    t.vz = c->vz;
-    t.xw = c->xw;
-    t.nzseg = (g.ldz + 64 * t.vz - 1) / (64 * t.vz);
+    t.vz = c->vz[kind];
+    t.xw = c->xw[kind];
+    t.nzseg = fwi::zsegments(g.ldz, t.vz, c->nsteps[kind]);
     t.nxchunk = (g.nxx - 8 + t.xw - 1) / t.xw;
     return t;
 }
@@ -141,8 +162,6 @@ fwi::FwdStep fwd_args(fwi_ctx *c, int kind, bool record)
     const GridBuf &gb = c->grid[kind];
     fwi::FwdStep a{};
     a.g = gb.g;
-    a.fa = c->d_f[0];
-    a.fb = c->d_f[1];
     a.c = (kind == FWI_BOUNDARY_ABC) ? c->d_cabc : c->d_crb;
     a.cstride = (kind == FWI_BOUNDARY_ABC) ? 0 : gb.g.fstride;
     a.b1d = c->d_b1d;
@@ -159,20 +178,11 @@ fwi::FwdStep fwd_args(fwi_ctx *c, int kind, bool record)
     return a;
 }
 
-// Reverse-pass arguments: the forward pass left u_T and u_{T-1} in d_f[0..1].
 fwi::RevStep rev_args(fwi_ctx *c, int mode)
 {
     const GridBuf &gb = c->grid[FWI_BOUNDARY_RBC];
-    const int T = T_of(c);
     fwi::RevStep a{};
     a.g = gb.g;
-    // forward step t writes into (t&1 ? fa : fb): u_T sits in ((T-1)&1 ? f0 : f1)
-    float *uT = ((T - 1) & 1) ? c->d_f[0] : c->d_f[1];
-    float *uTm1 = ((T - 1) & 1) ? c->d_f[1] : c->d_f[0];
-    a.sa = uT;      // reverse step 0: cur = sb = u_{T-1}, old = sa = u_T
-    a.sb = uTm1;
-    a.ra = c->d_f[2];
-    a.rb = c->d_f[3];
     a.c = c->d_crb;
     a.cstride = gb.g.fstride;
     a.shot0 = c->d_shot0;
@@ -180,7 +190,7 @@ fwi::RevStep rev_args(fwi_ctx *c, int mode)
     a.data_by_shot = (mode == 0) ? 1 : 0;
     a.idh2 = c->idh2;
     a.nt = c->cfg.nt;
-    a.T = T;
+    a.T = T_of(c);
     a.rec = {gb.col_ptr, gb.rec_iz, gb.rec_id, c->nrec};
     a.acc = c->d_acc_slot;
     a.ill = c->d_ill_slot;
@@ -188,30 +198,76 @@ fwi::RevStep rev_args(fwi_ctx *c, int mode)
     return a;
 }
 
-// Enqueues the T step launches of one pass (or replays its captured graph).
-int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fwi::RevStep *ra)
+// Enqueues the launches of one pass over T steps -- two steps per launch
+// when the tuning asks for it, a single step for an odd remainder -- or
+// replays the captured graph of the same sequence.  `sq` is the source (or
+// forward) wavefield, `rq` the receiver wavefield of a reverse pass; both
+// are advanced to their final state.
+int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fwi::RevStep *ra,
+              Quad &sq, Quad *rq)
 {
     const int T = T_of(c);
     const fwi::Grid &g = fa ? fa->g : ra->g;
-    const fwi::Tiling tl = tiling_of(c, g);
-    auto enqueue = [&]() -> hipError_t {
-        for (int t = 0; t < T; ++t) {
-            hipError_t e = fa ? fwi::launch_fwd_step(*fa, tl, nslots, t, c->stream)
-                              : fwi::launch_rev_step(*ra, tl, nslots, t, c->stream);
+    const int kind = fa ? 0 : 1;
+    const fwi::Tiling tl = tiling_of(c, g, kind);
+    const int per = c->nsteps[kind];
+    Quad s0 = sq, r0 = rq ? *rq : sq;
+    int launches = 0;
+    auto enqueue = [&](bool launch) -> hipError_t {
+        Quad s = s0, r = r0;
+        launches = 0;
+        for (int t = 0; t < T;) {
+            const int n = (per == 2 && t + 2 <= T) ? 2 : 1;
+            int f1 = 0, f2 = 0, h1 = 0, h2 = 0;
+            s.free_pair(&f1, &f2);
+            r.free_pair(&h1, &h2);
+            hipError_t e = hipSuccess;
+            if (fa) {
+                fwi::FwdStep a = *fa;
+                a.cur = s.b[s.cur];
+                a.old = s.b[s.old];
+                a.out1 = (n == 2) ? s.b[f1] : s.b[s.old];
+                a.out2 = s.b[f2];
+                if (launch) e = fwi::launch_fwd_step(a, tl, nslots, t, n, c->stream);
+            } else {
+                fwi::RevStep a = *ra;
+                a.scur = s.b[s.cur];
+                a.sold = s.b[s.old];
+                a.sout1 = (n == 2) ? s.b[f1] : s.b[s.old];
+                a.sout2 = s.b[f2];
+                a.rcur = r.b[r.cur];
+                a.rold = r.b[r.old];
+                a.rout1 = (n == 2) ? r.b[h1] : r.b[r.old];
+                a.rout2 = r.b[h2];
+                if (launch) e = fwi::launch_rev_step(a, tl, nslots, t, n, c->stream);
+            }
             if (e != hipSuccess) return e;
+            if (n == 2) {
+                s.after_double(f1, f2);
+                r.after_double(h1, h2);
+            } else {
+                s.after_single();
+                r.after_single();
+            }
+            t += n;
+            ++launches;
         }
+        sq = s;
+        if (rq) *rq = r;
         return hipSuccess;
     };
-    const int kind = fa ? 0 : 1;
     if (c->profile) HIP_TRY(hipEventRecord(c->ev0, c->stream));
     if (c->use_graph) {
-        const long key = ((long)pass << 40) | ((long)nslots << 20) | ((long)c->vz << 16) | c->xw;
+        // the captured launches bake in the starting buffers of both wavefields
+        const long key = ((long)pass << 56) | ((long)nslots << 40) | ((long)per << 36) |
+                         ((long)tl.vz << 32) | ((long)tl.xw << 16) | ((long)s0.cur << 12) |
+                         ((long)s0.old << 8) | ((long)r0.cur << 4) | (long)r0.old;
         auto it = c->graphs.find(key);
         if (it == c->graphs.end()) {
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
             HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e = enqueue();
+            hipError_t e = enqueue(true);
             hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
             if (e != hipSuccess) return fail(FWI_ERR_HIP, "kernel launch during capture: %s",
                                              hipGetErrorString(e));
@@ -222,10 +278,12 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
             if (e != hipSuccess) return fail(FWI_ERR_HIP, "hipGraphInstantiate: %s",
                                              hipGetErrorString(e));
             it = c->graphs.emplace(key, exec).first;
+        } else {
+            (void)enqueue(false);      // advance the buffer bookkeeping only
         }
         HIP_TRY(hipGraphLaunch(it->second, c->stream));
     } else {
-        hipError_t e = enqueue();
+        hipError_t e = enqueue(true);
         if (e != hipSuccess) return fail(FWI_ERR_HIP, "step launch: %s", hipGetErrorString(e));
     }
     if (c->profile) {
@@ -234,10 +292,29 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         c->prof_ms[kind] += ms;
-        c->prof_launches[kind] += T;
+        c->prof_launches[kind] += launches;
         c->prof_cells[kind] += (double)nslots * g.nzz * g.nxx * T * (kind ? 2.0 : 1.0);
     }
     return FWI_OK;
+}
+
+// Clamp range of the random boundary: [vmin (1 - rel) - abs, vmax (1 + rel) + abs],
+// the upper end capped below the CFL limit.  One rounding per operation (no
+// FMA contraction) so the bounds are reproducible by a plain C statement.
+void rb_bounds(const fwi_ctx *c, float *vlo, float *vhi)
+{
+#pragma clang fp contract(off)
+    const fwi_config &k = c->cfg;
+    const float lo_scale = 1.0f - k.rbc_varVp_rel;
+    const float hi_scale = 1.0f + k.rbc_varVp_rel;
+    const float lo = c->vmin * lo_scale;
+    const float hi = c->vmax * hi_scale;
+    *vlo = lo - k.rbc_varVp;
+    *vhi = hi + k.rbc_varVp;
+    const float a = 0.999f * kCflLimit;
+    const float b = a * k.dh;
+    const float vcfl = b / k.dt;
+    if (*vhi > vcfl) *vhi = vcfl;
 }
 
 int check_ready(fwi_ctx *c, bool need_obs = false)
@@ -264,7 +341,8 @@ int set_shot0(fwi_ctx *c, int shot0)
     return FWI_OK;
 }
 
-// Forward pass of one batch; fields end in d_f[0..1] (u_T, u_{T-1}).
+// Forward pass of one batch.  The wavefield lives in d_f[0..3]; its final
+// pair (u_T, u_{T-1}) is left in c->fwd (used by the reverse pass).
 int forward_batch(fwi_ctx *c, int kind, int shot0, int nslots, bool record, uint32_t iteration)
 {
     GridBuf &gb = c->grid[kind];
@@ -272,23 +350,25 @@ int forward_batch(fwi_ctx *c, int kind, int shot0, int nslots, bool record, uint
     const long nf = (long)nslots * gb.g.fstride;
     if (kind == FWI_BOUNDARY_RBC) {
         const fwi_config &k = c->cfg;
-        float vlo = c->vmin * (1.0f - k.rbc_varVp_rel) - k.rbc_varVp;
-        float vhi = c->vmax * (1.0f + k.rbc_varVp_rel) + k.rbc_varVp;
-        const float vcfl = 0.999f * kCflLimit * k.dh / k.dt;
-        if (vhi > vcfl) vhi = vcfl;
+        float vlo, vhi;
+        rb_bounds(c, &vlo, &vhi);
         HIP_TRY(fwi::launch_rb(gb.vpad, gb.g, nslots, c->d_shot0, iteration, k.rbc_seed,
                                k.rbc_varVp, k.rbc_varVp_rel, vlo, vhi, c->kc, c->d_crb, nullptr,
                                c->stream));
     }
-    HIP_TRY(fwi::launch_fill(c->d_f[0], nf, 0.f, c->stream));
-    HIP_TRY(fwi::launch_fill(c->d_f[1], nf, 0.f, c->stream));
+    // all four buffers start at zero in this grid's layout (rim and padding
+    // cells are never written and must read as 0)
+    for (int k = 0; k < 4; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
     fwi::FwdStep a = fwd_args(c, kind, record);
-    HIP_TRY(fwi::launch_fwd_init(a, nslots, c->stream));
+    Quad q{{c->d_f[0], c->d_f[1], c->d_f[2], c->d_f[3]}, 1, 0};   // u_0 in b[1], u_{-1} = 0
+    HIP_TRY(fwi::launch_fwd_init(a, q.b[q.cur], nslots, c->stream));
     if (record)
         HIP_TRY(fwi::launch_zero_sample0(c->d_seis, c->nrec, c->cfg.nt, nslots, c->stream));
     int pass = (kind == FWI_BOUNDARY_ABC) ? (record ? PASS_FWD_ABC_REC : PASS_FWD_ABC)
                                           : (record ? PASS_FWD_RBC_REC : PASS_FWD_RBC);
-    return run_steps(c, pass, nslots, &a, nullptr);
+    TRY(run_steps(c, pass, nslots, &a, nullptr, q, nullptr));
+    c->fwd = q;
+    return FWI_OK;
 }
 
 // Reverse pass of one batch after an rbc forward pass; mode 0 image, 1 gradient.
@@ -297,11 +377,14 @@ int reverse_batch(fwi_ctx *c, int shot0, int nslots, int mode)
     GridBuf &gb = c->grid[FWI_BOUNDARY_RBC];
     TRY(set_shot0(c, shot0));
     const long nf = (long)nslots * gb.g.fstride;
-    HIP_TRY(fwi::launch_fill(c->d_f[2], nf, 0.f, c->stream));
-    HIP_TRY(fwi::launch_fill(c->d_f[3], nf, 0.f, c->stream));
+    for (int k = 4; k < 8; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
     fwi::RevStep a = rev_args(c, mode);
-    HIP_TRY(fwi::launch_rev_init(a, nslots, c->stream));
-    TRY(run_steps(c, mode == 0 ? PASS_REV_IMAGE : PASS_REV_GRAD, nslots, nullptr, &a));
+    // source wavefield: cur = u_{T-1}, old = u_T (the forward's old / cur)
+    Quad sq = c->fwd;
+    std::swap(sq.cur, sq.old);
+    Quad rq{{c->d_f[4], c->d_f[5], c->d_f[6], c->d_f[7]}, 1, 0};  // R_0 in b[1], R_{-1} = 0
+    HIP_TRY(fwi::launch_rev_init(a, rq.b[rq.cur], nslots, c->stream));
+    TRY(run_steps(c, mode == 0 ? PASS_REV_IMAGE : PASS_REV_GRAD, nslots, nullptr, &a, sq, &rq));
     float *dst = (mode == 0) ? c->d_image : c->d_grad;
     HIP_TRY(fwi::launch_reduce_slots(c->d_acc_slot, gb.g.fstride, gb.g.fstride, nslots, dst,
                                      c->stream));
@@ -383,7 +466,7 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     if (r == FWI_OK) r = dalloc(&c->d_tmp, n_int);
     if (r == FWI_OK) r = dalloc(&c->d_shot0, 1);
     if (r == FWI_OK) r = dalloc(&c->d_wav, cfg->nt);
-    for (int k = 0; k < 4 && r == FWI_OK; ++k) r = dalloc(&c->d_f[k], S * c->fstride_max);
+    for (int k = 0; k < 8 && r == FWI_OK; ++k) r = dalloc(&c->d_f[k], S * c->fstride_max);
     for (int kind = 0; kind < 2 && r == FWI_OK; ++kind)
         if (c->grid[kind].on) r = dalloc(&c->grid[kind].vpad, c->grid[kind].g.fstride);
     if (r == FWI_OK && c->grid[0].on) {
@@ -407,9 +490,12 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     if (r == FWI_OK) r = dalloc(&c->d_partials, c->npartials);
     if (r == FWI_OK) r = dalloc(&c->d_sum, 1);
     if (r != FWI_OK) { fwi_destroy(c); return r; }
-    // default launch shape: 4 rows per lane on tall grids, 32-column chunks
-    c->vz = (c->fstride_max / std::max(1, cfg->nx) > 192) ? 4 : 2;
-    c->xw = 32;
+    // default launch shape (the autotuner refines it): two steps per launch,
+    // 2 rows per lane forward, 1 reverse (register pressure), 64-column chunks
+    c->nsteps[0] = c->nsteps[1] = 2;
+    c->vz[0] = 2;
+    c->vz[1] = 1;
+    c->xw[0] = c->xw[1] = 64;
     *out = c;
     return FWI_OK;
 }
@@ -420,7 +506,8 @@ int fwi_destroy(fwi_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
     void *bufs[] = {c->d_vp, c->d_cabc, c->d_crb, c->d_b1d, c->d_wav, c->d_shot0, c->d_f[0],
-                    c->d_f[1], c->d_f[2], c->d_f[3], c->d_acc_slot, c->d_ill_slot, c->d_image,
+                    c->d_f[1], c->d_f[2], c->d_f[3], c->d_f[4], c->d_f[5], c->d_f[6], c->d_f[7],
+                    c->d_acc_slot, c->d_ill_slot, c->d_image,
                     c->d_grad, c->d_illum, c->d_obs, c->d_seis, c->d_partials, c->d_sum, c->d_tmp};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
@@ -718,10 +805,8 @@ int fwi_rb_velocity(fwi_ctx *c, int32_t shot, uint32_t iteration, float *out)
     const long ncell = (long)gb.g.nzz * gb.g.nxx;
     TRY(set_shot0(c, shot));
     const fwi_config &k = c->cfg;
-    float vlo = c->vmin * (1.0f - k.rbc_varVp_rel) - k.rbc_varVp;
-    float vhi = c->vmax * (1.0f + k.rbc_varVp_rel) + k.rbc_varVp;
-    const float vcfl = 0.999f * kCflLimit * k.dh / k.dt;
-    if (vhi > vcfl) vhi = vcfl;
+    float vlo, vhi;
+    rb_bounds(c, &vlo, &vhi);
     float *d_out = nullptr;
     TRY(dalloc(&d_out, ncell));
     hipError_t e = fwi::launch_rb(gb.vpad, gb.g, 1, c->d_shot0, iteration, k.rbc_seed,
@@ -751,24 +836,130 @@ int fwi_synchronize(fwi_ctx *c)
     return FWI_OK;
 }
 
-int fwi_set_tuning(fwi_ctx *c, int32_t vz, int32_t xw, int32_t use_graph)
+int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_t xw,
+                   int32_t use_graph)
 {
     if (!c) return fail(FWI_ERR_ARG, "null context");
-    if (vz > 0) {
-        if (vz != 1 && vz != 2 && vz != 4) return fail(FWI_ERR_ARG, "vz must be 1, 2 or 4");
-        c->vz = vz;
+    if (kernel < -1 || kernel > 1) return fail(FWI_ERR_ARG, "kernel must be -1, 0 or 1");
+    if (vz > 0 && vz != 1 && vz != 2 && vz != 4) return fail(FWI_ERR_ARG, "vz must be 1, 2 or 4");
+    if (steps > 2) return fail(FWI_ERR_ARG, "steps per launch must be 1 or 2");
+    for (int k = 0; k < 2; ++k) {
+        if (kernel != -1 && kernel != k) continue;
+        if (steps > 0) c->nsteps[k] = steps;
+        if (vz > 0) c->vz[k] = vz;
+        if (xw > 0) c->xw[k] = xw;
     }
-    if (xw > 0) c->xw = xw;
     if (use_graph >= 0) c->use_graph = use_graph ? 1 : 0;
+    drop_graphs(c);
     return FWI_OK;
 }
 
-int fwi_get_tuning(fwi_ctx *c, int32_t *vz, int32_t *xw, int32_t *use_graph)
+int fwi_get_tuning(fwi_ctx *c, int32_t kernel, int32_t *steps, int32_t *vz, int32_t *xw,
+                   int32_t *use_graph)
 {
     if (!c) return fail(FWI_ERR_ARG, "null context");
-    if (vz) *vz = c->vz;
-    if (xw) *xw = c->xw;
+    if (kernel < 0 || kernel > 1) return fail(FWI_ERR_ARG, "kernel must be 0 or 1");
+    if (steps) *steps = c->nsteps[kernel];
+    if (vz) *vz = c->vz[kernel];
+    if (xw) *xw = c->xw[kernel];
     if (use_graph) *use_graph = c->use_graph;
+    return FWI_OK;
+}
+
+// Times `reps` launches of every candidate launch shape (steps per launch,
+// rows per lane, chunk width) of the forward and the reverse kernels on the
+// random-boundary grid with nslots slots and keeps the fastest per time step.
+// The launches use step index T (no injection, no sampling) on zeroed
+// fields, and everything is zeroed again afterwards, so no pass state changes.
+int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, double *rev_us)
+{
+    TRY(check_ready(c));
+    if (!c->grid[1].on) return fail(FWI_ERR_ARG, "autotune runs on the random-boundary grid");
+    if (nslots < 1 || nslots > c->cfg.max_batch) return fail(FWI_ERR_ARG, "bad slot count");
+    if (reps < 1) reps = 1;
+    const GridBuf &gb = c->grid[1];
+    const int T = T_of(c);
+    const long nf = (long)nslots * gb.g.fstride;
+    for (int k = 0; k < 8; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
+    TRY(set_shot0(c, 0));
+    const int vzs[3] = {1, 2, 4};
+    const int xws[5] = {16, 32, 64, 96, 128};
+    double best[2] = {1e30, 1e30};
+    for (int kind = 0; kind < 2; ++kind) {
+        int bst = c->nsteps[kind], bvz = c->vz[kind], bxw = c->xw[kind];
+        for (int st = 1; st <= 2; ++st)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 5; ++b) {
+                    c->nsteps[kind] = st;
+                    c->vz[kind] = vzs[a];
+                    c->xw[kind] = xws[b];
+                    const fwi::Tiling tl = tiling_of(c, gb.g, kind);
+                    fwi::FwdStep fa = fwd_args(c, FWI_BOUNDARY_RBC, false);
+                    fa.cur = c->d_f[1]; fa.old = c->d_f[0];
+                    fa.out1 = (st == 2) ? c->d_f[2] : c->d_f[0]; fa.out2 = c->d_f[3];
+                    fwi::RevStep ra = rev_args(c, 0);
+                    ra.scur = c->d_f[1]; ra.sold = c->d_f[0];
+                    ra.sout1 = (st == 2) ? c->d_f[2] : c->d_f[0]; ra.sout2 = c->d_f[3];
+                    ra.rcur = c->d_f[5]; ra.rold = c->d_f[4];
+                    ra.rout1 = (st == 2) ? c->d_f[6] : c->d_f[4]; ra.rout2 = c->d_f[7];
+                    auto launch = [&]() {
+                        return kind == 0 ? fwi::launch_fwd_step(fa, tl, nslots, T, st, c->stream)
+                                         : fwi::launch_rev_step(ra, tl, nslots, T, st, c->stream);
+                    };
+                    for (int w = 0; w < 2; ++w) HIP_TRY(launch());
+                    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+                    for (int r = 0; r < reps; ++r) HIP_TRY(launch());
+                    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+                    HIP_TRY(hipEventSynchronize(c->ev1));
+                    float ms = 0.f;
+                    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                    const double us_per_step = 1e3 * ms / reps / st;
+                    if (us_per_step < best[kind]) {
+                        best[kind] = us_per_step;
+                        bst = st;
+                        bvz = vzs[a];
+                        bxw = xws[b];
+                    }
+                }
+        c->nsteps[kind] = bst;
+        c->vz[kind] = bvz;
+        c->xw[kind] = bxw;
+    }
+    drop_graphs(c);
+    for (int k = 0; k < 8; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
+    HIP_TRY(fwi::launch_fill(c->d_acc_slot, nf, 0.f, c->stream));
+    HIP_TRY(fwi::launch_fill(c->d_ill_slot, nf, 0.f, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (fwd_us) *fwd_us = best[0];
+    if (rev_us) *rev_us = best[1];
+    return FWI_OK;
+}
+
+// Device copy bandwidth of a plain float4 copy kernel over nbytes.
+int fwi_stream_copy(int64_t nbytes, int32_t reps, double *gbs)
+{
+    if (nbytes < 1024 || reps < 1 || !gbs) return fail(FWI_ERR_ARG, "bad arguments");
+    float *a = nullptr, *b = nullptr;
+    const long n = (long)(nbytes / 16) * 4;
+    TRY(dalloc(&a, n));
+    int r = dalloc(&b, n);
+    if (r != FWI_OK) { (void)hipFree(a); return r; }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    hipError_t e = fwi::launch_copy(a, b, n, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int k = 0; k < reps && e == hipSuccess; ++k) e = fwi::launch_copy(a, b, n, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(FWI_ERR_HIP, "stream copy: %s", hipGetErrorString(e));
+    *gbs = 2.0 * n * 4.0 * reps / (ms * 1e-3) / 1e9;
     return FWI_OK;
 }
 

@@ -19,8 +19,9 @@ struct Grid {
     long fstride;      // floats per field per shot slot
 };
 
-// Wave tiling of one time step: each 64-lane wave owns 64*vz rows of one
-// x-chunk of xw columns of one shot slot.
+// Wave tiling of one launch: each 64-lane wave owns 64*vz rows of one x-chunk
+// of xw columns of one shot slot (in a two-step launch 64*vz rows are
+// computed and the inner 64*vz - 8 are written).
 struct Tiling {
     int vz;            // rows per lane: 1, 2 or 4
     int xw;            // columns per wave
@@ -35,9 +36,14 @@ struct RecTable {
     int nrec;
 };
 
+// One forward launch at step t.  Single step: reads cur = u_t and
+// old = u_{t-1}, writes u_{t+1} over old (out1 == old).  Two steps: reads
+// cur, old; writes u_{t+1} to out1 and u_{t+2} to out2 (four distinct buffers).
 struct FwdStep {
     Grid g;
-    float *fa, *fb;            // field buffers; step t reads cur = (t&1 ? fb : fa)
+    const float *cur;
+    const float *old;
+    float *out1, *out2;
     const float *c;            // dt^2 v^2 / dh^2
     long cstride;              // per-slot stride of c (0 = shared by all slots)
     const float *b1d;          // sponge table [nb] (abc only)
@@ -50,10 +56,15 @@ struct FwdStep {
     float *seis;               // [slot][nrec][nt] or nullptr
 };
 
+// One reverse launch at reverse step k.  S: cur = u_{T-1-k}, old = u_{T-k};
+// R: cur = R_k, old = R_{k-1}.  Single step writes over the olds; two steps
+// write steps k+1 / k+2 to the out buffers.
 struct RevStep {
     Grid g;
-    float *sa, *sb;            // source wavefield pair; step k: cur = (k&1 ? sa : sb)
-    float *ra, *rb;            // receiver wavefield pair; step k: cur = (k&1 ? rb : ra)
+    const float *scur, *sold;
+    float *sout1, *sout2;
+    const float *rcur, *rold;
+    float *rout1, *rout2;
     const float *c;
     long cstride;
     const int *shot0;
@@ -66,17 +77,20 @@ struct RevStep {
     int mode;                  // 0 image, 1 gradient
 };
 
-hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t,
+// steps = 1 or 2 time steps per launch
+hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
                            hipStream_t s);
-hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k,
+hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
                            hipStream_t s);
+// z segments of a launch shape (two-step launches write 64*vz - 8 rows per wave)
+int zsegments(int ldz, int vz, int steps);
 
-// Adds idh2*w[0] at each slot's source cell of field fa (before step 0).
-hipError_t launch_fwd_init(const FwdStep &a, int nslots, hipStream_t s);
-// Adds idh2*data[nt-1] at the receivers of field ra (before reverse step 0).
-hipError_t launch_rev_init(const RevStep &a, int nslots, hipStream_t s);
+// Adds idh2*w[0] at each slot's source cell of `field` (before step 0).
+hipError_t launch_fwd_init(const FwdStep &a, float *field, int nslots, hipStream_t s);
+// Adds idh2*data[nt-1] at the receivers of `field` (before reverse step 0).
+hipError_t launch_rev_init(const RevStep &a, float *field, int nslots, hipStream_t s);
 
-// Zero-fill of n floats (graph-safe replacement for memset).
+// Fill of n floats (graph-safe replacement for memset).
 hipError_t launch_fill(float *p, long n, float v, hipStream_t s);
 hipError_t launch_set_int(int *p, int v, hipStream_t s);
 
@@ -100,6 +114,8 @@ hipError_t launch_reduce_slots(float *src, long fstride, long ncell, int nslots,
 // Crop of a padded field into [nx][nz].
 hipError_t launch_crop(const float *src, const Grid &g, int nz, int nx, float *dst,
                        hipStream_t s);
+// dst = src over n floats (n a multiple of 4), float4 per lane.
+hipError_t launch_copy(const float *src, float *dst, long n, hipStream_t s);
 // Sets seis[slot][r][0] = 0 for all slots.
 hipError_t launch_zero_sample0(float *seis, int nrec, int nt, int nslots, hipStream_t s);
 

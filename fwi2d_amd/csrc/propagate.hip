@@ -149,11 +149,85 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
     return logical < ntiles ? logical : -1;
 }
 
+// z-extension of a field that exists only in registers (the inner field of a
+// two-step launch): rows outside the wave are never used for a written cell,
+// so the wrapped shuffle values stand in for them.
+template <int VZ>
+__device__ __forceinline__ void zext_local(float (&ext)[VZ + 8], const float (&own)[VZ], int lane)
+{
+#pragma unroll
+    for (int r = 0; r < VZ; ++r) ext[4 + r] = own[r];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k = kk - 4;
+        const int o = -((-k + VZ - 1) / VZ);
+        ext[kk] = __shfl(own[k - o * VZ], lane + o, 64);
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k = VZ + kk;
+        const int o = k / VZ;
+        ext[VZ + 4 + kk] = __shfl(own[k - o * VZ], lane + o, 64);
+    }
+}
+
+// Column `col` of a slot field into registers; zero outside the grid.
+template <int VZ>
+__device__ __forceinline__ void ldcol(float (&r)[VZ], const float *base, int col, int nxx,
+                                      int ldz, int zb, bool active)
+{
+    if (active && col >= 0 && col < nxx) vload<VZ>(r, base + (long)col * ldz + zb);
+    else vzero<VZ>(r);
+}
+
+// One leapfrog update, u_new = d ((p + (p - d o)) + c lap)  (d = 1 without sponge).
+template <bool ABC>
+__device__ __forceinline__ float leapfrog(float p, float o, float c, float lap, float d)
+{
+    if constexpr (ABC) return d * ((p + (p - d * o)) + c * lap);
+    else return (p + (p - o)) + c * lap;
+}
+
+// Receivers in column j whose row is in this lane: seis[rec][tsamp] = scale * v[row].
+template <int VZ, bool ABC>
+__device__ __forceinline__ void sample_receivers(const FwdStep &a, const float (&v)[VZ], int j,
+                                                 int zb, unsigned own_mask, float *seis,
+                                                 int tsamp)
+{
+    const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
+    for (int k = k0; k < k1; ++k) {
+        const int rr = a.rec.iz[k] - zb;
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) {
+            if (rr == r && ((own_mask >> r) & 1u)) {
+                float val = v[r];
+                if constexpr (ABC) val *= damper(a.b1d, zb + r, j, a.g);
+                seis[(long)a.rec.id[k] * a.nt + tsamp] = val;
+            }
+        }
+    }
+}
+
+// Adds scale * data[rec][trow] to the receivers of column j in this lane.
+template <int VZ>
+__device__ __forceinline__ void inject_receivers(const RevStep &a, float (&v)[VZ], int j, int zb,
+                                                 const float *dat, int trow)
+{
+    const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
+    for (int m = k0; m < k1; ++m) {
+        const int rr = a.rec.iz[m] - zb;
+#pragma unroll
+        for (int r = 0; r < VZ; ++r)
+            if (rr == r) v[r] += a.idh2 * dat[(long)a.rec.id[m] * a.nt + trow];
+    }
+}
+
 // ---------------------------------------------------------------- forward
 
+// One time step: u_{t+1} = leapfrog(u_t, u_{t-1}) written over u_{t-1}.
 template <int VZ, bool ABC>
-__global__ __launch_bounds__(64) void fwd_step_kernel(FwdStep a, int nzseg, int nxchunk,
-                                                      int xw, int nslots, int t)
+__global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int t)
 {
     const int tile = xcd_tile(nzseg * nxchunk * nslots);
     if (tile < 0) return;
@@ -166,76 +240,50 @@ __global__ __launch_bounds__(64) void fwd_step_kernel(FwdStep a, int nzseg, int 
     const bool active = zb < g.ldz;
     const int j0 = 4 + xc * xw;
     const int j1 = min(j0 + xw, g.nxx - 4);
+    const int ldz = g.ldz, nxx = g.nxx;
 
     const long soff = (long)slot * g.fstride;
-    const float *cur = ((t & 1) ? a.fb : a.fa) + soff;
-    float *old = ((t & 1) ? a.fa : a.fb) + soff;
+    const float *cur = a.cur + soff;
+    const float *old = a.old + soff;
+    float *out = a.out1 + soff;
     const float *cc = a.c + (long)slot * a.cstride;
     const int gshot = *a.shot0 + slot;
     const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
     const float wsrc = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
     const bool rec_on = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
-    const int tsamp = t + 1 - a.tlag;
     float *seis = rec_on ? a.seis + (long)slot * a.rec.nrec * a.nt : nullptr;
-    const int ldz = g.ldz;
+    const unsigned own_mask = (1u << VZ) - 1u;
 
     float q[9][VZ];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        if (active) vload<VZ>(q[k], cur + (long)(j0 - 4 + k) * ldz + zb);
-        else vzero<VZ>(q[k]);
-    }
+    for (int k = 0; k < 9; ++k) ldcol<VZ>(q[k], cur, j0 - 4 + k, nxx, ldz, zb, active);
     float on[VZ], cn[VZ], pn[VZ];
-    vzero<VZ>(on); vzero<VZ>(cn); vzero<VZ>(pn);
-    if (active) {
-        vload<VZ>(on, old + (long)j0 * ldz + zb);
-        vload<VZ>(cn, cc + (long)j0 * ldz + zb);
-    }
+    ldcol<VZ>(on, old, j0, nxx, ldz, zb, active);
+    ldcol<VZ>(cn, cc, j0, nxx, ldz, zb, active);
+    vzero<VZ>(pn);
 
     for (int j = j0; j < j1; ++j) {
         float o[VZ], c[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o[r] = on[r]; c[r] = cn[r]; }
-        if (active && j + 1 < j1) {           // prefetch the next column
-            vload<VZ>(pn, cur + (long)(j + 5) * ldz + zb);
-            vload<VZ>(on, old + (long)(j + 1) * ldz + zb);
-            vload<VZ>(cn, cc + (long)(j + 1) * ldz + zb);
+        if (j + 1 < j1) {                      // prefetch the next column
+            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb, active);
+            ldcol<VZ>(on, old, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
         }
         float ext[VZ + 8];
         zext<VZ>(ext, q[4], cur + (long)j * ldz, zb, lane, ldz);
-
         float nv[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) {
             const int i = zb + r;
-            const float p = q[4][r];
-            const float lap = lap8<VZ>(q, ext, r);
-            float val;
-            if constexpr (ABC) {
-                const float d = damper(a.b1d, i, j, g);
-                val = d * ((p + (p - d * o[r])) + c[r] * lap);
-            } else {
-                val = (p + (p - o[r])) + c[r] * lap;
-            }
+            const float d = ABC ? damper(a.b1d, i, j, g) : 1.f;
+            float val = leapfrog<ABC>(q[4][r], o[r], c[r], lap8<VZ>(q, ext, r), d);
             if (i == siz && j == six) val += wsrc;
             nv[r] = (i >= 4 && i < g.nzz - 4) ? val : o[r];
         }
-        if (active) vstore<VZ>(old + (long)j * ldz + zb, nv);
-
-        if (rec_on) {
-            const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
-            for (int k = k0; k < k1; ++k) {
-                const int rr = a.rec.iz[k] - zb;
-#pragma unroll
-                for (int r = 0; r < VZ; ++r) {
-                    if (rr == r) {
-                        float val = nv[r];
-                        if constexpr (ABC) val *= damper(a.b1d, zb + r, j, g);
-                        seis[(long)a.rec.id[k] * a.nt + tsamp] = val;
-                    }
-                }
-            }
-        }
+        if (active) vstore<VZ>(out + (long)j * ldz + zb, nv);
+        if (rec_on) sample_receivers<VZ, ABC>(a, nv, j, zb, own_mask, seis, t + 1 - a.tlag);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
 #pragma unroll
@@ -245,11 +293,149 @@ __global__ __launch_bounds__(64) void fwd_step_kernel(FwdStep a, int nzseg, int 
     }
 }
 
+// Two time steps per launch (temporal blocking).  The wave computes u_{t+1}
+// on 64*VZ rows and on its x-chunk plus 4 columns each side (kept in a second
+// register queue), then u_{t+2} on the inner 64*VZ - 8 rows of the chunk.
+// Reads u_t, u_{t-1} and c once for two steps; writes u_{t+1} (out1) and
+// u_{t+2} (out2) for the cells it owns.  All four buffers are distinct, so
+// no wave ever reads a value another wave of the same launch writes.
+template <int VZ, bool ABC>
+__global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int t)
+{
+    const int tile = xcd_tile(nzseg * nxchunk * nslots);
+    if (tile < 0) return;
+    const int zseg = tile % nzseg;
+    const int xc = (tile / nzseg) % nxchunk;
+    const int slot = tile / (nzseg * nxchunk);
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int ldz = g.ldz, nxx = g.nxx;
+    const int zs = zseg * (64 * VZ - 8) - 4;     // first computed row
+    const int zb = zs + lane * VZ;
+    const bool active = (zb >= 0) && (zb < ldz);
+    unsigned own_mask = 0u;                      // rows this lane writes
+#pragma unroll
+    for (int r = 0; r < VZ; ++r) {
+        const int i = zb + r;
+        if (i >= zs + 4 && i < zs + 64 * VZ - 4 && i >= 0 && i < ldz) own_mask |= 1u << r;
+    }
+    const int x0 = 4 + xc * xw;
+    const int x1 = min(x0 + xw, nxx - 4);
+
+    const long soff = (long)slot * g.fstride;
+    const float *cur = a.cur + soff;
+    const float *old = a.old + soff;
+    float *out1 = a.out1 + soff;
+    float *out2 = a.out2 + soff;
+    const float *cc = a.c + (long)slot * a.cstride;
+    const int gshot = *a.shot0 + slot;
+    const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
+    const float w1 = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
+    const float w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
+    const bool rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
+    float *seis = (rec1 || rec2) ? a.seis + (long)slot * a.rec.nrec * a.nt : nullptr;
+
+    float q1[9][VZ];     // u_t, columns j-4 .. j+4
+    float q2[9][VZ];     // u_{t+1}, columns j-8 .. j
+    float cq[5][VZ];     // c, columns j-4 .. j
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        ldcol<VZ>(q1[k], cur, x0 - 8 + k, nxx, ldz, zb, active);
+        vzero<VZ>(q2[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) vzero<VZ>(cq[k]);
+    float on[VZ], cn[VZ], pn[VZ];
+    ldcol<VZ>(on, old, x0 - 4, nxx, ldz, zb, active);
+    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb, active);
+    vzero<VZ>(pn);
+
+    for (int j = x0 - 4; j < x1 + 4; ++j) {
+        float o1[VZ], cj[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
+        if (j + 1 < x1 + 4) {
+            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb, active);
+            ldcol<VZ>(on, old, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+        }
+        // ---- step 1: u_{t+1} at column j (0 on the rim columns)
+        float n1[VZ];
+        vzero<VZ>(n1);
+        if (j >= 4 && j < nxx - 4) {
+            float ext[VZ + 8];
+            zext<VZ>(ext, q1[4], cur + (long)j * ldz, zb, lane, ldz);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const float d = ABC ? damper(a.b1d, i, j, g) : 1.f;
+                float val = leapfrog<ABC>(q1[4][r], o1[r], cj[r], lap8<VZ>(q1, ext, r), d);
+                if (i == siz && j == six) val += w1;
+                n1[r] = (i >= 4 && i < g.nzz - 4) ? val : o1[r];
+            }
+            if (j >= x0 && j < x1) {
+                if (active) {
+#pragma unroll
+                    for (int r = 0; r < VZ; ++r)
+                        if ((own_mask >> r) & 1u) out1[(long)j * ldz + zb + r] = n1[r];
+                }
+                if (rec1) sample_receivers<VZ, ABC>(a, n1, j, zb, own_mask, seis, t + 1 - a.tlag);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) q2[k][r] = q2[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) q2[8][r] = n1[r];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) cq[k][r] = cq[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) cq[4][r] = cj[r];
+
+        // ---- step 2: u_{t+2} at column jj = j - 4 (an owned column)
+        const int jj = j - 4;
+        if (jj >= x0) {
+            float ext[VZ + 8];
+            zext_local<VZ>(ext, q2[4], lane);
+            float n2[VZ];
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const float o2 = q1[0][r];              // u_t at column jj
+                const float d = ABC ? damper(a.b1d, i, jj, g) : 1.f;
+                float val = leapfrog<ABC>(q2[4][r], o2, cq[0][r], lap8<VZ>(q2, ext, r), d);
+                if (i == siz && jj == six) val += w2;
+                n2[r] = (i >= 4 && i < g.nzz - 4) ? val : o2;
+            }
+            if (active) {
+#pragma unroll
+                for (int r = 0; r < VZ; ++r)
+                    if ((own_mask >> r) & 1u) out2[(long)jj * ldz + zb + r] = n2[r];
+            }
+            if (rec2) sample_receivers<VZ, ABC>(a, n2, jj, zb, own_mask, seis, t + 2 - a.tlag);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) q1[k][r] = q1[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) q1[8][r] = pn[r];
+    }
+}
+
 // ---------------------------------------------------------------- reverse
 
+// One reverse step k: S_new = leapfrog(S_cur, S_old) over S_old, R likewise,
+// data injected into R_{k+1}, illum += S_new^2, image += S_new R_k or
+// gradient += R_k c lap(S_cur).
 template <int VZ, int MODE>
-__global__ __launch_bounds__(64) void rev_step_kernel(RevStep a, int nzseg, int nxchunk,
-                                                      int xw, int nslots, int k)
+__global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int k)
 {
     const int tile = xcd_tile(nzseg * nxchunk * nslots);
     if (tile < 0) return;
@@ -262,39 +448,35 @@ __global__ __launch_bounds__(64) void rev_step_kernel(RevStep a, int nzseg, int 
     const bool active = zb < g.ldz;
     const int j0 = 4 + xc * xw;
     const int j1 = min(j0 + xw, g.nxx - 4);
-    const int ldz = g.ldz;
+    const int ldz = g.ldz, nxx = g.nxx;
 
     const long soff = (long)slot * g.fstride;
-    const float *scur = ((k & 1) ? a.sa : a.sb) + soff;
-    float *sold = ((k & 1) ? a.sb : a.sa) + soff;
-    const float *rcur = ((k & 1) ? a.rb : a.ra) + soff;
-    float *rold = ((k & 1) ? a.ra : a.rb) + soff;
+    const float *scur = a.scur + soff;
+    const float *sold = a.sold + soff;
+    float *sout = a.sout1 + soff;
+    const float *rcur = a.rcur + soff;
+    const float *rold = a.rold + soff;
+    float *rout = a.rout1 + soff;
     const float *cc = a.c + (long)slot * a.cstride;
     float *acc = a.acc + soff;
     float *ill = a.ill + soff;
     const bool inj_on = (k + 1 < a.nt);
-    const int trow = a.nt - 2 - k;        // data sample entering R_{k+1}
     const int drow = a.data_by_shot ? (*a.shot0 + slot) : slot;
     const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
 
     float qs[9][VZ], qr[9][VZ];
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-        if (active) {
-            vload<VZ>(qs[m], scur + (long)(j0 - 4 + m) * ldz + zb);
-            vload<VZ>(qr[m], rcur + (long)(j0 - 4 + m) * ldz + zb);
-        } else {
-            vzero<VZ>(qs[m]); vzero<VZ>(qr[m]);
-        }
+        ldcol<VZ>(qs[m], scur, j0 - 4 + m, nxx, ldz, zb, active);
+        ldcol<VZ>(qr[m], rcur, j0 - 4 + m, nxx, ldz, zb, active);
     }
     float osn[VZ], orn[VZ], cn[VZ], an[VZ], in[VZ], psn[VZ], prn[VZ];
-    vzero<VZ>(osn); vzero<VZ>(orn); vzero<VZ>(cn); vzero<VZ>(an); vzero<VZ>(in);
+    ldcol<VZ>(osn, sold, j0, nxx, ldz, zb, active);
+    ldcol<VZ>(orn, rold, j0, nxx, ldz, zb, active);
+    ldcol<VZ>(cn, cc, j0, nxx, ldz, zb, active);
+    ldcol<VZ>(an, acc, j0, nxx, ldz, zb, active);
+    ldcol<VZ>(in, ill, j0, nxx, ldz, zb, active);
     vzero<VZ>(psn); vzero<VZ>(prn);
-    if (active) {
-        const long off = (long)j0 * ldz + zb;
-        vload<VZ>(osn, sold + off); vload<VZ>(orn, rold + off); vload<VZ>(cn, cc + off);
-        vload<VZ>(an, acc + off); vload<VZ>(in, ill + off);
-    }
 
     for (int j = j0; j < j1; ++j) {
         float os[VZ], orr[VZ], c[VZ], ac[VZ], il[VZ];
@@ -302,26 +484,26 @@ __global__ __launch_bounds__(64) void rev_step_kernel(RevStep a, int nzseg, int 
         for (int r = 0; r < VZ; ++r) {
             os[r] = osn[r]; orr[r] = orn[r]; c[r] = cn[r]; ac[r] = an[r]; il[r] = in[r];
         }
-        if (active && j + 1 < j1) {
-            const long offp = (long)(j + 5) * ldz + zb;
-            const long offn = (long)(j + 1) * ldz + zb;
-            vload<VZ>(psn, scur + offp); vload<VZ>(prn, rcur + offp);
-            vload<VZ>(osn, sold + offn); vload<VZ>(orn, rold + offn);
-            vload<VZ>(cn, cc + offn); vload<VZ>(an, acc + offn); vload<VZ>(in, ill + offn);
+        if (j + 1 < j1) {
+            ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb, active);
+            ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb, active);
+            ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(an, acc, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(in, ill, j + 1, nxx, ldz, zb, active);
         }
         float es[VZ + 8], er[VZ + 8];
         zext<VZ>(es, qs[4], scur + (long)j * ldz, zb, lane, ldz);
         zext<VZ>(er, qr[4], rcur + (long)j * ldz, zb, lane, ldz);
-
         float ns[VZ], nr[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) {
             const int i = zb + r;
             const bool inside = (i >= 4) && (i < g.nzz - 4);
-            const float ps = qs[4][r];
             const float pr = qr[4][r];
             const float lsc = c[r] * lap8<VZ>(qs, es, r);
-            const float sn = (ps + (ps - os[r])) + lsc;
+            const float sn = (qs[4][r] + (qs[4][r] - os[r])) + lsc;
             const float rn = (pr + (pr - orr[r])) + c[r] * lap8<VZ>(qr, er, r);
             ns[r] = inside ? sn : os[r];
             nr[r] = inside ? rn : orr[r];
@@ -331,18 +513,10 @@ __global__ __launch_bounds__(64) void rev_step_kernel(RevStep a, int nzseg, int 
                 else ac[r] += pr * lsc;
             }
         }
-        if (inj_on) {
-            const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
-            for (int m = k0; m < k1; ++m) {
-                const int rr = a.rec.iz[m] - zb;
-#pragma unroll
-                for (int r = 0; r < VZ; ++r)
-                    if (rr == r) nr[r] += a.idh2 * dat[(long)a.rec.id[m] * a.nt + trow];
-            }
-        }
+        if (inj_on) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
         if (active) {
             const long off = (long)j * ldz + zb;
-            vstore<VZ>(sold + off, ns); vstore<VZ>(rold + off, nr);
+            vstore<VZ>(sout + off, ns); vstore<VZ>(rout + off, nr);
             vstore<VZ>(acc + off, ac); vstore<VZ>(ill + off, il);
         }
 #pragma unroll
@@ -354,25 +528,197 @@ __global__ __launch_bounds__(64) void rev_step_kernel(RevStep a, int nzseg, int 
     }
 }
 
+// Two reverse steps per launch, tiled like fwd2_kernel, for both wavefields.
+// The step-1 image / illumination contributions wait in a 5-column queue so
+// the accumulators are read and written once per two steps.
+template <int VZ, int MODE>
+__global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int k)
+{
+    const int tile = xcd_tile(nzseg * nxchunk * nslots);
+    if (tile < 0) return;
+    const int zseg = tile % nzseg;
+    const int xc = (tile / nzseg) % nxchunk;
+    const int slot = tile / (nzseg * nxchunk);
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int ldz = g.ldz, nxx = g.nxx;
+    const int zs = zseg * (64 * VZ - 8) - 4;
+    const int zb = zs + lane * VZ;
+    const bool active = (zb >= 0) && (zb < ldz);
+    unsigned own_mask = 0u;
+#pragma unroll
+    for (int r = 0; r < VZ; ++r) {
+        const int i = zb + r;
+        if (i >= zs + 4 && i < zs + 64 * VZ - 4 && i >= 0 && i < ldz) own_mask |= 1u << r;
+    }
+    const int x0 = 4 + xc * xw;
+    const int x1 = min(x0 + xw, nxx - 4);
+
+    const long soff = (long)slot * g.fstride;
+    const float *scur = a.scur + soff;
+    const float *sold = a.sold + soff;
+    float *sout1 = a.sout1 + soff;
+    float *sout2 = a.sout2 + soff;
+    const float *rcur = a.rcur + soff;
+    const float *rold = a.rold + soff;
+    float *rout1 = a.rout1 + soff;
+    float *rout2 = a.rout2 + soff;
+    const float *cc = a.c + (long)slot * a.cstride;
+    float *acc = a.acc + soff;
+    float *ill = a.ill + soff;
+    const bool inj1 = (k + 1 < a.nt);
+    const bool inj2 = (k + 2 < a.nt);
+    const int drow = a.data_by_shot ? (*a.shot0 + slot) : slot;
+    const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
+
+    float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur: columns j-4 .. j+4
+    float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R: columns j-8 .. j
+    float cq[5][VZ];                 // c: columns j-4 .. j
+    float iq[5][VZ], aq[5][VZ];      // step-1 illum / image contributions
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        ldcol<VZ>(qs1[m], scur, x0 - 8 + m, nxx, ldz, zb, active);
+        ldcol<VZ>(qr1[m], rcur, x0 - 8 + m, nxx, ldz, zb, active);
+        vzero<VZ>(qs2[m]); vzero<VZ>(qr2[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) { vzero<VZ>(cq[m]); vzero<VZ>(iq[m]); vzero<VZ>(aq[m]); }
+    float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ];
+    ldcol<VZ>(osn, sold, x0 - 4, nxx, ldz, zb, active);
+    ldcol<VZ>(orn, rold, x0 - 4, nxx, ldz, zb, active);
+    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb, active);
+    vzero<VZ>(psn); vzero<VZ>(prn);
+
+    for (int j = x0 - 4; j < x1 + 4; ++j) {
+        float os[VZ], orr[VZ], cj[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r]; }
+        const int jj = j - 4;
+        float ac[VZ], il[VZ];
+        if (jj >= x0) {
+            ldcol<VZ>(ac, acc, jj, nxx, ldz, zb, active);
+            ldcol<VZ>(il, ill, jj, nxx, ldz, zb, active);
+        }
+        if (j + 1 < x1 + 4) {
+            ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb, active);
+            ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb, active);
+            ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+        }
+        // ---- step 1 at column j
+        float ns[VZ], nr[VZ], ci[VZ], ca[VZ];
+        vzero<VZ>(ns); vzero<VZ>(nr); vzero<VZ>(ci); vzero<VZ>(ca);
+        if (j >= 4 && j < nxx - 4) {
+            float es[VZ + 8], er[VZ + 8];
+            zext<VZ>(es, qs1[4], scur + (long)j * ldz, zb, lane, ldz);
+            zext<VZ>(er, qr1[4], rcur + (long)j * ldz, zb, lane, ldz);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = (i >= 4) && (i < g.nzz - 4);
+                const float pr = qr1[4][r];
+                const float lsc = cj[r] * lap8<VZ>(qs1, es, r);
+                const float sn = (qs1[4][r] + (qs1[4][r] - os[r])) + lsc;
+                const float rn = (pr + (pr - orr[r])) + cj[r] * lap8<VZ>(qr1, er, r);
+                ns[r] = inside ? sn : os[r];
+                nr[r] = inside ? rn : orr[r];
+                if (inside) {
+                    ci[r] = sn * sn;
+                    ca[r] = (MODE == 0) ? sn * pr : pr * lsc;
+                }
+            }
+            if (inj1) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
+            if (j >= x0 && j < x1 && active) {
+#pragma unroll
+                for (int r = 0; r < VZ; ++r)
+                    if ((own_mask >> r) & 1u) {
+                        sout1[(long)j * ldz + zb + r] = ns[r];
+                        rout1[(long)j * ldz + zb + r] = nr[r];
+                    }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) { qs2[m][r] = qs2[m + 1][r]; qr2[m][r] = qr2[m + 1][r]; }
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { qs2[8][r] = ns[r]; qr2[8][r] = nr[r]; }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                cq[m][r] = cq[m + 1][r]; iq[m][r] = iq[m + 1][r]; aq[m][r] = aq[m + 1][r];
+            }
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { cq[4][r] = cj[r]; iq[4][r] = ci[r]; aq[4][r] = ca[r]; }
+
+        // ---- step 2 at column jj
+        if (jj >= x0) {
+            float es[VZ + 8], er[VZ + 8];
+            zext_local<VZ>(es, qs2[4], lane);
+            zext_local<VZ>(er, qr2[4], lane);
+            float ns2[VZ], nr2[VZ];
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = (i >= 4) && (i < g.nzz - 4);
+                const float os2 = qs1[0][r];            // S_cur at column jj
+                const float or2 = qr1[0][r];            // R_k at column jj
+                const float pr = qr2[4][r];             // R_{k+1}
+                const float lsc = cq[0][r] * lap8<VZ>(qs2, es, r);
+                const float sn = (qs2[4][r] + (qs2[4][r] - os2)) + lsc;
+                const float rn = (pr + (pr - or2)) + cq[0][r] * lap8<VZ>(qr2, er, r);
+                ns2[r] = inside ? sn : os2;
+                nr2[r] = inside ? rn : or2;
+                if (inside) {
+                    il[r] = (il[r] + iq[0][r]) + sn * sn;
+                    if constexpr (MODE == 0) ac[r] = (ac[r] + aq[0][r]) + sn * pr;
+                    else ac[r] = (ac[r] + aq[0][r]) + pr * lsc;
+                }
+            }
+            if (inj2) inject_receivers<VZ>(a, nr2, jj, zb, dat, a.nt - 3 - k);
+            if (active) {
+#pragma unroll
+                for (int r = 0; r < VZ; ++r)
+                    if ((own_mask >> r) & 1u) {
+                        const long off = (long)jj * ldz + zb + r;
+                        sout2[off] = ns2[r];
+                        rout2[off] = nr2[r];
+                        acc[off] = ac[r];
+                        ill[off] = il[r];
+                    }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) { qs1[m][r] = qs1[m + 1][r]; qr1[m][r] = qr1[m + 1][r]; }
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { qs1[8][r] = psn[r]; qr1[8][r] = prn[r]; }
+    }
+}
+
 // ---------------------------------------------------------------- helpers
 
-__global__ void fwd_init_kernel(FwdStep a, int nslots)
+__global__ void fwd_init_kernel(FwdStep a, float *field, int nslots)
 {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= nslots || a.nt < 1) return;
     const int gshot = *a.shot0 + slot;
     const long idx = (long)slot * a.g.fstride + (long)a.src_ix[gshot] * a.g.ldz + a.src_iz[gshot];
-    a.fa[idx] += a.wav[0];
+    field[idx] += a.wav[0];
 }
 
-__global__ void rev_init_kernel(RevStep a, int nslots)
+__global__ void rev_init_kernel(RevStep a, float *field, int nslots)
 {
     // one thread per slot: receivers sharing a cell are summed in order
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= nslots || a.nt < 1) return;
     const int drow = a.data_by_shot ? (*a.shot0 + slot) : slot;
     const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
-    float *ra = a.ra + (long)slot * a.g.fstride;
+    float *ra = field + (long)slot * a.g.fstride;
     for (int j = 0; j < a.g.nxx; ++j)
         for (int m = a.rec.col_ptr[j]; m < a.rec.col_ptr[j + 1]; ++m)
             ra[(long)j * a.g.ldz + a.rec.iz[m]] += a.idh2 * dat[(long)a.rec.id[m] * a.nt + a.nt - 1];
@@ -424,21 +770,24 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
 
 __device__ __forceinline__ float rb_draw(uint64_t key, uint64_t cell, int k)
 {
+#pragma clang fp contract(off)
     const uint64_t h = mix64(key + (2 * cell + (uint64_t)k + 1) * kGolden);
     const int32_t s = (int32_t)(h >> 40) - 8388608;
-    return __fmul_rn((float)s, 1.0f / 8388608.0f);
+    return (float)s * (1.0f / 8388608.0f);
 }
 
 // Seeded random boundary: pad cells take the edge value plus a ramped
 // uniform perturbation, once for the x pad and once for the z pad, clamped to
 // [vlo, vhi] (the structure of random_boundary_bg, modeling.cu:591-628, with
-// a counter-based draw).  Every float operation is an explicit
-// round-to-nearest intrinsic so the result is bit-identical to the
-// sequential C statement of the same generator.
+// a counter-based draw).  FMA contraction is off in this kernel and every
+// float operation is one correctly rounded IEEE operation (the division
+// included), so the result is bit-identical to a sequential C statement of
+// the same generator.
 __global__ void rb_kernel(const float *vpad, Grid g, int nslots, const int *shot0,
                           uint32_t iteration, uint64_t seed, float amp_abs, float amp_rel,
                           float vlo, float vhi, float kc, float *c, float *v_out)
 {
+#pragma clang fp contract(off)
     const long ncell = (long)g.nzz * g.nxx;
     const int slot = blockIdx.y;
     const uint64_t stream = ((uint64_t)iteration << 32) | (uint32_t)(*shot0 + slot);
@@ -452,21 +801,28 @@ __global__ void rb_kernel(const float *vpad, Grid g, int nslots, const int *shot
         const bool zpad = (i < g.nb) || (i >= g.nzz - g.nb);
         float val = base;
         if (xpad || zpad) {
-            const float A = __fadd_rn(amp_abs, __fmul_rn(amp_rel, base));
+            const float ra = amp_rel * base;
+            const float A = amp_abs + ra;
             const uint64_t cell = (uint64_t)idx;
             if (xpad) {
                 const int dj = (j < g.nb) ? j : (g.nxx - 1 - j);
-                const float f = __fsub_rn(1.0f, __fdiv_rn((float)dj, fnb));
-                val = __fadd_rn(val, __fmul_rn(__fmul_rn(f, A), rb_draw(key, cell, 0)));
+                const float q = (float)dj / fnb;
+                const float f = 1.0f - q;
+                const float fa = f * A;
+                const float d = fa * rb_draw(key, cell, 0);
+                val = val + d;
             }
             if (zpad) {
                 const int di = (i < g.nb) ? i : (g.nzz - 1 - i);
-                const float f = __fsub_rn(1.0f, __fdiv_rn((float)di, fnb));
-                val = __fadd_rn(val, __fmul_rn(__fmul_rn(f, A), rb_draw(key, cell, 1)));
+                const float q = (float)di / fnb;
+                const float f = 1.0f - q;
+                const float fa = f * A;
+                const float d = fa * rb_draw(key, cell, 1);
+                val = val + d;
             }
             val = fminf(fmaxf(val, vlo), vhi);
         }
-        if (c) c[(long)slot * g.fstride + (long)j * g.ldz + i] = __fmul_rn(__fmul_rn(val, val), kc);
+        if (c) c[(long)slot * g.fstride + (long)j * g.ldz + i] = (val * val) * kc;
         if (v_out) v_out[(long)slot * ncell + idx] = val;
     }
 }
@@ -530,6 +886,13 @@ __global__ void crop_kernel(const float *src, Grid g, int nz, int nx, float *dst
     }
 }
 
+__global__ void copy_kernel(const float4 *src, float4 *dst, long n4)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void zero_sample0_kernel(float *seis, int nrec, int nt, long n)
 {
     for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
@@ -546,64 +909,94 @@ inline int grid1d(long n, int block, int cap = 8192)
 }
 
 template <int VZ>
-hipError_t fwd_dispatch(const FwdStep &a, const Tiling &tl, int nslots, int t, hipStream_t s)
+hipError_t fwd_dispatch(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
+                        hipStream_t s)
 {
     const int ntiles = tl.nzseg * tl.nxchunk * nslots;
     const int nblocks = ((ntiles + 7) >> 3) << 3;
-    if (a.abc)
-        hipLaunchKernelGGL((fwd_step_kernel<VZ, true>), dim3(nblocks), dim3(64), 0, s, a,
-                           tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
-    else
-        hipLaunchKernelGGL((fwd_step_kernel<VZ, false>), dim3(nblocks), dim3(64), 0, s, a,
-                           tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    if (steps == 2) {
+        if (a.abc)
+            hipLaunchKernelGGL((fwd2_kernel<VZ, true>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+        else
+            hipLaunchKernelGGL((fwd2_kernel<VZ, false>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    } else {
+        if (a.abc)
+            hipLaunchKernelGGL((fwd1_kernel<VZ, true>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+        else
+            hipLaunchKernelGGL((fwd1_kernel<VZ, false>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    }
     return hipGetLastError();
 }
 
 template <int VZ>
-hipError_t rev_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, hipStream_t s)
+hipError_t rev_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
+                        hipStream_t s)
 {
     const int ntiles = tl.nzseg * tl.nxchunk * nslots;
     const int nblocks = ((ntiles + 7) >> 3) << 3;
-    if (a.mode == 0)
-        hipLaunchKernelGGL((rev_step_kernel<VZ, 0>), dim3(nblocks), dim3(64), 0, s, a,
-                           tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
-    else
-        hipLaunchKernelGGL((rev_step_kernel<VZ, 1>), dim3(nblocks), dim3(64), 0, s, a,
-                           tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+    if (steps == 2) {
+        if (a.mode == 0)
+            hipLaunchKernelGGL((rev2_kernel<VZ, 0>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+        else
+            hipLaunchKernelGGL((rev2_kernel<VZ, 1>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+    } else {
+        if (a.mode == 0)
+            hipLaunchKernelGGL((rev1_kernel<VZ, 0>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+        else
+            hipLaunchKernelGGL((rev1_kernel<VZ, 1>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+    }
     return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t, hipStream_t s)
+int zsegments(int ldz, int vz, int steps)
+{
+    const int rows = (steps == 2) ? 64 * vz - 8 : 64 * vz;
+    return (ldz + rows - 1) / rows;
+}
+
+hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
+                           hipStream_t s)
 {
     switch (tl.vz) {
-    case 1: return fwd_dispatch<1>(a, tl, nslots, t, s);
-    case 2: return fwd_dispatch<2>(a, tl, nslots, t, s);
-    case 4: return fwd_dispatch<4>(a, tl, nslots, t, s);
+    case 1: return fwd_dispatch<1>(a, tl, nslots, t, steps, s);
+    case 2: return fwd_dispatch<2>(a, tl, nslots, t, steps, s);
+    case 4: return fwd_dispatch<4>(a, tl, nslots, t, steps, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k, hipStream_t s)
+hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
+                           hipStream_t s)
 {
     switch (tl.vz) {
-    case 1: return rev_dispatch<1>(a, tl, nslots, k, s);
-    case 2: return rev_dispatch<2>(a, tl, nslots, k, s);
-    case 4: return rev_dispatch<4>(a, tl, nslots, k, s);
+    case 1: return rev_dispatch<1>(a, tl, nslots, k, steps, s);
+    case 2: return rev_dispatch<2>(a, tl, nslots, k, steps, s);
+    case 4: return rev_dispatch<4>(a, tl, nslots, k, steps, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_fwd_init(const FwdStep &a, int nslots, hipStream_t s)
+hipError_t launch_fwd_init(const FwdStep &a, float *field, int nslots, hipStream_t s)
 {
-    hipLaunchKernelGGL(fwd_init_kernel, dim3((nslots + 63) / 64), dim3(64), 0, s, a, nslots);
+    hipLaunchKernelGGL(fwd_init_kernel, dim3((nslots + 63) / 64), dim3(64), 0, s, a, field,
+                       nslots);
     return hipGetLastError();
 }
 
-hipError_t launch_rev_init(const RevStep &a, int nslots, hipStream_t s)
+hipError_t launch_rev_init(const RevStep &a, float *field, int nslots, hipStream_t s)
 {
-    hipLaunchKernelGGL(rev_init_kernel, dim3((nslots + 63) / 64), dim3(64), 0, s, a, nslots);
+    hipLaunchKernelGGL(rev_init_kernel, dim3((nslots + 63) / 64), dim3(64), 0, s, a, field,
+                       nslots);
     return hipGetLastError();
 }
 
@@ -671,6 +1064,14 @@ hipError_t launch_crop(const float *src, const Grid &g, int nz, int nx, float *d
 {
     hipLaunchKernelGGL(crop_kernel, dim3(grid1d((long)nz * nx, 256)), dim3(256), 0, s, src, g,
                        nz, nx, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const float *src, float *dst, long n, hipStream_t s)
+{
+    const long n4 = n / 4;
+    hipLaunchKernelGGL(copy_kernel, dim3(grid1d(n4, 256, 65536)), dim3(256), 0, s,
+                       reinterpret_cast<const float4 *>(src), reinterpret_cast<float4 *>(dst), n4);
     return hipGetLastError();
 }
 

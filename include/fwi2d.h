@@ -153,11 +153,25 @@ int fwi_grid_shape(fwi_ctx *ctx, int32_t boundary, int32_t *nzz, int32_t *nxx, i
 /* Blocks until the context's stream is idle. */
 int fwi_synchronize(fwi_ctx *ctx);
 
-/* Launch tuning: cells per lane along z (1, 2 or 4), columns per wave and
- * hipGraph capture of the time loop (1 on, 0 off).  0 / negative keeps the
- * current value. */
-int fwi_set_tuning(fwi_ctx *ctx, int32_t vz, int32_t xw, int32_t use_graph);
-int fwi_get_tuning(fwi_ctx *ctx, int32_t *vz, int32_t *xw, int32_t *use_graph);
+/* Launch tuning of the step kernels (kernel 0 forward, 1 reverse, -1 both):
+ * time steps per launch (1, or 2 = temporal blocking), cells per lane along
+ * z (1, 2 or 4), columns per wave, and hipGraph capture of the time loops
+ * (1 on, 0 off).  0 / negative keeps the current value.  Every shape gives
+ * bit-identical results. */
+int fwi_set_tuning(fwi_ctx *ctx, int32_t kernel, int32_t steps, int32_t vz, int32_t xw,
+                   int32_t use_graph);
+int fwi_get_tuning(fwi_ctx *ctx, int32_t kernel, int32_t *steps, int32_t *vz, int32_t *xw,
+                   int32_t *use_graph);
+
+/* Autotuner: times every candidate launch shape of the forward and reverse
+ * kernels on the random-boundary grid with nslots resident shots (`reps`
+ * launches each) and keeps the fastest; returns the time per step (us). */
+int fwi_autotune(fwi_ctx *ctx, int32_t nslots, int32_t reps, double *fwd_us, double *rev_us);
+
+/* Measured device bandwidth (GB/s, read + write) of a plain float4 copy of
+ * nbytes, `reps` times: the practical HBM ceiling the step kernels are
+ * compared with. */
+int fwi_stream_copy(int64_t nbytes, int32_t reps, double *gbs);
 
 /* Per-pass timing with HIP events recorded on the context stream around each
  * time loop.  kind: 0 forward step, 1 reverse step.  Returns the summed GPU

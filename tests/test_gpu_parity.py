@@ -114,13 +114,41 @@ def test_forward_rbc_matches_oracle():
         assert err <= SEIS_TOL
 
 
+SHAPES = [(1, 1, 8, 0), (1, 2, 17, 1), (1, 4, 64, 1), (1, 4, 1000, 0), (1, 1, 3, 1),
+          (2, 1, 8, 0), (2, 2, 17, 1), (2, 4, 64, 1), (2, 4, 1000, 0), (2, 1, 3, 1), (2, 2, 5, 0)]
+
+
 def test_launch_shapes_bit_identical():
+    # every (steps per launch, rows per lane, chunk width, graph) gives the same bits
     s = overthrust_survey(abc_nb=51, abc_factor=0.004, max_batch=3)
+    s.eng.set_tuning(steps=1, vz=4, xw=32, use_graph=0)
     base = s.eng.modeling(F.ABC)
-    for vz, xw, g in [(1, 8, 0), (2, 17, 1), (4, 64, 1), (4, 1000, 0), (1, 3, 1)]:
-        s.eng.set_tuning(vz, xw, g)
+    for st, vz, xw, g in SHAPES:
+        s.eng.set_tuning(vz, xw, g, steps=st)
         out = s.eng.modeling(F.ABC)
-        assert np.array_equal(out, base), (vz, xw, g)
+        assert np.array_equal(out, base), (st, vz, xw, g)
+
+
+def test_reverse_launch_shapes_bit_identical():
+    s = overthrust_survey(nsrc=2, abc_nb=51, abc_factor=0.004, rbc_nb=41, varVp=900.0,
+                          seed=8, max_batch=2)
+    obs = s.eng.modeling(F.ABC)
+    s.eng.set_observed(obs)
+    s.eng.set_model((s.v * 0.98).astype(np.float32))
+    results = []
+    for st, vz, xw, g in [(1, 2, 32, 0)] + SHAPES[5:]:
+        s.eng.set_tuning(vz, xw, g, steps=st)
+        s.eng.reset()
+        s.eng.migration()
+        img = s.eng.get_image()
+        s.eng.reset()
+        f = s.eng.gradient(iteration=2)
+        grd = s.eng.get_gradient(100.0)
+        results.append((img[1], img[2], grd[1], grd[2], f))
+    for r in results[1:]:
+        for a, b in zip(r[:4], results[0][:4]):
+            assert np.array_equal(a, b)
+        assert r[4] == results[0][4]
 
 
 def test_batching_bit_identical():
