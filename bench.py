@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--nt", type=int, default=0,
                    help="override time_samples (short profiling runs; same per-step work)")
     p.add_argument("--no-autotune", action="store_true")
+    p.add_argument("--fwd-tuning", default="", help="steps,vz,xw of the forward kernel")
+    p.add_argument("--rev-tuning", default="", help="steps,vz,xw of the reverse kernel")
     p.add_argument("--graph", type=int, default=-1)
     return p.parse_args()
 
@@ -125,7 +127,12 @@ def main():
     eng.set_observed(obs)
     eng.set_model(mig_v)
     nzz, nxx, _ = eng.grid_shape(F.RBC)
-    if not a.no_autotune:
+    if a.fwd_tuning or a.rev_tuning:
+        for kernel, spec in ((0, a.fwd_tuning), (1, a.rev_tuning)):
+            if spec:
+                st, vz, xw = (int(x) for x in spec.split(","))
+                eng.set_tuning(vz, xw, a.graph, kernel=kernel, steps=st)
+    elif not a.no_autotune:
         eng.autotune(nshots, reps=8)
 
     for _ in range(a.warmup):

@@ -144,16 +144,20 @@ void drop_graphs(fwi_ctx *c)
     c->graphs.clear();
 }
 
+// Device allocation zeroed on `stream` and waited for: the context stream is
+// non-blocking, so a zero-fill on the legacy null stream would not be ordered
+// before the kernels that use the buffer.
 template <typename T>
-int dalloc(T **p, size_t n)
+int dalloc(T **p, size_t n, hipStream_t stream)
 {
     if (n == 0) n = 1;
     hipError_t e = hipMalloc((void **)p, n * sizeof(T));
     if (e != hipSuccess)
         return fail(FWI_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
                     hipGetErrorString(e));
-    e = hipMemset(*p, 0, n * sizeof(T));
-    if (e != hipSuccess) return fail(FWI_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    e = hipMemsetAsync(*p, 0, n * sizeof(T), stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return fail(FWI_ERR_HIP, "zero-fill failed: %s", hipGetErrorString(e));
     return FWI_OK;
 }
 
@@ -462,33 +466,33 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     const long S = cfg->max_batch;
     const long n_int = (long)cfg->nz * cfg->nx;
     r = FWI_OK;
-    if (r == FWI_OK) r = dalloc(&c->d_vp, n_int);
-    if (r == FWI_OK) r = dalloc(&c->d_tmp, n_int);
-    if (r == FWI_OK) r = dalloc(&c->d_shot0, 1);
-    if (r == FWI_OK) r = dalloc(&c->d_wav, cfg->nt);
-    for (int k = 0; k < 8 && r == FWI_OK; ++k) r = dalloc(&c->d_f[k], S * c->fstride_max);
+    if (r == FWI_OK) r = dalloc(&c->d_vp, n_int, c->stream);
+    if (r == FWI_OK) r = dalloc(&c->d_tmp, n_int, c->stream);
+    if (r == FWI_OK) r = dalloc(&c->d_shot0, 1, c->stream);
+    if (r == FWI_OK) r = dalloc(&c->d_wav, cfg->nt, c->stream);
+    for (int k = 0; k < 8 && r == FWI_OK; ++k) r = dalloc(&c->d_f[k], S * c->fstride_max, c->stream);
     for (int kind = 0; kind < 2 && r == FWI_OK; ++kind)
-        if (c->grid[kind].on) r = dalloc(&c->grid[kind].vpad, c->grid[kind].g.fstride);
+        if (c->grid[kind].on) r = dalloc(&c->grid[kind].vpad, c->grid[kind].g.fstride, c->stream);
     if (r == FWI_OK && c->grid[0].on) {
         std::vector<float> b1d;
         fwi::cerjan_table(cfg->abc_nb, cfg->abc_factor, b1d);
-        r = dalloc(&c->d_b1d, b1d.size());
+        r = dalloc(&c->d_b1d, b1d.size(), c->stream);
         if (r == FWI_OK)
             r = hipMemcpy(c->d_b1d, b1d.data(), b1d.size() * 4, hipMemcpyHostToDevice) == hipSuccess
                     ? FWI_OK : fail(FWI_ERR_HIP, "b1d upload");
-        if (r == FWI_OK) r = dalloc(&c->d_cabc, c->grid[0].g.fstride);
+        if (r == FWI_OK) r = dalloc(&c->d_cabc, c->grid[0].g.fstride, c->stream);
     }
     if (r == FWI_OK && c->grid[1].on) {
         const long fs = c->grid[1].g.fstride;
-        r = dalloc(&c->d_crb, S * fs);
-        if (r == FWI_OK) r = dalloc(&c->d_acc_slot, S * fs);
-        if (r == FWI_OK) r = dalloc(&c->d_ill_slot, S * fs);
-        if (r == FWI_OK) r = dalloc(&c->d_image, fs);
-        if (r == FWI_OK) r = dalloc(&c->d_grad, fs);
-        if (r == FWI_OK) r = dalloc(&c->d_illum, fs);
+        r = dalloc(&c->d_crb, S * fs, c->stream);
+        if (r == FWI_OK) r = dalloc(&c->d_acc_slot, S * fs, c->stream);
+        if (r == FWI_OK) r = dalloc(&c->d_ill_slot, S * fs, c->stream);
+        if (r == FWI_OK) r = dalloc(&c->d_image, fs, c->stream);
+        if (r == FWI_OK) r = dalloc(&c->d_grad, fs, c->stream);
+        if (r == FWI_OK) r = dalloc(&c->d_illum, fs, c->stream);
     }
-    if (r == FWI_OK) r = dalloc(&c->d_partials, c->npartials);
-    if (r == FWI_OK) r = dalloc(&c->d_sum, 1);
+    if (r == FWI_OK) r = dalloc(&c->d_partials, c->npartials, c->stream);
+    if (r == FWI_OK) r = dalloc(&c->d_sum, 1, c->stream);
     if (r != FWI_OK) { fwi_destroy(c); return r; }
     // default launch shape (the autotuner refines it): two steps per launch,
     // 2 rows per lane forward, 1 reverse (register pressure), 64-column chunks
@@ -589,11 +593,11 @@ int fwi_set_geometry(fwi_ctx *c, int32_t nsrc, const float *xsrc, const float *z
         for (void *p : old)
             if (p) (void)hipFree(p);
         gb.src_iz = gb.src_ix = gb.col_ptr = gb.rec_iz = gb.rec_id = nullptr;
-        TRY(dalloc(&gb.src_iz, nsrc));
-        TRY(dalloc(&gb.src_ix, nsrc));
-        TRY(dalloc(&gb.col_ptr, gb.g.nxx + 1));
-        TRY(dalloc(&gb.rec_iz, nrec));
-        TRY(dalloc(&gb.rec_id, nrec));
+        TRY(dalloc(&gb.src_iz, nsrc, c->stream));
+        TRY(dalloc(&gb.src_ix, nsrc, c->stream));
+        TRY(dalloc(&gb.col_ptr, gb.g.nxx + 1, c->stream));
+        TRY(dalloc(&gb.rec_iz, nrec, c->stream));
+        TRY(dalloc(&gb.rec_id, nrec, c->stream));
         HIP_TRY(hipMemcpy(gb.src_iz, siz.data(), nsrc * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(gb.src_ix, six.data(), nsrc * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(gb.col_ptr, col_ptr.data(), col_ptr.size() * 4, hipMemcpyHostToDevice));
@@ -604,8 +608,8 @@ int fwi_set_geometry(fwi_ctx *c, int32_t nsrc, const float *xsrc, const float *z
     if (c->d_obs) { (void)hipFree(c->d_obs); c->d_obs = nullptr; }
     if (c->d_seis) { (void)hipFree(c->d_seis); c->d_seis = nullptr; }
     const long per_shot = (long)nrec * c->cfg.nt;
-    TRY(dalloc(&c->d_obs, (long)nsrc * per_shot));
-    TRY(dalloc(&c->d_seis, (long)c->cfg.max_batch * per_shot));
+    TRY(dalloc(&c->d_obs, (long)nsrc * per_shot, c->stream));
+    TRY(dalloc(&c->d_seis, (long)c->cfg.max_batch * per_shot, c->stream));
     c->nsrc = nsrc;
     c->nrec = nrec;
     c->have_geometry = true;
@@ -808,7 +812,7 @@ int fwi_rb_velocity(fwi_ctx *c, int32_t shot, uint32_t iteration, float *out)
     float vlo, vhi;
     rb_bounds(c, &vlo, &vhi);
     float *d_out = nullptr;
-    TRY(dalloc(&d_out, ncell));
+    TRY(dalloc(&d_out, ncell, c->stream));
     hipError_t e = fwi::launch_rb(gb.vpad, gb.g, 1, c->d_shot0, iteration, k.rbc_seed,
                                   k.rbc_varVp, k.rbc_varVp_rel, vlo, vhi, c->kc, nullptr, d_out,
                                   c->stream);
@@ -941,8 +945,8 @@ int fwi_stream_copy(int64_t nbytes, int32_t reps, double *gbs)
     if (nbytes < 1024 || reps < 1 || !gbs) return fail(FWI_ERR_ARG, "bad arguments");
     float *a = nullptr, *b = nullptr;
     const long n = (long)(nbytes / 16) * 4;
-    TRY(dalloc(&a, n));
-    int r = dalloc(&b, n);
+    TRY(dalloc(&a, n, nullptr));
+    int r = dalloc(&b, n, nullptr);
     if (r != FWI_OK) { (void)hipFree(a); return r; }
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);

@@ -22,7 +22,9 @@
 // centre cell and the update as (p + (p - old)) + c*lap, with c = dt^2 v^2/dh^2.
 // This is the reference's recurrence with less rounding error (about 7x less
 // than the reference's order against an FP64 evaluation of the same
-// recurrence, over 2,000+ steps).
+// recurrence, over 2,000+ steps).  Every multiply-add that feeds a stored
+// value is an explicit fmaf, so all launch shapes (one or two steps per
+// launch, any tiling) produce bit-identical fields, images and gradients.
 //
 // Source injection is folded into the write of the field it enters:
 // the field written at step t receives idh2*w[t+1], which is what the
@@ -108,20 +110,6 @@ __device__ __forceinline__ void zext(float (&ext)[VZ + 8], const float (&own)[VZ
     }
 }
 
-// Laplacian (times dh^2) at row r of the lane from the x queue q[0..8]
-// (columns j-4..j+4) and the z extension ext, summed as centre differences.
-template <int VZ>
-__device__ __forceinline__ float lap8(const float (&q)[9][VZ], const float (&ext)[VZ + 8],
-                                      int r)
-{
-    const float p = q[4][r];
-    const float s1 = ((q[3][r] - p) + (q[5][r] - p)) + ((ext[r + 3] - p) + (ext[r + 5] - p));
-    const float s2 = ((q[2][r] - p) + (q[6][r] - p)) + ((ext[r + 2] - p) + (ext[r + 6] - p));
-    const float s3 = ((q[1][r] - p) + (q[7][r] - p)) + ((ext[r + 1] - p) + (ext[r + 7] - p));
-    const float s4 = ((q[0][r] - p) + (q[8][r] - p)) + ((ext[r + 0] - p) + (ext[r + 8] - p));
-    return kC4 * s1 - kC3 * s2 + kC2 * s3 - kC1 * s4;
-}
-
 // Cerjan factor of cell (i,j): 1 inside, b1d on the edges, b1d[a]+b1d[b]-1
 // in the corners (modeling.cu:459-506).  i is clamped into the grid.
 __device__ __forceinline__ float damper(const float *b1d, int i, int j, const Grid &g)
@@ -147,6 +135,35 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
     const int per = (ntiles + 7) >> 3;
     const int logical = (id & 7) * per + (id >> 3);
     return logical < ntiles ? logical : -1;
+}
+
+// Laplacian (times dh^2) from a 9-column register queue whose logical column
+// m (j-4+m) sits in slot (m + OFF) % 9, and the z extension of the centre
+// column, summed as centre differences; the weight sum is one explicit FMA
+// chain so that every kernel rounds it identically.
+template <int VZ, int OFF>
+__device__ __forceinline__ float lap8r(const float (&q)[9][VZ], const float (&ext)[VZ + 8],
+                                       int r)
+{
+    constexpr int P0 = (0 + OFF) % 9, P1 = (1 + OFF) % 9, P2 = (2 + OFF) % 9,
+                  P3 = (3 + OFF) % 9, P4 = (4 + OFF) % 9, P5 = (5 + OFF) % 9,
+                  P6 = (6 + OFF) % 9, P7 = (7 + OFF) % 9, P8 = (8 + OFF) % 9;
+    const float p = q[P4][r];
+    const float s1 = ((q[P3][r] - p) + (q[P5][r] - p)) + ((ext[r + 3] - p) + (ext[r + 5] - p));
+    const float s2 = ((q[P2][r] - p) + (q[P6][r] - p)) + ((ext[r + 2] - p) + (ext[r + 6] - p));
+    const float s3 = ((q[P1][r] - p) + (q[P7][r] - p)) + ((ext[r + 1] - p) + (ext[r + 7] - p));
+    const float s4 = ((q[P0][r] - p) + (q[P8][r] - p)) + ((ext[r + 0] - p) + (ext[r + 8] - p));
+    return fmaf(kC4, s1, fmaf(-kC3, s2, fmaf(kC2, s3, -kC1 * s4)));
+}
+
+// One leapfrog update with explicit FMAs:
+//   no sponge: u_new = (p + (p - o)) + c lap
+//   sponge:    u_new = d ((p + (p - d o)) + c lap)
+template <bool ABC>
+__device__ __forceinline__ float leapfrog(float p, float o, float c, float lap, float d)
+{
+    if constexpr (ABC) return d * fmaf(c, lap, p + fmaf(-d, o, p));
+    else return fmaf(c, lap, p + (p - o));
 }
 
 // z-extension of a field that exists only in registers (the inner field of a
@@ -180,26 +197,17 @@ __device__ __forceinline__ void ldcol(float (&r)[VZ], const float *base, int col
     else vzero<VZ>(r);
 }
 
-// One leapfrog update, u_new = d ((p + (p - d o)) + c lap)  (d = 1 without sponge).
-template <bool ABC>
-__device__ __forceinline__ float leapfrog(float p, float o, float c, float lap, float d)
-{
-    if constexpr (ABC) return d * ((p + (p - d * o)) + c * lap);
-    else return (p + (p - o)) + c * lap;
-}
-
-// Receivers in column j whose row is in this lane: seis[rec][tsamp] = scale * v[row].
+// Receivers in column j whose row is in this lane: seis[rec][tsamp] = d * v[row].
 template <int VZ, bool ABC>
 __device__ __forceinline__ void sample_receivers(const FwdStep &a, const float (&v)[VZ], int j,
-                                                 int zb, unsigned own_mask, float *seis,
-                                                 int tsamp)
+                                                 int zb, float *seis, int tsamp)
 {
     const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
     for (int k = k0; k < k1; ++k) {
         const int rr = a.rec.iz[k] - zb;
 #pragma unroll
         for (int r = 0; r < VZ; ++r) {
-            if (rr == r && ((own_mask >> r) & 1u)) {
+            if (rr == r) {
                 float val = v[r];
                 if constexpr (ABC) val *= damper(a.b1d, zb + r, j, a.g);
                 seis[(long)a.rec.id[k] * a.nt + tsamp] = val;
@@ -208,7 +216,7 @@ __device__ __forceinline__ void sample_receivers(const FwdStep &a, const float (
     }
 }
 
-// Adds scale * data[rec][trow] to the receivers of column j in this lane.
+// Adds idh2 * data[rec][trow] to the receivers of column j in this lane.
 template <int VZ>
 __device__ __forceinline__ void inject_receivers(const RevStep &a, float (&v)[VZ], int j, int zb,
                                                  const float *dat, int trow)
@@ -218,41 +226,52 @@ __device__ __forceinline__ void inject_receivers(const RevStep &a, float (&v)[VZ
         const int rr = a.rec.iz[m] - zb;
 #pragma unroll
         for (int r = 0; r < VZ; ++r)
-            if (rr == r) v[r] += a.idh2 * dat[(long)a.rec.id[m] * a.nt + trow];
+            if (rr == r) v[r] = fmaf(a.idh2, dat[(long)a.rec.id[m] * a.nt + trow], v[r]);
     }
+}
+
+// Decodes the wave's tile after the XCD renumbering.
+struct TileId {
+    int zseg, xc, slot;
+};
+__device__ __forceinline__ bool tile_of(int nzseg, int nxchunk, int nslots, TileId &t)
+{
+    const int tile = xcd_tile(nzseg * nxchunk * nslots);
+    if (tile < 0) return false;
+    t.zseg = tile % nzseg;
+    t.xc = (tile / nzseg) % nxchunk;
+    t.slot = tile / (nzseg * nxchunk);
+    return true;
 }
 
 // ---------------------------------------------------------------- forward
 
-// One time step: u_{t+1} = leapfrog(u_t, u_{t-1}) written over u_{t-1}.
+// One time step: u_{t+1} = leapfrog(u_t, u_{t-1}) written over u_{t-1}
+// (the odd remainder of a pass, and a tuning alternative).
 template <int VZ, bool ABC>
 __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int t)
 {
-    const int tile = xcd_tile(nzseg * nxchunk * nslots);
-    if (tile < 0) return;
-    const int zseg = tile % nzseg;
-    const int xc = (tile / nzseg) % nxchunk;
-    const int slot = tile / (nzseg * nxchunk);
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
     const Grid g = a.g;
     const int lane = threadIdx.x;
-    const int zb = zseg * 64 * VZ + lane * VZ;
+    const int zb = id.zseg * 64 * VZ + lane * VZ;
     const bool active = zb < g.ldz;
-    const int j0 = 4 + xc * xw;
+    const int j0 = 4 + id.xc * xw;
     const int j1 = min(j0 + xw, g.nxx - 4);
     const int ldz = g.ldz, nxx = g.nxx;
 
-    const long soff = (long)slot * g.fstride;
+    const long soff = (long)id.slot * g.fstride;
     const float *cur = a.cur + soff;
     const float *old = a.old + soff;
     float *out = a.out1 + soff;
-    const float *cc = a.c + (long)slot * a.cstride;
-    const int gshot = *a.shot0 + slot;
+    const float *cc = a.c + (long)id.slot * a.cstride;
+    const int gshot = *a.shot0 + id.slot;
     const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
     const float wsrc = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
     const bool rec_on = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
-    float *seis = rec_on ? a.seis + (long)slot * a.rec.nrec * a.nt : nullptr;
-    const unsigned own_mask = (1u << VZ) - 1u;
+    float *seis = rec_on ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
 
     float q[9][VZ];
 #pragma unroll
@@ -266,7 +285,7 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
         float o[VZ], c[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o[r] = on[r]; c[r] = cn[r]; }
-        if (j + 1 < j1) {                      // prefetch the next column
+        if (j + 1 < j1) {
             ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb, active);
             ldcol<VZ>(on, old, j + 1, nxx, ldz, zb, active);
             ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
@@ -278,12 +297,14 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
         for (int r = 0; r < VZ; ++r) {
             const int i = zb + r;
             const float d = ABC ? damper(a.b1d, i, j, g) : 1.f;
-            float val = leapfrog<ABC>(q[4][r], o[r], c[r], lap8<VZ>(q, ext, r), d);
+            float val = leapfrog<ABC>(q[4][r], o[r], c[r], lap8r<VZ, 0>(q, ext, r), d);
             if (i == siz && j == six) val += wsrc;
             nv[r] = (i >= 4 && i < g.nzz - 4) ? val : o[r];
         }
-        if (active) vstore<VZ>(out + (long)j * ldz + zb, nv);
-        if (rec_on) sample_receivers<VZ, ABC>(a, nv, j, zb, own_mask, seis, t + 1 - a.tlag);
+        if (active) {
+            vstore<VZ>(out + (long)j * ldz + zb, nv);
+            if (rec_on) sample_receivers<VZ, ABC>(a, nv, j, zb, seis, t + 1 - a.tlag);
+        }
 #pragma unroll
         for (int k = 0; k < 8; ++k)
 #pragma unroll
@@ -294,137 +315,154 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
 }
 
 // Two time steps per launch (temporal blocking).  The wave computes u_{t+1}
-// on 64*VZ rows and on its x-chunk plus 4 columns each side (kept in a second
-// register queue), then u_{t+2} on the inner 64*VZ - 8 rows of the chunk.
-// Reads u_t, u_{t-1} and c once for two steps; writes u_{t+1} (out1) and
-// u_{t+2} (out2) for the cells it owns.  All four buffers are distinct, so
-// no wave ever reads a value another wave of the same launch writes.
+// on 64*VZ rows and on its x-chunk plus 4 columns each side, and u_{t+2} on
+// the inner 64*VZ - 8 rows of the chunk.  It reads u_t, u_{t-1} and c once
+// for both steps and writes u_{t+1} (out1) and u_{t+2} (out2) for the cells
+// it owns; the four buffers are distinct, so no wave reads what another wave
+// of the same launch writes.  The column loop is unrolled by 9 and the three
+// register queues rotate by index (slot of logical column m at phase U is
+// (m + U) % 9), so advancing a column moves no registers.
 template <int VZ, bool ABC>
-__global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
-                                                  int nslots, int t)
-{
-    const int tile = xcd_tile(nzseg * nxchunk * nslots);
-    if (tile < 0) return;
-    const int zseg = tile % nzseg;
-    const int xc = (tile / nzseg) % nxchunk;
-    const int slot = tile / (nzseg * nxchunk);
-    const Grid g = a.g;
-    const int lane = threadIdx.x;
-    const int ldz = g.ldz, nxx = g.nxx;
-    const int zs = zseg * (64 * VZ - 8) - 4;     // first computed row
-    const int zb = zs + lane * VZ;
-    const bool active = (zb >= 0) && (zb < ldz);
-    unsigned own_mask = 0u;                      // rows this lane writes
-#pragma unroll
-    for (int r = 0; r < VZ; ++r) {
-        const int i = zb + r;
-        if (i >= zs + 4 && i < zs + 64 * VZ - 4 && i >= 0 && i < ldz) own_mask |= 1u << r;
-    }
-    const int x0 = 4 + xc * xw;
-    const int x1 = min(x0 + xw, nxx - 4);
+struct Fwd2Wave {
+    const FwdStep &a;
+    Grid g;
+    int lane, zb, x0, x1, siz, six, t;
+    bool active, own, rec1, rec2;
+    float w1, w2;
+    const float *cur, *old, *cc;
+    float *out1, *out2, *seis;
+    float q1[9][VZ];     // u_t,     logical m = column j-4+m
+    float q2[9][VZ];     // u_{t+1}, logical m = column j-8+m (after the push)
+    float cq[9][VZ];     // c,       logical m = column j-8+m (after the push)
+    float on[VZ], cn[VZ];
 
-    const long soff = (long)slot * g.fstride;
-    const float *cur = a.cur + soff;
-    const float *old = a.old + soff;
-    float *out1 = a.out1 + soff;
-    float *out2 = a.out2 + soff;
-    const float *cc = a.c + (long)slot * a.cstride;
-    const int gshot = *a.shot0 + slot;
-    const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
-    const float w1 = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
-    const float w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
-    const bool rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
-    const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
-    float *seis = (rec1 || rec2) ? a.seis + (long)slot * a.rec.nrec * a.nt : nullptr;
+    __device__ Fwd2Wave(const FwdStep &a_) : a(a_) {}
 
-    float q1[9][VZ];     // u_t, columns j-4 .. j+4
-    float q2[9][VZ];     // u_{t+1}, columns j-8 .. j
-    float cq[5][VZ];     // c, columns j-4 .. j
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        ldcol<VZ>(q1[k], cur, x0 - 8 + k, nxx, ldz, zb, active);
-        vzero<VZ>(q2[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) vzero<VZ>(cq[k]);
-    float on[VZ], cn[VZ], pn[VZ];
-    ldcol<VZ>(on, old, x0 - 4, nxx, ldz, zb, active);
-    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb, active);
-    vzero<VZ>(pn);
-
-    for (int j = x0 - 4; j < x1 + 4; ++j) {
-        float o1[VZ], cj[VZ];
+    template <int U>
+    __device__ __forceinline__ void column(int j)
+    {
+        const int ldz = g.ldz, nxx = g.nxx;
+        float o1[VZ], cj[VZ], pn[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
-        if (j + 1 < x1 + 4) {
-            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb, active);
-            ldcol<VZ>(on, old, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
-        }
+        // prefetch: u_t at j+5, u_{t-1} and c at j+1
+        ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb, active);
+        ldcol<VZ>(on, old, j + 1, nxx, ldz, zb, active);
+        ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+
         // ---- step 1: u_{t+1} at column j (0 on the rim columns)
+        constexpr int C1 = (4 + U) % 9;                 // q1 slot of column j
         float n1[VZ];
         vzero<VZ>(n1);
         if (j >= 4 && j < nxx - 4) {
             float ext[VZ + 8];
-            zext<VZ>(ext, q1[4], cur + (long)j * ldz, zb, lane, ldz);
+            zext<VZ>(ext, q1[C1], cur + (long)j * ldz, zb, lane, ldz);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float d = ABC ? damper(a.b1d, i, j, g) : 1.f;
-                float val = leapfrog<ABC>(q1[4][r], o1[r], cj[r], lap8<VZ>(q1, ext, r), d);
+                float val = leapfrog<ABC>(q1[C1][r], o1[r], cj[r], lap8r<VZ, U>(q1, ext, r), d);
                 if (i == siz && j == six) val += w1;
                 n1[r] = (i >= 4 && i < g.nzz - 4) ? val : o1[r];
             }
-            if (j >= x0 && j < x1) {
-                if (active) {
-#pragma unroll
-                    for (int r = 0; r < VZ; ++r)
-                        if ((own_mask >> r) & 1u) out1[(long)j * ldz + zb + r] = n1[r];
-                }
-                if (rec1) sample_receivers<VZ, ABC>(a, n1, j, zb, own_mask, seis, t + 1 - a.tlag);
+            if (own && j >= x0 && j < x1) {
+                vstore<VZ>(out1 + (long)j * ldz + zb, n1);
+                if (rec1) sample_receivers<VZ, ABC>(a, n1, j, zb, seis, t + 1 - a.tlag);
             }
         }
+        // push column j into the u_{t+1} and c queues (slot of logical column j-9)
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) q2[k][r] = q2[k + 1][r];
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) q2[8][r] = n1[r];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) cq[k][r] = cq[k + 1][r];
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) cq[4][r] = cj[r];
+        for (int r = 0; r < VZ; ++r) { q2[U % 9][r] = n1[r]; cq[U % 9][r] = cj[r]; }
 
-        // ---- step 2: u_{t+2} at column jj = j - 4 (an owned column)
+        // ---- step 2: u_{t+2} at column jj = j - 4
         const int jj = j - 4;
         if (jj >= x0) {
+            constexpr int C2 = (4 + U + 1) % 9;         // q2 / cq slot of column jj
+            constexpr int O2 = U % 9;                   // q1 slot of column jj (= j-4)
             float ext[VZ + 8];
-            zext_local<VZ>(ext, q2[4], lane);
+            zext_local<VZ>(ext, q2[C2], lane);
             float n2[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
-                const float o2 = q1[0][r];              // u_t at column jj
                 const float d = ABC ? damper(a.b1d, i, jj, g) : 1.f;
-                float val = leapfrog<ABC>(q2[4][r], o2, cq[0][r], lap8<VZ>(q2, ext, r), d);
+                float val = leapfrog<ABC>(q2[C2][r], q1[O2][r], cq[C2][r],
+                                          lap8r<VZ, U + 1>(q2, ext, r), d);
                 if (i == siz && jj == six) val += w2;
-                n2[r] = (i >= 4 && i < g.nzz - 4) ? val : o2;
+                n2[r] = (i >= 4 && i < g.nzz - 4) ? val : q1[O2][r];
             }
-            if (active) {
-#pragma unroll
-                for (int r = 0; r < VZ; ++r)
-                    if ((own_mask >> r) & 1u) out2[(long)jj * ldz + zb + r] = n2[r];
+            if (own) {
+                vstore<VZ>(out2 + (long)jj * ldz + zb, n2);
+                if (rec2) sample_receivers<VZ, ABC>(a, n2, jj, zb, seis, t + 2 - a.tlag);
             }
-            if (rec2) sample_receivers<VZ, ABC>(a, n2, jj, zb, own_mask, seis, t + 2 - a.tlag);
         }
+        // the u_t slot of column j-4 now takes column j+5
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int r = 0; r < VZ; ++r) q1[U % 9][r] = pn[r];
+    }
+};
+
+template <int VZ, bool ABC>
+__global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int t)
+{
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
+    Fwd2Wave<VZ, ABC> w(a);
+    w.g = a.g;
+    w.t = t;
+    w.lane = threadIdx.x;
+    const int ldz = w.g.ldz, nxx = w.g.nxx;
+    const int zs = id.zseg * (64 * VZ - 8) - 4;     // first computed row
+    w.zb = zs + w.lane * VZ;
+    w.active = (w.zb >= 0) && (w.zb < ldz);
+    // a lane owns either all or none of its rows (the bounds are multiples of VZ)
+    w.own = w.active && (w.zb >= zs + 4) && (w.zb < zs + 64 * VZ - 4);
+    w.x0 = 4 + id.xc * xw;
+    w.x1 = min(w.x0 + xw, nxx - 4);
+    const long soff = (long)id.slot * w.g.fstride;
+    w.cur = a.cur + soff;
+    w.old = a.old + soff;
+    w.out1 = a.out1 + soff;
+    w.out2 = a.out2 + soff;
+    w.cc = a.c + (long)id.slot * a.cstride;
+    const int gshot = *a.shot0 + id.slot;
+    w.siz = a.src_iz[gshot];
+    w.six = a.src_ix[gshot];
+    w.w1 = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
+    w.w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
+    w.rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    w.rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
+    w.seis = (w.rec1 || w.rec2) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
+
 #pragma unroll
-            for (int r = 0; r < VZ; ++r) q1[k][r] = q1[k + 1][r];
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) q1[8][r] = pn[r];
+    for (int k = 0; k < 9; ++k) {
+        ldcol<VZ>(w.q1[k], w.cur, w.x0 - 8 + k, nxx, ldz, w.zb, w.active);
+        vzero<VZ>(w.q2[k]);
+        vzero<VZ>(w.cq[k]);
+    }
+    ldcol<VZ>(w.on, w.old, w.x0 - 4, nxx, ldz, w.zb, w.active);
+    ldcol<VZ>(w.cn, w.cc, w.x0 - 4, nxx, ldz, w.zb, w.active);
+
+    const int jend = w.x1 + 4;
+    for (int j = w.x0 - 4; j < jend; j += 9) {
+        w.template column<0>(j);
+        if (j + 1 >= jend) break;
+        w.template column<1>(j + 1);
+        if (j + 2 >= jend) break;
+        w.template column<2>(j + 2);
+        if (j + 3 >= jend) break;
+        w.template column<3>(j + 3);
+        if (j + 4 >= jend) break;
+        w.template column<4>(j + 4);
+        if (j + 5 >= jend) break;
+        w.template column<5>(j + 5);
+        if (j + 6 >= jend) break;
+        w.template column<6>(j + 6);
+        if (j + 7 >= jend) break;
+        w.template column<7>(j + 7);
+        if (j + 8 >= jend) break;
+        w.template column<8>(j + 8);
     }
 }
 
@@ -437,31 +475,28 @@ template <int VZ, int MODE>
 __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int k)
 {
-    const int tile = xcd_tile(nzseg * nxchunk * nslots);
-    if (tile < 0) return;
-    const int zseg = tile % nzseg;
-    const int xc = (tile / nzseg) % nxchunk;
-    const int slot = tile / (nzseg * nxchunk);
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
     const Grid g = a.g;
     const int lane = threadIdx.x;
-    const int zb = zseg * 64 * VZ + lane * VZ;
+    const int zb = id.zseg * 64 * VZ + lane * VZ;
     const bool active = zb < g.ldz;
-    const int j0 = 4 + xc * xw;
+    const int j0 = 4 + id.xc * xw;
     const int j1 = min(j0 + xw, g.nxx - 4);
     const int ldz = g.ldz, nxx = g.nxx;
 
-    const long soff = (long)slot * g.fstride;
+    const long soff = (long)id.slot * g.fstride;
     const float *scur = a.scur + soff;
     const float *sold = a.sold + soff;
     float *sout = a.sout1 + soff;
     const float *rcur = a.rcur + soff;
     const float *rold = a.rold + soff;
     float *rout = a.rout1 + soff;
-    const float *cc = a.c + (long)slot * a.cstride;
+    const float *cc = a.c + (long)id.slot * a.cstride;
     float *acc = a.acc + soff;
     float *ill = a.ill + soff;
     const bool inj_on = (k + 1 < a.nt);
-    const int drow = a.data_by_shot ? (*a.shot0 + slot) : slot;
+    const int drow = a.data_by_shot ? (*a.shot0 + id.slot) : id.slot;
     const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
 
     float qs[9][VZ], qr[9][VZ];
@@ -502,15 +537,15 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
             const int i = zb + r;
             const bool inside = (i >= 4) && (i < g.nzz - 4);
             const float pr = qr[4][r];
-            const float lsc = c[r] * lap8<VZ>(qs, es, r);
-            const float sn = (qs[4][r] + (qs[4][r] - os[r])) + lsc;
-            const float rn = (pr + (pr - orr[r])) + c[r] * lap8<VZ>(qr, er, r);
+            const float lapS = lap8r<VZ, 0>(qs, es, r);
+            const float sn = fmaf(c[r], lapS, qs[4][r] + (qs[4][r] - os[r]));
+            const float rn = fmaf(c[r], lap8r<VZ, 0>(qr, er, r), pr + (pr - orr[r]));
             ns[r] = inside ? sn : os[r];
             nr[r] = inside ? rn : orr[r];
             if (inside) {
-                il[r] += sn * sn;
-                if constexpr (MODE == 0) ac[r] += sn * pr;
-                else ac[r] += pr * lsc;
+                il[r] = fmaf(sn, sn, il[r]);
+                if constexpr (MODE == 0) ac[r] = fmaf(sn, pr, ac[r]);
+                else ac[r] = fmaf(pr, c[r] * lapS, ac[r]);
             }
         }
         if (inj_on) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
@@ -528,175 +563,190 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
     }
 }
 
-// Two reverse steps per launch, tiled like fwd2_kernel, for both wavefields.
-// The step-1 image / illumination contributions wait in a 5-column queue so
-// the accumulators are read and written once per two steps.
+// Two reverse steps per launch, tiled and rotated like Fwd2Wave, for both
+// wavefields.  The step-1 accumulations are applied at column j to the
+// loaded image / illumination and the partial sums wait in a queue until
+// step 2 reaches the column, so the accumulators are read and written once
+// per two steps with the same FMA sequence as two single steps.
+template <int VZ, int MODE>
+struct Rev2Wave {
+    const RevStep &a;
+    Grid g;
+    int lane, zb, x0, x1, k;
+    bool active, own, inj1, inj2;
+    const float *scur, *sold, *rcur, *rold, *cc, *dat;
+    float *sout1, *sout2, *rout1, *rout2, *acc, *ill;
+    float qs1[9][VZ], qr1[9][VZ];   // S_cur, R_cur: logical m = column j-4+m
+    float qs2[9][VZ], qr2[9][VZ];   // step-1 S, R:   logical m = column j-8+m
+    float cq[9][VZ];                // c:             logical m = column j-8+m
+    float iq[9][VZ], aq[9][VZ];     // partial illum / image: logical m = column j-8+m
+    float osn[VZ], orn[VZ], cn[VZ], an[VZ], in[VZ];
+
+    __device__ Rev2Wave(const RevStep &a_) : a(a_) {}
+
+    template <int U>
+    __device__ __forceinline__ void column(int j)
+    {
+        const int ldz = g.ldz, nxx = g.nxx;
+        float os[VZ], orr[VZ], cj[VZ], aj[VZ], ij[VZ], psn[VZ], prn[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) {
+            os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r]; aj[r] = an[r]; ij[r] = in[r];
+        }
+        ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb, active);
+        ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb, active);
+        ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb, active);
+        ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb, active);
+        ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+        const bool acc_next = own && (j + 1 >= x0) && (j + 1 < x1);
+        ldcol<VZ>(an, acc, j + 1, nxx, ldz, zb, acc_next);
+        ldcol<VZ>(in, ill, j + 1, nxx, ldz, zb, acc_next);
+
+        // ---- step 1 at column j
+        constexpr int C1 = (4 + U) % 9;
+        float ns[VZ], nr[VZ];
+        vzero<VZ>(ns); vzero<VZ>(nr);
+        if (j >= 4 && j < nxx - 4) {
+            float es[VZ + 8], er[VZ + 8];
+            zext<VZ>(es, qs1[C1], scur + (long)j * ldz, zb, lane, ldz);
+            zext<VZ>(er, qr1[C1], rcur + (long)j * ldz, zb, lane, ldz);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = (i >= 4) && (i < g.nzz - 4);
+                const float ps = qs1[C1][r], pr = qr1[C1][r];
+                const float lapS = lap8r<VZ, U>(qs1, es, r);
+                const float sn = fmaf(cj[r], lapS, ps + (ps - os[r]));
+                const float rn = fmaf(cj[r], lap8r<VZ, U>(qr1, er, r), pr + (pr - orr[r]));
+                ns[r] = inside ? sn : os[r];
+                nr[r] = inside ? rn : orr[r];
+                if (inside) {
+                    ij[r] = fmaf(sn, sn, ij[r]);
+                    if constexpr (MODE == 0) aj[r] = fmaf(sn, pr, aj[r]);
+                    else aj[r] = fmaf(pr, cj[r] * lapS, aj[r]);
+                }
+            }
+            if (inj1) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
+            if (own && j >= x0 && j < x1) {
+                vstore<VZ>(sout1 + (long)j * ldz + zb, ns);
+                vstore<VZ>(rout1 + (long)j * ldz + zb, nr);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) {
+            qs2[U % 9][r] = ns[r]; qr2[U % 9][r] = nr[r]; cq[U % 9][r] = cj[r];
+            iq[U % 9][r] = ij[r]; aq[U % 9][r] = aj[r];
+        }
+
+        // ---- step 2 at column jj = j - 4
+        const int jj = j - 4;
+        if (jj >= x0) {
+            constexpr int C2 = (4 + U + 1) % 9;
+            constexpr int O2 = U % 9;
+            float es[VZ + 8], er[VZ + 8];
+            zext_local<VZ>(es, qs2[C2], lane);
+            zext_local<VZ>(er, qr2[C2], lane);
+            float ns2[VZ], nr2[VZ], il[VZ], ac[VZ];
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = (i >= 4) && (i < g.nzz - 4);
+                const float os2 = qs1[O2][r];           // S_cur at column jj
+                const float or2 = qr1[O2][r];           // R_k at column jj
+                const float ps = qs2[C2][r], pr = qr2[C2][r];
+                const float c2 = cq[C2][r];
+                const float lapS = lap8r<VZ, U + 1>(qs2, es, r);
+                const float sn = fmaf(c2, lapS, ps + (ps - os2));
+                const float rn = fmaf(c2, lap8r<VZ, U + 1>(qr2, er, r), pr + (pr - or2));
+                ns2[r] = inside ? sn : os2;
+                nr2[r] = inside ? rn : or2;
+                il[r] = iq[C2][r];
+                ac[r] = aq[C2][r];
+                if (inside) {
+                    il[r] = fmaf(sn, sn, il[r]);
+                    if constexpr (MODE == 0) ac[r] = fmaf(sn, pr, ac[r]);
+                    else ac[r] = fmaf(pr, c2 * lapS, ac[r]);
+                }
+            }
+            if (inj2) inject_receivers<VZ>(a, nr2, jj, zb, dat, a.nt - 3 - k);
+            if (own) {
+                const long off = (long)jj * ldz + zb;
+                vstore<VZ>(sout2 + off, ns2);
+                vstore<VZ>(rout2 + off, nr2);
+                vstore<VZ>(acc + off, ac);
+                vstore<VZ>(ill + off, il);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { qs1[U % 9][r] = psn[r]; qr1[U % 9][r] = prn[r]; }
+    }
+};
+
 template <int VZ, int MODE>
 __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int k)
 {
-    const int tile = xcd_tile(nzseg * nxchunk * nslots);
-    if (tile < 0) return;
-    const int zseg = tile % nzseg;
-    const int xc = (tile / nzseg) % nxchunk;
-    const int slot = tile / (nzseg * nxchunk);
-    const Grid g = a.g;
-    const int lane = threadIdx.x;
-    const int ldz = g.ldz, nxx = g.nxx;
-    const int zs = zseg * (64 * VZ - 8) - 4;
-    const int zb = zs + lane * VZ;
-    const bool active = (zb >= 0) && (zb < ldz);
-    unsigned own_mask = 0u;
-#pragma unroll
-    for (int r = 0; r < VZ; ++r) {
-        const int i = zb + r;
-        if (i >= zs + 4 && i < zs + 64 * VZ - 4 && i >= 0 && i < ldz) own_mask |= 1u << r;
-    }
-    const int x0 = 4 + xc * xw;
-    const int x1 = min(x0 + xw, nxx - 4);
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
+    Rev2Wave<VZ, MODE> w(a);
+    w.g = a.g;
+    w.k = k;
+    w.lane = threadIdx.x;
+    const int ldz = w.g.ldz, nxx = w.g.nxx;
+    const int zs = id.zseg * (64 * VZ - 8) - 4;
+    w.zb = zs + w.lane * VZ;
+    w.active = (w.zb >= 0) && (w.zb < ldz);
+    w.own = w.active && (w.zb >= zs + 4) && (w.zb < zs + 64 * VZ - 4);
+    w.x0 = 4 + id.xc * xw;
+    w.x1 = min(w.x0 + xw, nxx - 4);
+    const long soff = (long)id.slot * w.g.fstride;
+    w.scur = a.scur + soff;
+    w.sold = a.sold + soff;
+    w.sout1 = a.sout1 + soff;
+    w.sout2 = a.sout2 + soff;
+    w.rcur = a.rcur + soff;
+    w.rold = a.rold + soff;
+    w.rout1 = a.rout1 + soff;
+    w.rout2 = a.rout2 + soff;
+    w.cc = a.c + (long)id.slot * a.cstride;
+    w.acc = a.acc + soff;
+    w.ill = a.ill + soff;
+    w.inj1 = (k + 1 < a.nt);
+    w.inj2 = (k + 2 < a.nt);
+    const int drow = a.data_by_shot ? (*a.shot0 + id.slot) : id.slot;
+    w.dat = a.data + (long)drow * a.rec.nrec * a.nt;
 
-    const long soff = (long)slot * g.fstride;
-    const float *scur = a.scur + soff;
-    const float *sold = a.sold + soff;
-    float *sout1 = a.sout1 + soff;
-    float *sout2 = a.sout2 + soff;
-    const float *rcur = a.rcur + soff;
-    const float *rold = a.rold + soff;
-    float *rout1 = a.rout1 + soff;
-    float *rout2 = a.rout2 + soff;
-    const float *cc = a.c + (long)slot * a.cstride;
-    float *acc = a.acc + soff;
-    float *ill = a.ill + soff;
-    const bool inj1 = (k + 1 < a.nt);
-    const bool inj2 = (k + 2 < a.nt);
-    const int drow = a.data_by_shot ? (*a.shot0 + slot) : slot;
-    const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
-
-    float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur: columns j-4 .. j+4
-    float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R: columns j-8 .. j
-    float cq[5][VZ];                 // c: columns j-4 .. j
-    float iq[5][VZ], aq[5][VZ];      // step-1 illum / image contributions
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-        ldcol<VZ>(qs1[m], scur, x0 - 8 + m, nxx, ldz, zb, active);
-        ldcol<VZ>(qr1[m], rcur, x0 - 8 + m, nxx, ldz, zb, active);
-        vzero<VZ>(qs2[m]); vzero<VZ>(qr2[m]);
+        ldcol<VZ>(w.qs1[m], w.scur, w.x0 - 8 + m, nxx, ldz, w.zb, w.active);
+        ldcol<VZ>(w.qr1[m], w.rcur, w.x0 - 8 + m, nxx, ldz, w.zb, w.active);
+        vzero<VZ>(w.qs2[m]); vzero<VZ>(w.qr2[m]); vzero<VZ>(w.cq[m]);
+        vzero<VZ>(w.iq[m]); vzero<VZ>(w.aq[m]);
     }
-#pragma unroll
-    for (int m = 0; m < 5; ++m) { vzero<VZ>(cq[m]); vzero<VZ>(iq[m]); vzero<VZ>(aq[m]); }
-    float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ];
-    ldcol<VZ>(osn, sold, x0 - 4, nxx, ldz, zb, active);
-    ldcol<VZ>(orn, rold, x0 - 4, nxx, ldz, zb, active);
-    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb, active);
-    vzero<VZ>(psn); vzero<VZ>(prn);
+    ldcol<VZ>(w.osn, w.sold, w.x0 - 4, nxx, ldz, w.zb, w.active);
+    ldcol<VZ>(w.orn, w.rold, w.x0 - 4, nxx, ldz, w.zb, w.active);
+    ldcol<VZ>(w.cn, w.cc, w.x0 - 4, nxx, ldz, w.zb, w.active);
+    vzero<VZ>(w.an); vzero<VZ>(w.in);      // column x0-4 is never an owned column
 
-    for (int j = x0 - 4; j < x1 + 4; ++j) {
-        float os[VZ], orr[VZ], cj[VZ];
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) { os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r]; }
-        const int jj = j - 4;
-        float ac[VZ], il[VZ];
-        if (jj >= x0) {
-            ldcol<VZ>(ac, acc, jj, nxx, ldz, zb, active);
-            ldcol<VZ>(il, ill, jj, nxx, ldz, zb, active);
-        }
-        if (j + 1 < x1 + 4) {
-            ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb, active);
-            ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb, active);
-            ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
-        }
-        // ---- step 1 at column j
-        float ns[VZ], nr[VZ], ci[VZ], ca[VZ];
-        vzero<VZ>(ns); vzero<VZ>(nr); vzero<VZ>(ci); vzero<VZ>(ca);
-        if (j >= 4 && j < nxx - 4) {
-            float es[VZ + 8], er[VZ + 8];
-            zext<VZ>(es, qs1[4], scur + (long)j * ldz, zb, lane, ldz);
-            zext<VZ>(er, qr1[4], rcur + (long)j * ldz, zb, lane, ldz);
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) {
-                const int i = zb + r;
-                const bool inside = (i >= 4) && (i < g.nzz - 4);
-                const float pr = qr1[4][r];
-                const float lsc = cj[r] * lap8<VZ>(qs1, es, r);
-                const float sn = (qs1[4][r] + (qs1[4][r] - os[r])) + lsc;
-                const float rn = (pr + (pr - orr[r])) + cj[r] * lap8<VZ>(qr1, er, r);
-                ns[r] = inside ? sn : os[r];
-                nr[r] = inside ? rn : orr[r];
-                if (inside) {
-                    ci[r] = sn * sn;
-                    ca[r] = (MODE == 0) ? sn * pr : pr * lsc;
-                }
-            }
-            if (inj1) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
-            if (j >= x0 && j < x1 && active) {
-#pragma unroll
-                for (int r = 0; r < VZ; ++r)
-                    if ((own_mask >> r) & 1u) {
-                        sout1[(long)j * ldz + zb + r] = ns[r];
-                        rout1[(long)j * ldz + zb + r] = nr[r];
-                    }
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < 8; ++m)
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) { qs2[m][r] = qs2[m + 1][r]; qr2[m][r] = qr2[m + 1][r]; }
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) { qs2[8][r] = ns[r]; qr2[8][r] = nr[r]; }
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) {
-                cq[m][r] = cq[m + 1][r]; iq[m][r] = iq[m + 1][r]; aq[m][r] = aq[m + 1][r];
-            }
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) { cq[4][r] = cj[r]; iq[4][r] = ci[r]; aq[4][r] = ca[r]; }
-
-        // ---- step 2 at column jj
-        if (jj >= x0) {
-            float es[VZ + 8], er[VZ + 8];
-            zext_local<VZ>(es, qs2[4], lane);
-            zext_local<VZ>(er, qr2[4], lane);
-            float ns2[VZ], nr2[VZ];
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) {
-                const int i = zb + r;
-                const bool inside = (i >= 4) && (i < g.nzz - 4);
-                const float os2 = qs1[0][r];            // S_cur at column jj
-                const float or2 = qr1[0][r];            // R_k at column jj
-                const float pr = qr2[4][r];             // R_{k+1}
-                const float lsc = cq[0][r] * lap8<VZ>(qs2, es, r);
-                const float sn = (qs2[4][r] + (qs2[4][r] - os2)) + lsc;
-                const float rn = (pr + (pr - or2)) + cq[0][r] * lap8<VZ>(qr2, er, r);
-                ns2[r] = inside ? sn : os2;
-                nr2[r] = inside ? rn : or2;
-                if (inside) {
-                    il[r] = (il[r] + iq[0][r]) + sn * sn;
-                    if constexpr (MODE == 0) ac[r] = (ac[r] + aq[0][r]) + sn * pr;
-                    else ac[r] = (ac[r] + aq[0][r]) + pr * lsc;
-                }
-            }
-            if (inj2) inject_receivers<VZ>(a, nr2, jj, zb, dat, a.nt - 3 - k);
-            if (active) {
-#pragma unroll
-                for (int r = 0; r < VZ; ++r)
-                    if ((own_mask >> r) & 1u) {
-                        const long off = (long)jj * ldz + zb + r;
-                        sout2[off] = ns2[r];
-                        rout2[off] = nr2[r];
-                        acc[off] = ac[r];
-                        ill[off] = il[r];
-                    }
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < 8; ++m)
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) { qs1[m][r] = qs1[m + 1][r]; qr1[m][r] = qr1[m + 1][r]; }
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) { qs1[8][r] = psn[r]; qr1[8][r] = prn[r]; }
+    const int jend = w.x1 + 4;
+    for (int j = w.x0 - 4; j < jend; j += 9) {
+        w.template column<0>(j);
+        if (j + 1 >= jend) break;
+        w.template column<1>(j + 1);
+        if (j + 2 >= jend) break;
+        w.template column<2>(j + 2);
+        if (j + 3 >= jend) break;
+        w.template column<3>(j + 3);
+        if (j + 4 >= jend) break;
+        w.template column<4>(j + 4);
+        if (j + 5 >= jend) break;
+        w.template column<5>(j + 5);
+        if (j + 6 >= jend) break;
+        w.template column<6>(j + 6);
+        if (j + 7 >= jend) break;
+        w.template column<7>(j + 7);
+        if (j + 8 >= jend) break;
+        w.template column<8>(j + 8);
     }
 }
 
