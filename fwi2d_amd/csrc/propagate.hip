@@ -137,10 +137,10 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
     return logical < ntiles ? logical : -1;
 }
 
-// Laplacian (times dh^2) from a 9-column register queue whose logical column
-// m (j-4+m) sits in slot (m + OFF) % 9, and the z extension of the centre
-// column, summed as centre differences; the weight sum is one explicit FMA
-// chain so that every kernel rounds it identically.
+// Laplacian (times dh^2) from a 9-column register queue whose column j-4+m
+// sits in slot (m + OFF) % 9, and the z extension of the centre column,
+// summed as centre differences; the weight sum is one explicit FMA chain so
+// that every kernel rounds it identically.
 template <int VZ, int OFF>
 __device__ __forceinline__ float lap8r(const float (&q)[9][VZ], const float (&ext)[VZ + 8],
                                        int r)
@@ -315,53 +315,77 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
 }
 
 // Two time steps per launch (temporal blocking).  The wave computes u_{t+1}
-// on 64*VZ rows and on its x-chunk plus 4 columns each side, and u_{t+2} on
-// the inner 64*VZ - 8 rows of the chunk.  It reads u_t, u_{t-1} and c once
-// for both steps and writes u_{t+1} (out1) and u_{t+2} (out2) for the cells
-// it owns; the four buffers are distinct, so no wave reads what another wave
-// of the same launch writes.  The column loop is unrolled by 9 and the three
-// register queues rotate by index (slot of logical column m at phase U is
-// (m + U) % 9), so advancing a column moves no registers.
+// on 64*VZ rows and on its x-chunk plus 4 columns each side (kept in a second
+// register queue), then u_{t+2} on the inner 64*VZ - 8 rows of the chunk.
+// It reads u_t, u_{t-1} and c once for both steps and writes u_{t+1} (out1)
+// and u_{t+2} (out2) for the cells it owns; the four buffers are distinct,
+// so no wave reads a value another wave of the same launch writes.
 template <int VZ, bool ABC>
-struct Fwd2Wave {
-    const FwdStep &a;
-    Grid g;
-    int lane, zb, x0, x1, siz, six, t;
-    bool active, own, rec1, rec2;
-    float w1, w2;
-    const float *cur, *old, *cc;
-    float *out1, *out2, *seis;
-    float q1[9][VZ];     // u_t,     logical m = column j-4+m
-    float q2[9][VZ];     // u_{t+1}, logical m = column j-8+m (after the push)
-    float cq[9][VZ];     // c,       logical m = column j-8+m (after the push)
-    float on[VZ], cn[VZ];
+__global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int t)
+{
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int ldz = g.ldz, nxx = g.nxx;
+    const int zs = id.zseg * (64 * VZ - 8) - 4;     // first computed row
+    const int zb = zs + lane * VZ;
+    const bool active = (zb >= 0) && (zb < ldz);
+    // a lane owns all or none of its rows (the bounds are multiples of VZ)
+    const bool own = active && (zb >= zs + 4) && (zb < zs + 64 * VZ - 4);
+    const int x0 = 4 + id.xc * xw;
+    const int x1 = min(x0 + xw, nxx - 4);
 
-    __device__ Fwd2Wave(const FwdStep &a_) : a(a_) {}
+    const long soff = (long)id.slot * g.fstride;
+    const float *cur = a.cur + soff;
+    const float *old = a.old + soff;
+    float *out1 = a.out1 + soff;
+    float *out2 = a.out2 + soff;
+    const float *cc = a.c + (long)id.slot * a.cstride;
+    const int gshot = *a.shot0 + id.slot;
+    const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
+    const float w1 = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
+    const float w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
+    const bool rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
+    float *seis = (rec1 || rec2) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
 
-    template <int U>
-    __device__ __forceinline__ void column(int j)
-    {
-        const int ldz = g.ldz, nxx = g.nxx;
-        float o1[VZ], cj[VZ], pn[VZ];
+    float q1[9][VZ];     // u_t, columns j-4 .. j+4
+    float q2[9][VZ];     // u_{t+1}, columns j-8 .. j
+    float cq[5][VZ];     // c, columns j-4 .. j
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        ldcol<VZ>(q1[k], cur, x0 - 8 + k, nxx, ldz, zb, active);
+        vzero<VZ>(q2[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) vzero<VZ>(cq[k]);
+    float on[VZ], cn[VZ], pn[VZ];
+    ldcol<VZ>(on, old, x0 - 4, nxx, ldz, zb, active);
+    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb, active);
+    vzero<VZ>(pn);
+
+    for (int j = x0 - 4; j < x1 + 4; ++j) {
+        float o1[VZ], cj[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
-        // prefetch: u_t at j+5, u_{t-1} and c at j+1
-        ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb, active);
-        ldcol<VZ>(on, old, j + 1, nxx, ldz, zb, active);
-        ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
-
+        if (j + 1 < x1 + 4) {
+            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb, active);
+            ldcol<VZ>(on, old, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+        }
         // ---- step 1: u_{t+1} at column j (0 on the rim columns)
-        constexpr int C1 = (4 + U) % 9;                 // q1 slot of column j
         float n1[VZ];
         vzero<VZ>(n1);
         if (j >= 4 && j < nxx - 4) {
             float ext[VZ + 8];
-            zext<VZ>(ext, q1[C1], cur + (long)j * ldz, zb, lane, ldz);
+            zext<VZ>(ext, q1[4], cur + (long)j * ldz, zb, lane, ldz);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float d = ABC ? damper(a.b1d, i, j, g) : 1.f;
-                float val = leapfrog<ABC>(q1[C1][r], o1[r], cj[r], lap8r<VZ, U>(q1, ext, r), d);
+                float val = leapfrog<ABC>(q1[4][r], o1[r], cj[r], lap8r<VZ, 0>(q1, ext, r), d);
                 if (i == siz && j == six) val += w1;
                 n1[r] = (i >= 4 && i < g.nzz - 4) ? val : o1[r];
             }
@@ -370,99 +394,43 @@ struct Fwd2Wave {
                 if (rec1) sample_receivers<VZ, ABC>(a, n1, j, zb, seis, t + 1 - a.tlag);
             }
         }
-        // push column j into the u_{t+1} and c queues (slot of logical column j-9)
 #pragma unroll
-        for (int r = 0; r < VZ; ++r) { q2[U % 9][r] = n1[r]; cq[U % 9][r] = cj[r]; }
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) q2[k][r] = q2[k + 1][r];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) cq[k][r] = cq[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { q2[8][r] = n1[r]; cq[4][r] = cj[r]; }
 
-        // ---- step 2: u_{t+2} at column jj = j - 4
+        // ---- step 2: u_{t+2} at column jj = j - 4 (an owned column)
         const int jj = j - 4;
         if (jj >= x0) {
-            constexpr int C2 = (4 + U + 1) % 9;         // q2 / cq slot of column jj
-            constexpr int O2 = U % 9;                   // q1 slot of column jj (= j-4)
             float ext[VZ + 8];
-            zext_local<VZ>(ext, q2[C2], lane);
+            zext_local<VZ>(ext, q2[4], lane);
             float n2[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
+                const float o2 = q1[0][r];              // u_t at column jj
                 const float d = ABC ? damper(a.b1d, i, jj, g) : 1.f;
-                float val = leapfrog<ABC>(q2[C2][r], q1[O2][r], cq[C2][r],
-                                          lap8r<VZ, U + 1>(q2, ext, r), d);
+                float val = leapfrog<ABC>(q2[4][r], o2, cq[0][r], lap8r<VZ, 0>(q2, ext, r), d);
                 if (i == siz && jj == six) val += w2;
-                n2[r] = (i >= 4 && i < g.nzz - 4) ? val : q1[O2][r];
+                n2[r] = (i >= 4 && i < g.nzz - 4) ? val : o2;
             }
             if (own) {
                 vstore<VZ>(out2 + (long)jj * ldz + zb, n2);
                 if (rec2) sample_receivers<VZ, ABC>(a, n2, jj, zb, seis, t + 2 - a.tlag);
             }
         }
-        // the u_t slot of column j-4 now takes column j+5
 #pragma unroll
-        for (int r = 0; r < VZ; ++r) q1[U % 9][r] = pn[r];
-    }
-};
-
-template <int VZ, bool ABC>
-__global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
-                                                  int nslots, int t)
-{
-    TileId id;
-    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
-    Fwd2Wave<VZ, ABC> w(a);
-    w.g = a.g;
-    w.t = t;
-    w.lane = threadIdx.x;
-    const int ldz = w.g.ldz, nxx = w.g.nxx;
-    const int zs = id.zseg * (64 * VZ - 8) - 4;     // first computed row
-    w.zb = zs + w.lane * VZ;
-    w.active = (w.zb >= 0) && (w.zb < ldz);
-    // a lane owns either all or none of its rows (the bounds are multiples of VZ)
-    w.own = w.active && (w.zb >= zs + 4) && (w.zb < zs + 64 * VZ - 4);
-    w.x0 = 4 + id.xc * xw;
-    w.x1 = min(w.x0 + xw, nxx - 4);
-    const long soff = (long)id.slot * w.g.fstride;
-    w.cur = a.cur + soff;
-    w.old = a.old + soff;
-    w.out1 = a.out1 + soff;
-    w.out2 = a.out2 + soff;
-    w.cc = a.c + (long)id.slot * a.cstride;
-    const int gshot = *a.shot0 + id.slot;
-    w.siz = a.src_iz[gshot];
-    w.six = a.src_ix[gshot];
-    w.w1 = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
-    w.w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
-    w.rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
-    w.rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
-    w.seis = (w.rec1 || w.rec2) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
-
+        for (int k = 0; k < 8; ++k)
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        ldcol<VZ>(w.q1[k], w.cur, w.x0 - 8 + k, nxx, ldz, w.zb, w.active);
-        vzero<VZ>(w.q2[k]);
-        vzero<VZ>(w.cq[k]);
-    }
-    ldcol<VZ>(w.on, w.old, w.x0 - 4, nxx, ldz, w.zb, w.active);
-    ldcol<VZ>(w.cn, w.cc, w.x0 - 4, nxx, ldz, w.zb, w.active);
-
-    const int jend = w.x1 + 4;
-    for (int j = w.x0 - 4; j < jend; j += 9) {
-        w.template column<0>(j);
-        if (j + 1 >= jend) break;
-        w.template column<1>(j + 1);
-        if (j + 2 >= jend) break;
-        w.template column<2>(j + 2);
-        if (j + 3 >= jend) break;
-        w.template column<3>(j + 3);
-        if (j + 4 >= jend) break;
-        w.template column<4>(j + 4);
-        if (j + 5 >= jend) break;
-        w.template column<5>(j + 5);
-        if (j + 6 >= jend) break;
-        w.template column<6>(j + 6);
-        if (j + 7 >= jend) break;
-        w.template column<7>(j + 7);
-        if (j + 8 >= jend) break;
-        w.template column<8>(j + 8);
+            for (int r = 0; r < VZ; ++r) q1[k][r] = q1[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) q1[8][r] = pn[r];
     }
 }
 
@@ -563,67 +531,98 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
     }
 }
 
-// Two reverse steps per launch, tiled and rotated like Fwd2Wave, for both
-// wavefields.  The step-1 accumulations are applied at column j to the
-// loaded image / illumination and the partial sums wait in a queue until
-// step 2 reaches the column, so the accumulators are read and written once
-// per two steps with the same FMA sequence as two single steps.
+// Two reverse steps per launch, tiled like fwd2_kernel, for both wavefields.
+// The step-1 accumulations are applied at column j to the loaded image /
+// illumination, and the partial sums wait in a 5-column queue until step 2
+// reaches the column, so the accumulators are read and written once per two
+// steps with the same FMA sequence as two single steps.
 template <int VZ, int MODE>
-struct Rev2Wave {
-    const RevStep &a;
-    Grid g;
-    int lane, zb, x0, x1, k;
-    bool active, own, inj1, inj2;
-    const float *scur, *sold, *rcur, *rold, *cc, *dat;
-    float *sout1, *sout2, *rout1, *rout2, *acc, *ill;
-    float qs1[9][VZ], qr1[9][VZ];   // S_cur, R_cur: logical m = column j-4+m
-    float qs2[9][VZ], qr2[9][VZ];   // step-1 S, R:   logical m = column j-8+m
-    float cq[9][VZ];                // c:             logical m = column j-8+m
-    float iq[9][VZ], aq[9][VZ];     // partial illum / image: logical m = column j-8+m
-    float osn[VZ], orn[VZ], cn[VZ], an[VZ], in[VZ];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VZ == 4 ? 1 : 3)))
+void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw, int nslots, int k)
+{
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int ldz = g.ldz, nxx = g.nxx;
+    const int zs = id.zseg * (64 * VZ - 8) - 4;
+    const int zb = zs + lane * VZ;
+    const bool active = (zb >= 0) && (zb < ldz);
+    const bool own = active && (zb >= zs + 4) && (zb < zs + 64 * VZ - 4);
+    const int x0 = 4 + id.xc * xw;
+    const int x1 = min(x0 + xw, nxx - 4);
 
-    __device__ Rev2Wave(const RevStep &a_) : a(a_) {}
+    const long soff = (long)id.slot * g.fstride;
+    const float *scur = a.scur + soff;
+    const float *sold = a.sold + soff;
+    float *sout1 = a.sout1 + soff;
+    float *sout2 = a.sout2 + soff;
+    const float *rcur = a.rcur + soff;
+    const float *rold = a.rold + soff;
+    float *rout1 = a.rout1 + soff;
+    float *rout2 = a.rout2 + soff;
+    const float *cc = a.c + (long)id.slot * a.cstride;
+    float *acc = a.acc + soff;
+    float *ill = a.ill + soff;
+    const bool inj1 = (k + 1 < a.nt);
+    const bool inj2 = (k + 2 < a.nt);
+    const int drow = a.data_by_shot ? (*a.shot0 + id.slot) : id.slot;
+    const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
 
-    template <int U>
-    __device__ __forceinline__ void column(int j)
-    {
-        const int ldz = g.ldz, nxx = g.nxx;
-        float os[VZ], orr[VZ], cj[VZ], aj[VZ], ij[VZ], psn[VZ], prn[VZ];
+    float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur: columns j-4 .. j+4
+    float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R: columns j-8 .. j
+    float cq[5][VZ];                 // c: columns j-4 .. j
+    float iq[5][VZ], aq[5][VZ];      // partial illum / image: columns j-4 .. j
 #pragma unroll
-        for (int r = 0; r < VZ; ++r) {
-            os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r]; aj[r] = an[r]; ij[r] = in[r];
-        }
-        ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb, active);
-        ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb, active);
-        ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb, active);
-        ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb, active);
-        ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
-        const bool acc_next = own && (j + 1 >= x0) && (j + 1 < x1);
-        ldcol<VZ>(an, acc, j + 1, nxx, ldz, zb, acc_next);
-        ldcol<VZ>(in, ill, j + 1, nxx, ldz, zb, acc_next);
+    for (int m = 0; m < 9; ++m) {
+        ldcol<VZ>(qs1[m], scur, x0 - 8 + m, nxx, ldz, zb, active);
+        ldcol<VZ>(qr1[m], rcur, x0 - 8 + m, nxx, ldz, zb, active);
+        vzero<VZ>(qs2[m]); vzero<VZ>(qr2[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) { vzero<VZ>(cq[m]); vzero<VZ>(iq[m]); vzero<VZ>(aq[m]); }
+    float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ];
+    ldcol<VZ>(osn, sold, x0 - 4, nxx, ldz, zb, active);
+    ldcol<VZ>(orn, rold, x0 - 4, nxx, ldz, zb, active);
+    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb, active);
+    vzero<VZ>(psn); vzero<VZ>(prn);
 
+    for (int j = x0 - 4; j < x1 + 4; ++j) {
+        float os[VZ], orr[VZ], cj[VZ], ac1[VZ], il1[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r]; }
+        // accumulators of an owned column (the other lanes / columns carry zeros)
+        const bool acc_here = own && (j >= x0) && (j < x1);
+        ldcol<VZ>(ac1, acc, j, nxx, ldz, zb, acc_here);
+        ldcol<VZ>(il1, ill, j, nxx, ldz, zb, acc_here);
+        if (j + 1 < x1 + 4) {
+            ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb, active);
+            ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb, active);
+            ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+        }
         // ---- step 1 at column j
-        constexpr int C1 = (4 + U) % 9;
         float ns[VZ], nr[VZ];
         vzero<VZ>(ns); vzero<VZ>(nr);
         if (j >= 4 && j < nxx - 4) {
             float es[VZ + 8], er[VZ + 8];
-            zext<VZ>(es, qs1[C1], scur + (long)j * ldz, zb, lane, ldz);
-            zext<VZ>(er, qr1[C1], rcur + (long)j * ldz, zb, lane, ldz);
+            zext<VZ>(es, qs1[4], scur + (long)j * ldz, zb, lane, ldz);
+            zext<VZ>(er, qr1[4], rcur + (long)j * ldz, zb, lane, ldz);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const bool inside = (i >= 4) && (i < g.nzz - 4);
-                const float ps = qs1[C1][r], pr = qr1[C1][r];
-                const float lapS = lap8r<VZ, U>(qs1, es, r);
+                const float ps = qs1[4][r], pr = qr1[4][r];
+                const float lapS = lap8r<VZ, 0>(qs1, es, r);
                 const float sn = fmaf(cj[r], lapS, ps + (ps - os[r]));
-                const float rn = fmaf(cj[r], lap8r<VZ, U>(qr1, er, r), pr + (pr - orr[r]));
+                const float rn = fmaf(cj[r], lap8r<VZ, 0>(qr1, er, r), pr + (pr - orr[r]));
                 ns[r] = inside ? sn : os[r];
                 nr[r] = inside ? rn : orr[r];
                 if (inside) {
-                    ij[r] = fmaf(sn, sn, ij[r]);
-                    if constexpr (MODE == 0) aj[r] = fmaf(sn, pr, aj[r]);
-                    else aj[r] = fmaf(pr, cj[r] * lapS, aj[r]);
+                    il1[r] = fmaf(sn, sn, il1[r]);
+                    if constexpr (MODE == 0) ac1[r] = fmaf(sn, pr, ac1[r]);
+                    else ac1[r] = fmaf(pr, cj[r] * lapS, ac1[r]);
                 }
             }
             if (inj1) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
@@ -633,35 +632,43 @@ struct Rev2Wave {
             }
         }
 #pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) { qs2[m][r] = qs2[m + 1][r]; qr2[m][r] = qr2[m + 1][r]; }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                cq[m][r] = cq[m + 1][r]; iq[m][r] = iq[m + 1][r]; aq[m][r] = aq[m + 1][r];
+            }
+#pragma unroll
         for (int r = 0; r < VZ; ++r) {
-            qs2[U % 9][r] = ns[r]; qr2[U % 9][r] = nr[r]; cq[U % 9][r] = cj[r];
-            iq[U % 9][r] = ij[r]; aq[U % 9][r] = aj[r];
+            qs2[8][r] = ns[r]; qr2[8][r] = nr[r];
+            cq[4][r] = cj[r]; iq[4][r] = il1[r]; aq[4][r] = ac1[r];
         }
 
         // ---- step 2 at column jj = j - 4
         const int jj = j - 4;
         if (jj >= x0) {
-            constexpr int C2 = (4 + U + 1) % 9;
-            constexpr int O2 = U % 9;
             float es[VZ + 8], er[VZ + 8];
-            zext_local<VZ>(es, qs2[C2], lane);
-            zext_local<VZ>(er, qr2[C2], lane);
+            zext_local<VZ>(es, qs2[4], lane);
+            zext_local<VZ>(er, qr2[4], lane);
             float ns2[VZ], nr2[VZ], il[VZ], ac[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const bool inside = (i >= 4) && (i < g.nzz - 4);
-                const float os2 = qs1[O2][r];           // S_cur at column jj
-                const float or2 = qr1[O2][r];           // R_k at column jj
-                const float ps = qs2[C2][r], pr = qr2[C2][r];
-                const float c2 = cq[C2][r];
-                const float lapS = lap8r<VZ, U + 1>(qs2, es, r);
+                const float os2 = qs1[0][r];            // S_cur at column jj
+                const float or2 = qr1[0][r];            // R_k at column jj
+                const float ps = qs2[4][r], pr = qr2[4][r];
+                const float c2 = cq[0][r];
+                const float lapS = lap8r<VZ, 0>(qs2, es, r);
                 const float sn = fmaf(c2, lapS, ps + (ps - os2));
-                const float rn = fmaf(c2, lap8r<VZ, U + 1>(qr2, er, r), pr + (pr - or2));
+                const float rn = fmaf(c2, lap8r<VZ, 0>(qr2, er, r), pr + (pr - or2));
                 ns2[r] = inside ? sn : os2;
                 nr2[r] = inside ? rn : or2;
-                il[r] = iq[C2][r];
-                ac[r] = aq[C2][r];
+                il[r] = iq[0][r];
+                ac[r] = aq[0][r];
                 if (inside) {
                     il[r] = fmaf(sn, sn, il[r]);
                     if constexpr (MODE == 0) ac[r] = fmaf(sn, pr, ac[r]);
@@ -678,75 +685,11 @@ struct Rev2Wave {
             }
         }
 #pragma unroll
-        for (int r = 0; r < VZ; ++r) { qs1[U % 9][r] = psn[r]; qr1[U % 9][r] = prn[r]; }
-    }
-};
-
-template <int VZ, int MODE>
-__global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw,
-                                                  int nslots, int k)
-{
-    TileId id;
-    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
-    Rev2Wave<VZ, MODE> w(a);
-    w.g = a.g;
-    w.k = k;
-    w.lane = threadIdx.x;
-    const int ldz = w.g.ldz, nxx = w.g.nxx;
-    const int zs = id.zseg * (64 * VZ - 8) - 4;
-    w.zb = zs + w.lane * VZ;
-    w.active = (w.zb >= 0) && (w.zb < ldz);
-    w.own = w.active && (w.zb >= zs + 4) && (w.zb < zs + 64 * VZ - 4);
-    w.x0 = 4 + id.xc * xw;
-    w.x1 = min(w.x0 + xw, nxx - 4);
-    const long soff = (long)id.slot * w.g.fstride;
-    w.scur = a.scur + soff;
-    w.sold = a.sold + soff;
-    w.sout1 = a.sout1 + soff;
-    w.sout2 = a.sout2 + soff;
-    w.rcur = a.rcur + soff;
-    w.rold = a.rold + soff;
-    w.rout1 = a.rout1 + soff;
-    w.rout2 = a.rout2 + soff;
-    w.cc = a.c + (long)id.slot * a.cstride;
-    w.acc = a.acc + soff;
-    w.ill = a.ill + soff;
-    w.inj1 = (k + 1 < a.nt);
-    w.inj2 = (k + 2 < a.nt);
-    const int drow = a.data_by_shot ? (*a.shot0 + id.slot) : id.slot;
-    w.dat = a.data + (long)drow * a.rec.nrec * a.nt;
-
+        for (int m = 0; m < 8; ++m)
 #pragma unroll
-    for (int m = 0; m < 9; ++m) {
-        ldcol<VZ>(w.qs1[m], w.scur, w.x0 - 8 + m, nxx, ldz, w.zb, w.active);
-        ldcol<VZ>(w.qr1[m], w.rcur, w.x0 - 8 + m, nxx, ldz, w.zb, w.active);
-        vzero<VZ>(w.qs2[m]); vzero<VZ>(w.qr2[m]); vzero<VZ>(w.cq[m]);
-        vzero<VZ>(w.iq[m]); vzero<VZ>(w.aq[m]);
-    }
-    ldcol<VZ>(w.osn, w.sold, w.x0 - 4, nxx, ldz, w.zb, w.active);
-    ldcol<VZ>(w.orn, w.rold, w.x0 - 4, nxx, ldz, w.zb, w.active);
-    ldcol<VZ>(w.cn, w.cc, w.x0 - 4, nxx, ldz, w.zb, w.active);
-    vzero<VZ>(w.an); vzero<VZ>(w.in);      // column x0-4 is never an owned column
-
-    const int jend = w.x1 + 4;
-    for (int j = w.x0 - 4; j < jend; j += 9) {
-        w.template column<0>(j);
-        if (j + 1 >= jend) break;
-        w.template column<1>(j + 1);
-        if (j + 2 >= jend) break;
-        w.template column<2>(j + 2);
-        if (j + 3 >= jend) break;
-        w.template column<3>(j + 3);
-        if (j + 4 >= jend) break;
-        w.template column<4>(j + 4);
-        if (j + 5 >= jend) break;
-        w.template column<5>(j + 5);
-        if (j + 6 >= jend) break;
-        w.template column<6>(j + 6);
-        if (j + 7 >= jend) break;
-        w.template column<7>(j + 7);
-        if (j + 8 >= jend) break;
-        w.template column<8>(j + 8);
+            for (int r = 0; r < VZ; ++r) { qs1[m][r] = qs1[m + 1][r]; qr1[m][r] = qr1[m + 1][r]; }
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { qs1[8][r] = psn[r]; qr1[8][r] = prn[r]; }
     }
 }
 
