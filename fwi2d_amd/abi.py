@@ -84,6 +84,20 @@ SIGNATURES = [
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
 ]
 
+NN_SIGNATURES = [
+    ("fwi_nn_create", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                C.POINTER(C.c_void_p)]),
+    ("fwi_nn_destroy", C.c_int, [_ctx]),
+    ("fwi_nn_num_params", C.c_int, [_ctx, C.POINTER(C.c_int64)]),
+    ("fwi_nn_get_params", C.c_int, [_ctx, _f32p]),
+    ("fwi_nn_set_params", C.c_int, [_ctx, _f32p]),
+    ("fwi_nn_forward", C.c_int, [_ctx, C.c_int32, _f32p, _f32p]),
+    ("fwi_nn_gradient", C.c_int, [_ctx, C.c_int32, _f32p, _f32p, C.POINTER(C.c_float), _vp]),
+    ("fwi_nn_train_step", C.c_int, [_ctx, C.c_int32, _f32p, _f32p, C.c_float,
+                                    C.POINTER(C.c_float)]),
+    ("fwi_nn_mfma_probe", C.c_int, [_f32p, _f32p, _f32p]),
+]
+
 _lib = None
 
 
@@ -96,7 +110,7 @@ def lib():
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
                 f"g.build()'` or `make -C fwi2d_amd/csrc`")
         L = C.CDLL(LIB_PATH)
-        for name, res, args in SIGNATURES:
+        for name, res, args in SIGNATURES + NN_SIGNATURES:
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -260,3 +274,65 @@ class Engine:
         ms, n, cells = C.c_double(), C.c_int64(), C.c_double()
         _check(lib().fwi_profile_read(self._h, kind, C.byref(ms), C.byref(n), C.byref(cells)))
         return ms.value, n.value, cells.value
+
+
+def mfma_probe(a, b):
+    """d = a @ b through one bf16 MFMA (a: 16x32, b: 32x16)."""
+    d = np.zeros((16, 16), np.float32)
+    _check(lib().fwi_nn_mfma_probe(np.ascontiguousarray(a, np.float32),
+                                   np.ascontiguousarray(b, np.float32), d))
+    return d
+
+
+class Net:
+    """The learned preconditioner network on the GPU (owner of an ``fwi_nn``)."""
+
+    def __init__(self, batch, height, width, seed=0):
+        h = C.c_void_p()
+        _check(lib().fwi_nn_create(batch, height, width, seed, C.byref(h)))
+        self._h = h
+        self.batch, self.height, self.width = batch, height, width
+        n = C.c_int64()
+        _check(lib().fwi_nn_num_params(h, C.byref(n)))
+        self.nparams = n.value
+
+    def close(self):
+        if self._h:
+            lib().fwi_nn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparams, np.float32)
+        _check(lib().fwi_nn_get_params(self._h, p))
+        return p
+
+    def set_params(self, p):
+        _check(lib().fwi_nn_set_params(self._h, np.ascontiguousarray(p, np.float32)))
+
+    def forward(self, x):
+        """x: [nb, 3, H, W] -> y: [nb, H, W]."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.height, self.width), np.float32)
+        _check(lib().fwi_nn_forward(self._h, x.shape[0], x, y))
+        return y
+
+    def gradient(self, x, target):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_float()
+        _check(lib().fwi_nn_gradient(self._h, x.shape[0], x, t, C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train_step(self, x, target, lr):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_float()
+        _check(lib().fwi_nn_train_step(self._h, x.shape[0], x, t, lr, C.byref(loss)))
+        return loss.value

@@ -10,6 +10,7 @@
 #include "fwi2d.h"
 #include "kernels.h"
 #include "dsp.h"
+#include "status.h"
 
 #include <hip/hip_runtime.h>
 
@@ -23,10 +24,10 @@
 #include <vector>
 
 namespace {
-
 thread_local std::string g_last_error;
+}
 
-int fail(int code, const char *fmt, ...)
+int fwi::fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -36,6 +37,9 @@ int fail(int code, const char *fmt, ...)
     g_last_error = buf;
     return code;
 }
+
+namespace {
+using fwi::fail;
 
 #define HIP_TRY(expr)                                                                   \
     do {                                                                                \

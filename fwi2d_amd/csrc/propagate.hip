@@ -156,6 +156,37 @@ __device__ __forceinline__ float lap8r(const float (&q)[9][VZ], const float (&ex
     return fmaf(kC4, s1, fmaf(-kC3, s2, fmaf(kC2, s3, -kC1 * s4)));
 }
 
+// acc + a*b with the product rounded on its own (no FMA): the one-step and
+// two-step kernels then accumulate images and illumination identically.
+__device__ __forceinline__ float mul_nc(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add_nc(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+// Sponge factors split by axis: dz(i) = b1d of the row's edge distance in
+// the z pads (1 elsewhere), dx(j) likewise; the reference's lookup
+// (modeling.cu:459-506) is then d = dz (x inside), dx (z inside),
+// (dz + dx) - 1 (corners), 1 (interior).
+__device__ __forceinline__ float pad_factor(const float *b1d, int k, int n, int nb, bool &in)
+{
+    k = min(max(k, 0), n - 1);
+    in = (k >= nb) && (k < n - nb);
+    return in ? 1.f : b1d[(k < nb) ? k : (n - 1 - k)];
+}
+__device__ __forceinline__ float sponge(float dz, bool inz, float dx, bool inx)
+{
+    if (inz && inx) return 1.f;
+    if (inx) return dz;
+    if (inz) return dx;
+    return (dz + dx) - 1.f;
+}
+
 // One leapfrog update with explicit FMAs:
 //   no sponge: u_new = (p + (p - o)) + c lap
 //   sponge:    u_new = d ((p + (p - d o)) + c lap)
@@ -273,6 +304,11 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
     const bool rec_on = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
     float *seis = rec_on ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
 
+    float dzr[VZ];
+    bool inz[VZ];
+#pragma unroll
+    for (int r = 0; r < VZ; ++r) dzr[r] = ABC ? pad_factor(a.b1d, zb + r, g.nzz, g.nb, inz[r]) : 1.f;
+
     float q[9][VZ];
 #pragma unroll
     for (int k = 0; k < 9; ++k) ldcol<VZ>(q[k], cur, j0 - 4 + k, nxx, ldz, zb, active);
@@ -282,6 +318,8 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
     vzero<VZ>(pn);
 
     for (int j = j0; j < j1; ++j) {
+        bool inx = true;
+        const float dxj = ABC ? pad_factor(a.b1d, j, nxx, g.nb, inx) : 1.f;
         float o[VZ], c[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o[r] = on[r]; c[r] = cn[r]; }
@@ -296,7 +334,7 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
 #pragma unroll
         for (int r = 0; r < VZ; ++r) {
             const int i = zb + r;
-            const float d = ABC ? damper(a.b1d, i, j, g) : 1.f;
+            const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
             float val = leapfrog<ABC>(q[4][r], o[r], c[r], lap8r<VZ, 0>(q, ext, r), d);
             if (i == siz && j == six) val += wsrc;
             nv[r] = (i >= 4 && i < g.nzz - 4) ? val : o[r];
@@ -351,16 +389,18 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
     const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
     float *seis = (rec1 || rec2) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
 
+    float dzr[VZ];
+    bool inz[VZ];
+#pragma unroll
+    for (int r = 0; r < VZ; ++r) dzr[r] = ABC ? pad_factor(a.b1d, zb + r, g.nzz, g.nb, inz[r]) : 1.f;
+
     float q1[9][VZ];     // u_t, columns j-4 .. j+4
     float q2[9][VZ];     // u_{t+1}, columns j-8 .. j
-    float cq[5][VZ];     // c, columns j-4 .. j
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         ldcol<VZ>(q1[k], cur, x0 - 8 + k, nxx, ldz, zb, active);
         vzero<VZ>(q2[k]);
     }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) vzero<VZ>(cq[k]);
     float on[VZ], cn[VZ], pn[VZ];
     ldcol<VZ>(on, old, x0 - 4, nxx, ldz, zb, active);
     ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb, active);
@@ -379,12 +419,14 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
         float n1[VZ];
         vzero<VZ>(n1);
         if (j >= 4 && j < nxx - 4) {
+            bool inx = true;
+            const float dxj = ABC ? pad_factor(a.b1d, j, nxx, g.nb, inx) : 1.f;
             float ext[VZ + 8];
             zext<VZ>(ext, q1[4], cur + (long)j * ldz, zb, lane, ldz);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
-                const float d = ABC ? damper(a.b1d, i, j, g) : 1.f;
+                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
                 float val = leapfrog<ABC>(q1[4][r], o1[r], cj[r], lap8r<VZ, 0>(q1, ext, r), d);
                 if (i == siz && j == six) val += w1;
                 n1[r] = (i >= 4 && i < g.nzz - 4) ? val : o1[r];
@@ -399,15 +441,15 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
 #pragma unroll
             for (int r = 0; r < VZ; ++r) q2[k][r] = q2[k + 1][r];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) cq[k][r] = cq[k + 1][r];
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) { q2[8][r] = n1[r]; cq[4][r] = cj[r]; }
+        for (int r = 0; r < VZ; ++r) q2[8][r] = n1[r];
 
         // ---- step 2: u_{t+2} at column jj = j - 4 (an owned column)
         const int jj = j - 4;
         if (jj >= x0) {
+            bool inx = true;
+            const float dxj = ABC ? pad_factor(a.b1d, jj, nxx, g.nb, inx) : 1.f;
+            float c2[VZ];                               // c at column jj (read 4 columns ago)
+            ldcol<VZ>(c2, cc, jj, nxx, ldz, zb, active);
             float ext[VZ + 8];
             zext_local<VZ>(ext, q2[4], lane);
             float n2[VZ];
@@ -415,8 +457,8 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float o2 = q1[0][r];              // u_t at column jj
-                const float d = ABC ? damper(a.b1d, i, jj, g) : 1.f;
-                float val = leapfrog<ABC>(q2[4][r], o2, cq[0][r], lap8r<VZ, 0>(q2, ext, r), d);
+                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
+                float val = leapfrog<ABC>(q2[4][r], o2, c2[r], lap8r<VZ, 0>(q2, ext, r), d);
                 if (i == siz && jj == six) val += w2;
                 n2[r] = (i >= 4 && i < g.nzz - 4) ? val : o2;
             }
@@ -511,9 +553,9 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
             ns[r] = inside ? sn : os[r];
             nr[r] = inside ? rn : orr[r];
             if (inside) {
-                il[r] = fmaf(sn, sn, il[r]);
-                if constexpr (MODE == 0) ac[r] = fmaf(sn, pr, ac[r]);
-                else ac[r] = fmaf(pr, c[r] * lapS, ac[r]);
+                il[r] = add_nc(il[r], mul_nc(sn, sn));
+                if constexpr (MODE == 0) ac[r] = add_nc(ac[r], mul_nc(sn, pr));
+                else ac[r] = add_nc(ac[r], mul_nc(pr, mul_nc(c[r], lapS)));
             }
         }
         if (inj_on) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
@@ -532,13 +574,12 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
 }
 
 // Two reverse steps per launch, tiled like fwd2_kernel, for both wavefields.
-// The step-1 accumulations are applied at column j to the loaded image /
-// illumination, and the partial sums wait in a 5-column queue until step 2
-// reaches the column, so the accumulators are read and written once per two
-// steps with the same FMA sequence as two single steps.
+// The step-1 image / illumination contributions (rounded products) wait in a
+// 5-column queue; image and illumination are read and written once per two
+// steps and summed in the same order as two single steps.
 template <int VZ, int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VZ == 4 ? 1 : 3)))
-void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw, int nslots, int k)
+__global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int k)
 {
     TileId id;
     if (!tile_of(nzseg, nxchunk, nslots, id)) return;
@@ -571,8 +612,7 @@ void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw, int nslots, int k)
 
     float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur: columns j-4 .. j+4
     float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R: columns j-8 .. j
-    float cq[5][VZ];                 // c: columns j-4 .. j
-    float iq[5][VZ], aq[5][VZ];      // partial illum / image: columns j-4 .. j
+    float iq[5][VZ], aq[5][VZ];      // step-1 illum / image contributions
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
         ldcol<VZ>(qs1[m], scur, x0 - 8 + m, nxx, ldz, zb, active);
@@ -580,7 +620,7 @@ void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw, int nslots, int k)
         vzero<VZ>(qs2[m]); vzero<VZ>(qr2[m]);
     }
 #pragma unroll
-    for (int m = 0; m < 5; ++m) { vzero<VZ>(cq[m]); vzero<VZ>(iq[m]); vzero<VZ>(aq[m]); }
+    for (int m = 0; m < 5; ++m) { vzero<VZ>(iq[m]); vzero<VZ>(aq[m]); }
     float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ];
     ldcol<VZ>(osn, sold, x0 - 4, nxx, ldz, zb, active);
     ldcol<VZ>(orn, rold, x0 - 4, nxx, ldz, zb, active);
@@ -588,13 +628,10 @@ void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw, int nslots, int k)
     vzero<VZ>(psn); vzero<VZ>(prn);
 
     for (int j = x0 - 4; j < x1 + 4; ++j) {
-        float os[VZ], orr[VZ], cj[VZ], ac1[VZ], il1[VZ];
+        float os[VZ], orr[VZ], cj[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r]; }
-        // accumulators of an owned column (the other lanes / columns carry zeros)
-        const bool acc_here = own && (j >= x0) && (j < x1);
-        ldcol<VZ>(ac1, acc, j, nxx, ldz, zb, acc_here);
-        ldcol<VZ>(il1, ill, j, nxx, ldz, zb, acc_here);
+        const int jj = j - 4;
         if (j + 1 < x1 + 4) {
             ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb, active);
             ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb, active);
@@ -603,8 +640,8 @@ void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw, int nslots, int k)
             ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
         }
         // ---- step 1 at column j
-        float ns[VZ], nr[VZ];
-        vzero<VZ>(ns); vzero<VZ>(nr);
+        float ns[VZ], nr[VZ], ci[VZ], ca[VZ];
+        vzero<VZ>(ns); vzero<VZ>(nr); vzero<VZ>(ci); vzero<VZ>(ca);
         if (j >= 4 && j < nxx - 4) {
             float es[VZ + 8], er[VZ + 8];
             zext<VZ>(es, qs1[4], scur + (long)j * ldz, zb, lane, ldz);
@@ -620,9 +657,8 @@ void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw, int nslots, int k)
                 ns[r] = inside ? sn : os[r];
                 nr[r] = inside ? rn : orr[r];
                 if (inside) {
-                    il1[r] = fmaf(sn, sn, il1[r]);
-                    if constexpr (MODE == 0) ac1[r] = fmaf(sn, pr, ac1[r]);
-                    else ac1[r] = fmaf(pr, cj[r] * lapS, ac1[r]);
+                    ci[r] = mul_nc(sn, sn);
+                    ca[r] = (MODE == 0) ? mul_nc(sn, pr) : mul_nc(pr, mul_nc(cj[r], lapS));
                 }
             }
             if (inj1) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
@@ -638,22 +674,22 @@ void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw, int nslots, int k)
 #pragma unroll
         for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int r = 0; r < VZ; ++r) {
-                cq[m][r] = cq[m + 1][r]; iq[m][r] = iq[m + 1][r]; aq[m][r] = aq[m + 1][r];
-            }
+            for (int r = 0; r < VZ; ++r) { iq[m][r] = iq[m + 1][r]; aq[m][r] = aq[m + 1][r]; }
 #pragma unroll
         for (int r = 0; r < VZ; ++r) {
-            qs2[8][r] = ns[r]; qr2[8][r] = nr[r];
-            cq[4][r] = cj[r]; iq[4][r] = il1[r]; aq[4][r] = ac1[r];
+            qs2[8][r] = ns[r]; qr2[8][r] = nr[r]; iq[4][r] = ci[r]; aq[4][r] = ca[r];
         }
 
-        // ---- step 2 at column jj = j - 4
-        const int jj = j - 4;
+        // ---- step 2 at column jj
         if (jj >= x0) {
+            float ac[VZ], il[VZ], cjj[VZ];
+            ldcol<VZ>(ac, acc, jj, nxx, ldz, zb, own);
+            ldcol<VZ>(il, ill, jj, nxx, ldz, zb, own);
+            ldcol<VZ>(cjj, cc, jj, nxx, ldz, zb, active);   // c at jj (read 4 columns ago)
             float es[VZ + 8], er[VZ + 8];
             zext_local<VZ>(es, qs2[4], lane);
             zext_local<VZ>(er, qr2[4], lane);
-            float ns2[VZ], nr2[VZ], il[VZ], ac[VZ];
+            float ns2[VZ], nr2[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
@@ -661,18 +697,16 @@ void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw, int nslots, int k)
                 const float os2 = qs1[0][r];            // S_cur at column jj
                 const float or2 = qr1[0][r];            // R_k at column jj
                 const float ps = qs2[4][r], pr = qr2[4][r];
-                const float c2 = cq[0][r];
+                const float c2 = cjj[r];
                 const float lapS = lap8r<VZ, 0>(qs2, es, r);
                 const float sn = fmaf(c2, lapS, ps + (ps - os2));
                 const float rn = fmaf(c2, lap8r<VZ, 0>(qr2, er, r), pr + (pr - or2));
                 ns2[r] = inside ? sn : os2;
                 nr2[r] = inside ? rn : or2;
-                il[r] = iq[0][r];
-                ac[r] = aq[0][r];
                 if (inside) {
-                    il[r] = fmaf(sn, sn, il[r]);
-                    if constexpr (MODE == 0) ac[r] = fmaf(sn, pr, ac[r]);
-                    else ac[r] = fmaf(pr, c2 * lapS, ac[r]);
+                    il[r] = add_nc(add_nc(il[r], iq[0][r]), mul_nc(sn, sn));
+                    const float c2a = (MODE == 0) ? mul_nc(sn, pr) : mul_nc(pr, mul_nc(c2, lapS));
+                    ac[r] = add_nc(add_nc(ac[r], aq[0][r]), c2a);
                 }
             }
             if (inj2) inject_receivers<VZ>(a, nr2, jj, zb, dat, a.nt - 3 - k);

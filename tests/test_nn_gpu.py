@@ -1,0 +1,84 @@
+"""GPU tests of the learned-preconditioner network through the C ABI:
+the MFMA lane layout, forward and backward against the FP64 oracle (with the
+GPU's bf16 rounding points emulated), the Adam update in closed form, and a
+training run that reduces the loss."""
+import numpy as np
+import pytest
+
+import fwi2d_amd as F
+import nn_oracle as no
+from cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_lane_layout():
+    rng = np.random.default_rng(0)
+    a = no.bf16(rng.standard_normal((16, 32))).astype(np.float32)
+    b = no.bf16(rng.standard_normal((32, 16))).astype(np.float32)
+    d = F.mfma_probe(a, b)
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    assert np.abs(d - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def small_problem(seed=0, nb=2, h=16, w=24):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((nb, 3, h, w)).astype(np.float32)
+    t = (0.1 * rng.standard_normal((nb, h, w))).astype(np.float32)
+    return x, t
+
+
+def test_forward_matches_oracle():
+    x, _ = small_problem()
+    net = F.Net(2, 16, 24, seed=7)
+    p = net.get_params()
+    y = net.forward(x)
+    yo, _ = no.forward(p.astype(np.float64), x.astype(np.float64), rnd=True)
+    err = rel_l2(y, yo)
+    print(f"forward relL2 (bf16 emulated) {err:.3e}")
+    assert err < 1e-2
+    yo64, _ = no.forward(p.astype(np.float64), x.astype(np.float64), rnd=False)
+    print(f"forward relL2 (exact FP64) {rel_l2(y, yo64):.3e}")
+    assert rel_l2(y, yo64) < 5e-2
+
+
+def test_gradient_matches_oracle():
+    x, t = small_problem(1)
+    net = F.Net(2, 16, 24, seed=3)
+    p = net.get_params()
+    loss, g = net.gradient(x, t)
+    lo, go, _ = no.loss_and_grad(p.astype(np.float64), x.astype(np.float64),
+                                 t.astype(np.float64), rnd=True)
+    assert abs(loss - lo) <= 1e-2 * lo
+    # per layer relative error of the weight and bias gradients
+    o = 0
+    for k, (cin, cout) in enumerate(no.ARCH):
+        n = cout * 9 * cin + cout
+        e = rel_l2(g[o:o + n], go[o:o + n])
+        print(f"layer {k + 1} gradient relL2 {e:.3e}")
+        assert e < 3e-2, k
+        o += n
+
+
+def test_adam_step_closed_form():
+    x, t = small_problem(2)
+    net = F.Net(2, 16, 24, seed=5)
+    p0 = net.get_params()
+    _, g = net.gradient(x, t)
+    lr = 1e-3
+    net.train_step(x, t, lr)
+    p1 = net.get_params()
+    # first Adam step from zero moments: w -= lr * g / (|g| + eps) (bias-corrected)
+    expect = p0 - lr * g / (np.abs(g) + 1e-8)
+    assert np.abs(p1 - expect).max() <= 1e-6 + 1e-4 * lr
+
+
+def test_set_get_roundtrip_and_training_reduces_loss():
+    x, t = small_problem(3, nb=4)
+    net = F.Net(4, 16, 24, seed=11)
+    p = net.get_params()
+    net.set_params(p)
+    assert np.array_equal(net.get_params(), p)
+    losses = [net.train_step(x, t, 3e-3) for _ in range(60)]
+    print("loss", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.5 * losses[0]
