@@ -41,16 +41,25 @@ inline int blocks_for(long n, int bs, int cap = 65536)
 
 // ---------------------------------------------------------------- conv fwd
 
-// Block: 4 waves; the block's weights [coutp][kp] sit in LDS; each wave
-// computes 32 output pixels x coutp channels per tile (2 x coutp/16 MFMA
-// accumulators) and walks the K dimension (tap, ci) in chunks of 32.
+// Block: 4 waves; the block's weights [coutp][kp] sit in LDS when they fit
+// in 64 KiB (otherwise the B operands are read from global memory, where
+// they stay L2-resident); each wave computes 32 output pixels x coutp
+// channels per tile (2 x coutp/16 MFMA accumulators) and walks the K
+// dimension (tap, ci) in chunks of 32.
+constexpr int kLdsWeightBytes = 65536;
+
+template <bool LDS>
 __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvFwd a)
 {
     extern __shared__ bf16 sw[];
-    const int nw = a.coutp * a.kp;
-    for (int i = threadIdx.x * 8; i < nw; i += 256 * 8)
-        *reinterpret_cast<bf16x8 *>(sw + i) = ld8(a.wgt + i);
-    __syncthreads();
+    const bf16 *wsrc = a.wgt;
+    if constexpr (LDS) {
+        const int nw = a.coutp * a.kp;
+        for (int i = threadIdx.x * 8; i < nw; i += 256 * 8)
+            *reinterpret_cast<bf16x8 *>(sw + i) = ld8(a.wgt + i);
+        __syncthreads();
+        wsrc = sw;
+    }
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r16 = lane & 15;
@@ -93,7 +102,7 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvFwd a)
 #pragma unroll
             for (int n = 0; n < 4; ++n) {
                 if (n < nblk) {
-                    const bf16x8 B = *reinterpret_cast<const bf16x8 *>(sw + (n * 16 + r16) * a.kp + k0);
+                    const bf16x8 B = *reinterpret_cast<const bf16x8 *>(wsrc + (n * 16 + r16) * a.kp + k0);
                     acc[0][n] = mfma(A[0], B, acc[0][n]);
                     acc[1][n] = mfma(A[1], B, acc[1][n]);
                 }
@@ -418,7 +427,10 @@ hipError_t conv_fwd(const ConvFwd &a, hipStream_t s)
     const int ntile = (int)((npix + 31) / 32);
     const int nblocks = blocks_for(ntile, 4, 2048);
     const size_t lds = (size_t)a.coutp * a.kp * sizeof(bf16);
-    hipLaunchKernelGGL(conv_fwd_kernel, dim3(nblocks), dim3(256), lds, s, a);
+    if (lds <= (size_t)kLdsWeightBytes)
+        hipLaunchKernelGGL(conv_fwd_kernel<true>, dim3(nblocks), dim3(256), lds, s, a);
+    else
+        hipLaunchKernelGGL(conv_fwd_kernel<false>, dim3(nblocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
