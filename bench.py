@@ -13,7 +13,10 @@ Per step: random-boundary generation, one forward pass and one reverse pass
 (source reversal + receiver back-propagation + imaging) over all 64 shots.
 Cell-updates = shots x nzz x nxx x T per pass, 2 per cell in the reverse pass.
 
-Prints one JSON line (the metric is Gcell-updates/s).
+Prints one JSON line (the metric is Gcell-updates/s).  roofline.traffic is
+the HBM bytes per launch of the reverse kernel from two rocprofv3 --pmc passes
+(FETCH_SIZE, then WRITE_SIZE) over a short run of the same kernels in a child
+process, each scaled by what the counter reads for a copy of known size.
 """
 import argparse
 import json
@@ -29,6 +32,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_FWD = 16.0      # read u_t, u_{t-1}, v; write u_{t+1}  (FP32)
 BYTES_REV = 22.0      # per cell-update of the reverse pass (44 B per cell-step)
+PMC_NT = 300                  # time samples of the counter-collection run
+PMC_COPY_BYTES = 256 << 20    # calibration copy (reads and writes this many bytes)
+PMC_TIMEOUT = 240             # seconds per counter pass
 
 
 def parse():
@@ -46,6 +52,10 @@ def parse():
     p.add_argument("--fwd-tuning", default="", help="steps,vz,xw of the forward kernel")
     p.add_argument("--rev-tuning", default="", help="steps,vz,xw of the reverse kernel")
     p.add_argument("--graph", type=int, default=-1)
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the rocprofv3 counter passes (roofline.traffic = null)")
+    p.add_argument("--pmc-child", action="store_true",
+                   help="internal: the short run profiled by the counter passes")
     return p.parse_args()
 
 
@@ -87,6 +97,117 @@ def cpu_baseline(s, vmig, xs, zs, xr, zr, w, tlag, obs, threads, nshots):
                       f"{el:.1f} s of CPU time"}
 
 
+def kernel_key(kind, steps, vz):
+    """Name fragment of the random-boundary forward (kind 0) or imaging
+    reverse (kind 1) kernel instance."""
+    return f"fwd{steps}_kernel<{vz}, false>" if kind == 0 else f"rev{steps}_kernel<{vz}, 0>"
+
+
+def counter_pass(prof, counter, child_args, workdir):
+    """One rocprofv3 --pmc pass over the child run; returns {kernel: [KiB per
+    dispatch]} or raises RuntimeError."""
+    import csv
+    import glob
+    import signal
+    import subprocess
+    out = os.path.join(workdir, counter.lower())
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+           sys.executable, os.path.abspath(__file__), "--pmc-child"] + child_args
+    log = open(os.path.join(workdir, counter.lower() + ".log"), "w")
+    p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=log, start_new_session=True,
+                         text=True)
+    try:
+        so, _ = p.communicate(timeout=PMC_TIMEOUT)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.communicate()
+        raise RuntimeError(f"{counter} pass timed out")
+    finally:
+        log.close()
+    # rocprofv3 may exit non-zero after a complete run; the child's own line
+    # and the counter file decide
+    if "pmc-child done" not in so:
+        raise RuntimeError(f"{counter} pass: child did not finish (rc {p.returncode})")
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"{counter} pass: no counter file")
+    vals = {}
+    with open(files[0], newline="") as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") != counter:
+                continue
+            vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return vals
+
+
+def measure_traffic(nshots, tuning):
+    """HBM bytes per launch of the forward and reverse kernels (FETCH_SIZE and
+    WRITE_SIZE, each times the copy-calibrated factor).  Returns (dict, note)."""
+    import shutil
+    import tempfile
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3"
+                                         if os.path.exists("/opt/rocm/bin/rocprofv3") else None)
+    if prof is None:
+        return None, "rocprofv3 not found"
+    (stf, vzf, xwf), (str_, vzr, xwr) = tuning
+    child = ["--nt", str(PMC_NT), "--shots", str(nshots), "--graph", "0",
+             "--fwd-tuning", f"{stf},{vzf},{xwf}", "--rev-tuning", f"{str_},{vzr},{xwr}"]
+    want = {"forward": kernel_key(0, stf, vzf), "reverse": kernel_key(1, str_, vzr)}
+    res = {"forward": {}, "reverse": {}, "calibration": {}}
+    with tempfile.TemporaryDirectory(prefix="fwi_pmc_") as wd:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            vals = counter_pass(prof, counter, child, wd)
+            copy = [v for k, vs in vals.items() if "copy_kernel" in k for v in vs]
+            if not copy:
+                raise RuntimeError(f"{counter}: copy kernel not seen")
+            factor = (PMC_COPY_BYTES / 1024.0) / float(np.median(copy))
+            res["calibration"][counter] = round(factor, 4)
+            for kind, pre in want.items():
+                vs = [v for k, v_ in vals.items() if pre in k for v in v_]
+                if not vs:
+                    raise RuntimeError(f"{counter}: {pre} not seen")
+                res[kind][counter] = float(np.median(vs)) * factor * 1024.0
+    out = {k: res[k]["FETCH_SIZE"] + res[k]["WRITE_SIZE"] for k in ("forward", "reverse")}
+    note = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, median KiB per dispatch over a "
+            f"{PMC_NT}-sample run of the same kernels and launch shapes, scaled by the "
+            f"counter's reading for a {PMC_COPY_BYTES >> 20} MiB copy (factors "
+            f"{res['calibration']})")
+    return {"bytes": out, "read": {k: res[k]["FETCH_SIZE"] for k in out},
+            "write": {k: res[k]["WRITE_SIZE"] for k in out}}, note
+
+
+def pmc_child(a):
+    """The run profiled by the counter passes: a calibration copy of known
+    size, then one short migration with the given launch shapes."""
+    import fwi2d_amd as F
+    from fwi2d_amd import configs as C
+    import dataclasses
+    s = dataclasses.replace(C.CONFIG2, nt=a.nt)
+    true_v, mig_v = C.config2_models()
+    xs, zs, xr, zr = s.acquisition()
+    if a.shots != s.nsrc:
+        sel = np.linspace(0, s.nsrc - 1, a.shots).round().astype(int)
+        xs, zs = xs[sel], zs[sel]
+    w, tlag = F.ricker_wavelet(s.nt, s.dt, s.fmax)
+    F.abi.stream_copy(PMC_COPY_BYTES, reps=3)
+    eng = F.Engine(s.nz, s.nx, s.dh, s.nt, s.dt, abc_nb=s.abc_nb, abc_factor=s.abc_factor,
+                   rbc_nb=s.nb, rbc_varVp=s.varVp, rbc_varVp_rel=s.varVp_rel, rbc_seed=s.seed,
+                   max_batch=a.shots)
+    eng.set_wavelet(w, tlag)
+    eng.set_geometry(xs, zs, xr, zr)
+    eng.set_model(true_v)
+    obs = eng.modeling(F.ABC)
+    eng.set_observed(obs)
+    eng.set_model(mig_v)
+    for kernel, spec in ((0, a.fwd_tuning), (1, a.rev_tuning)):
+        st, vz, xw = (int(x) for x in spec.split(","))
+        eng.set_tuning(vz, xw, 0, kernel=kernel, steps=st)
+    eng.migration()
+    eng.synchronize()
+    eng.close()
+    print("pmc-child done", flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -94,6 +215,8 @@ def main():
               f"would shard across GPUs, but this build measures one GPU's share.",
               file=sys.stderr)
         sys.exit(2)
+    if a.pmc_child:
+        return pmc_child(a)
     import fwi2d_amd as F
     from fwi2d_amd import configs as C
 
@@ -190,6 +313,19 @@ def main():
         "roofline_forward": fwd_roof,
     }
     eng.close()
+    if not a.no_traffic:
+        try:
+            tr, note = measure_traffic(nshots, ((stf, vzf, xwf), (str_, vzr, xwr)))
+        except RuntimeError as e:
+            tr, note = None, f"counter passes failed: {e}"
+        if tr is not None:
+            for key, kind in (("roofline", "reverse"), ("roofline_forward", "forward")):
+                out[key]["traffic"] = round(tr["bytes"][kind], 0)
+                out[key]["traffic_read"] = round(tr["read"][kind], 0)
+                out[key]["traffic_write"] = round(tr["write"][kind], 0)
+                out[key]["traffic_over_algorithmic"] = round(
+                    tr["bytes"][kind] / out[key]["algorithmic_bytes_per_launch"], 4)
+        out["roofline"]["traffic_method"] = note
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(s, mig_v, xs, zs, xr, zr, w, tlag, obs, threads,

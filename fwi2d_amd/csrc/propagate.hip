@@ -261,6 +261,118 @@ __device__ __forceinline__ void inject_receivers(const RevStep &a, float (&v)[VZ
     }
 }
 
+// Per-wave LDS copy of the receiver table (and, for the sponge, of the x
+// factors) over the wave's column window [xa, xa + nc).  It is filled once
+// at kernel start so that the column march never waits on a global load whose
+// result it needs at once: such a wait also drains the loads prefetched for
+// the next column.  Windows with more than kStageRec receivers use the
+// global-memory path instead.
+constexpr int kStageCols = 160;
+constexpr int kStageRec = 256;
+
+struct FwdStage {
+    int cp[kStageCols + 1];     // col_ptr[xa + c] - col_ptr[xa]
+    int iz[kStageRec];
+    int id[kStageRec];
+    float dx[kStageCols];       // sponge x factor of column xa + c
+};
+struct RevStage {
+    int cp[kStageCols + 1];
+    int iz[kStageRec];
+    float v1[kStageRec];        // data row of the first step (rows in the wave)
+    float v2[kStageRec];        // data row of the second step
+};
+
+// Column offsets and rows of the window's receivers; returns false (for the
+// whole wave) when the window does not fit.
+template <class Stage>
+__device__ __forceinline__ bool stage_table(const RecTable &rt, int xa, int nc, int nxx,
+                                            Stage &st, int lane, int &e0, int &ne)
+{
+    const int ca = min(max(xa, 0), nxx), cb = min(max(xa + nc, 0), nxx);
+    e0 = rt.col_ptr[ca];
+    ne = rt.col_ptr[cb] - e0;
+    if (nc > kStageCols || ne > kStageRec) return false;
+    for (int c = lane; c <= nc; c += 64) st.cp[c] = rt.col_ptr[min(max(xa + c, 0), nxx)] - e0;
+    for (int e = lane; e < ne; e += 64) st.iz[e] = rt.iz[e0 + e];
+    return true;
+}
+
+__device__ __forceinline__ bool stage_fwd(const FwdStep &a, bool rec, bool abc, int xa, int nc,
+                                          FwdStage &st, int lane)
+{
+    bool ok = nc <= kStageCols;
+    if (abc && ok)
+        for (int c = lane; c < nc; c += 64) {
+            bool in;
+            st.dx[c] = pad_factor(a.b1d, xa + c, a.g.nxx, a.g.nb, in);
+        }
+    if (rec && ok) {
+        int e0, ne;
+        ok = stage_table(a.rec, xa, nc, a.g.nxx, st, lane, e0, ne);
+        if (ok) {
+            for (int e = lane; e < ne; e += 64) st.id[e] = a.rec.id[e0 + e];
+        }
+    }
+    __syncthreads();
+    return ok;
+}
+
+// Rows [zlo, zhi) are the wave's; data of other rows is never read.
+__device__ __forceinline__ bool stage_rev(const RevStep &a, const float *dat, int xa, int nc,
+                                          int zlo, int zhi, bool on1, int trow1, bool on2,
+                                          int trow2, RevStage &st, int lane)
+{
+    int e0 = 0, ne = 0;
+    const bool ok = (on1 || on2) && stage_table(a.rec, xa, nc, a.g.nxx, st, lane, e0, ne);
+    if (ok) {
+        for (int e = lane; e < ne; e += 64) {
+            const int iz = st.iz[e];        // written by this lane
+            float v1 = 0.f, v2 = 0.f;
+            if (iz >= zlo && iz < zhi) {
+                const float *row = dat + (long)a.rec.id[e0 + e] * a.nt;
+                if (on1) v1 = row[trow1];
+                if (on2) v2 = row[trow2];
+            }
+            st.v1[e] = v1;
+            st.v2[e] = v2;
+        }
+    }
+    __syncthreads();
+    return ok;
+}
+
+// seis[rec][tsamp] = d * v for the receivers of window column c in this lane.
+template <int VZ>
+__device__ __forceinline__ void sample_staged(const FwdStage &st, int nt, const float (&v)[VZ],
+                                              const float (&d)[VZ], int c, int zb, float *seis,
+                                              int tsamp)
+{
+    const int m0 = st.cp[c], m1 = st.cp[c + 1];
+    for (int m = m0; m < m1; ++m) {
+        const int rr = st.iz[m] - zb;
+#pragma unroll
+        for (int r = 0; r < VZ; ++r)
+            if (rr == r) seis[(long)st.id[m] * nt + tsamp] = v[r] * d[r];
+    }
+}
+
+// Adds idh2 * data to the receivers of window column c in this lane, in
+// table order.
+template <int VZ, bool SECOND>
+__device__ __forceinline__ void inject_staged(const RevStage &st, float idh2, float (&v)[VZ], int c,
+                                              int zb)
+{
+    const int m0 = st.cp[c], m1 = st.cp[c + 1];
+    for (int m = m0; m < m1; ++m) {
+        const int rr = st.iz[m] - zb;
+        const float val = SECOND ? st.v2[m] : st.v1[m];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r)
+            if (rr == r) v[r] = fmaf(idh2, val, v[r]);
+    }
+}
+
 // Decodes the wave's tile after the XCD renumbering.
 struct TileId {
     int zseg, xc, slot;
@@ -303,6 +415,8 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
     const float wsrc = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
     const bool rec_on = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
     float *seis = rec_on ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
+    __shared__ FwdStage st;
+    const bool staged = stage_fwd(a, rec_on, ABC, j0, j1 - j0, st, lane);
 
     float dzr[VZ];
     bool inz[VZ];
@@ -318,8 +432,12 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
     vzero<VZ>(pn);
 
     for (int j = j0; j < j1; ++j) {
-        bool inx = true;
-        const float dxj = ABC ? pad_factor(a.b1d, j, nxx, g.nb, inx) : 1.f;
+        const bool inx = (j >= g.nb) && (j < nxx - g.nb);
+        float dxj = 1.f;
+        if constexpr (ABC) {
+            bool in;
+            dxj = staged ? st.dx[j - j0] : pad_factor(a.b1d, j, nxx, g.nb, in);
+        }
         float o[VZ], c[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o[r] = on[r]; c[r] = cn[r]; }
@@ -330,18 +448,22 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
         }
         float ext[VZ + 8];
         zext<VZ>(ext, q[4], cur + (long)j * ldz, zb, lane, ldz);
-        float nv[VZ];
+        float nv[VZ], dv[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) {
             const int i = zb + r;
             const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
+            dv[r] = d;
             float val = leapfrog<ABC>(q[4][r], o[r], c[r], lap8r<VZ, 0>(q, ext, r), d);
             if (i == siz && j == six) val += wsrc;
             nv[r] = (i >= 4 && i < g.nzz - 4) ? val : o[r];
         }
         if (active) {
             vstore<VZ>(out + (long)j * ldz + zb, nv);
-            if (rec_on) sample_receivers<VZ, ABC>(a, nv, j, zb, seis, t + 1 - a.tlag);
+            if (rec_on) {
+                if (staged) sample_staged<VZ>(st, a.nt, nv, dv, j - j0, zb, seis, t + 1 - a.tlag);
+                else sample_receivers<VZ, ABC>(a, nv, j, zb, seis, t + 1 - a.tlag);
+            }
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k)
@@ -388,6 +510,9 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
     const bool rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
     const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
     float *seis = (rec1 || rec2) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
+    const int xa = x0 - 4;                           // window [x0 - 4, x1 + 4)
+    __shared__ FwdStage st;
+    const bool staged = stage_fwd(a, rec1 || rec2, ABC, xa, x1 + 4 - xa, st, lane);
 
     float dzr[VZ];
     bool inz[VZ];
@@ -419,21 +544,29 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
         float n1[VZ];
         vzero<VZ>(n1);
         if (j >= 4 && j < nxx - 4) {
-            bool inx = true;
-            const float dxj = ABC ? pad_factor(a.b1d, j, nxx, g.nb, inx) : 1.f;
-            float ext[VZ + 8];
+            const bool inx = (j >= g.nb) && (j < nxx - g.nb);
+            float dxj = 1.f;
+            if constexpr (ABC) {
+                bool in;
+                dxj = staged ? st.dx[j - xa] : pad_factor(a.b1d, j, nxx, g.nb, in);
+            }
+            float ext[VZ + 8], dv[VZ];
             zext<VZ>(ext, q1[4], cur + (long)j * ldz, zb, lane, ldz);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
+                dv[r] = d;
                 float val = leapfrog<ABC>(q1[4][r], o1[r], cj[r], lap8r<VZ, 0>(q1, ext, r), d);
                 if (i == siz && j == six) val += w1;
                 n1[r] = (i >= 4 && i < g.nzz - 4) ? val : o1[r];
             }
             if (own && j >= x0 && j < x1) {
                 vstore<VZ>(out1 + (long)j * ldz + zb, n1);
-                if (rec1) sample_receivers<VZ, ABC>(a, n1, j, zb, seis, t + 1 - a.tlag);
+                if (rec1) {
+                    if (staged) sample_staged<VZ>(st, a.nt, n1, dv, j - xa, zb, seis, t + 1 - a.tlag);
+                    else sample_receivers<VZ, ABC>(a, n1, j, zb, seis, t + 1 - a.tlag);
+                }
             }
         }
 #pragma unroll
@@ -446,11 +579,15 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
         // ---- step 2: u_{t+2} at column jj = j - 4 (an owned column)
         const int jj = j - 4;
         if (jj >= x0) {
-            bool inx = true;
-            const float dxj = ABC ? pad_factor(a.b1d, jj, nxx, g.nb, inx) : 1.f;
+            const bool inx = (jj >= g.nb) && (jj < nxx - g.nb);
+            float dxj = 1.f;
+            if constexpr (ABC) {
+                bool in;
+                dxj = staged ? st.dx[jj - xa] : pad_factor(a.b1d, jj, nxx, g.nb, in);
+            }
             float c2[VZ];                               // c at column jj (read 4 columns ago)
             ldcol<VZ>(c2, cc, jj, nxx, ldz, zb, active);
-            float ext[VZ + 8];
+            float ext[VZ + 8], dv[VZ];
             zext_local<VZ>(ext, q2[4], lane);
             float n2[VZ];
 #pragma unroll
@@ -458,13 +595,17 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
                 const int i = zb + r;
                 const float o2 = q1[0][r];              // u_t at column jj
                 const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
+                dv[r] = d;
                 float val = leapfrog<ABC>(q2[4][r], o2, c2[r], lap8r<VZ, 0>(q2, ext, r), d);
                 if (i == siz && jj == six) val += w2;
                 n2[r] = (i >= 4 && i < g.nzz - 4) ? val : o2;
             }
             if (own) {
                 vstore<VZ>(out2 + (long)jj * ldz + zb, n2);
-                if (rec2) sample_receivers<VZ, ABC>(a, n2, jj, zb, seis, t + 2 - a.tlag);
+                if (rec2) {
+                    if (staged) sample_staged<VZ>(st, a.nt, n2, dv, jj - xa, zb, seis, t + 2 - a.tlag);
+                    else sample_receivers<VZ, ABC>(a, n2, jj, zb, seis, t + 2 - a.tlag);
+                }
             }
         }
 #pragma unroll
@@ -508,6 +649,10 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
     const bool inj_on = (k + 1 < a.nt);
     const int drow = a.data_by_shot ? (*a.shot0 + id.slot) : id.slot;
     const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
+    const int zw = id.zseg * 64 * VZ;
+    __shared__ RevStage st;
+    const bool staged = stage_rev(a, dat, j0, j1 - j0, zw, zw + 64 * VZ, inj_on, a.nt - 2 - k,
+                                  false, 0, st, lane);
 
     float qs[9][VZ], qr[9][VZ];
 #pragma unroll
@@ -558,7 +703,10 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
                 else ac[r] = add_nc(ac[r], mul_nc(pr, mul_nc(c[r], lapS)));
             }
         }
-        if (inj_on) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
+        if (inj_on) {
+            if (staged) inject_staged<VZ, false>(st, a.idh2, nr, j - j0, zb);
+            else inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
+        }
         if (active) {
             const long off = (long)j * ldz + zb;
             vstore<VZ>(sout + off, ns); vstore<VZ>(rout + off, nr);
@@ -609,6 +757,10 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
     const bool inj2 = (k + 2 < a.nt);
     const int drow = a.data_by_shot ? (*a.shot0 + id.slot) : id.slot;
     const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
+    const int xa = x0 - 4;                           // window [x0 - 4, x1 + 4)
+    __shared__ RevStage st;
+    const bool staged = stage_rev(a, dat, xa, x1 + 4 - xa, zs, zs + 64 * VZ, inj1, a.nt - 2 - k,
+                                  inj2, a.nt - 3 - k, st, lane);
 
     float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur: columns j-4 .. j+4
     float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R: columns j-8 .. j
@@ -661,7 +813,10 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
                     ca[r] = (MODE == 0) ? mul_nc(sn, pr) : mul_nc(pr, mul_nc(cj[r], lapS));
                 }
             }
-            if (inj1) inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
+            if (inj1) {
+                if (staged) inject_staged<VZ, false>(st, a.idh2, nr, j - xa, zb);
+                else inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
+            }
             if (own && j >= x0 && j < x1) {
                 vstore<VZ>(sout1 + (long)j * ldz + zb, ns);
                 vstore<VZ>(rout1 + (long)j * ldz + zb, nr);
@@ -709,7 +864,10 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
                     ac[r] = add_nc(add_nc(ac[r], aq[0][r]), c2a);
                 }
             }
-            if (inj2) inject_receivers<VZ>(a, nr2, jj, zb, dat, a.nt - 3 - k);
+            if (inj2) {
+                if (staged) inject_staged<VZ, true>(st, a.idh2, nr2, jj - xa, zb);
+                else inject_receivers<VZ>(a, nr2, jj, zb, dat, a.nt - 3 - k);
+            }
             if (own) {
                 const long off = (long)jj * ldz + zb;
                 vstore<VZ>(sout2 + off, ns2);
