@@ -56,6 +56,7 @@ using fwi::fail;
     } while (0)
 
 constexpr float kCflLimit = 0.5546f;   // 2-D leapfrog limit of the 8th-order stencil
+constexpr long kGuard = 1024;           // zeroed floats after each step-kernel buffer
 
 struct GridBuf {
     fwi::Grid g{};
@@ -137,7 +138,7 @@ fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g, int kind)
     fwi::Tiling t;
     t.vz = c->vz[kind];
     t.xw = c->xw[kind];
-    t.nzseg = fwi::zsegments(g.ldz, t.vz, c->nsteps[kind]);
+    t.nzseg = fwi::zsegments(g.nzz, t.vz, c->nsteps[kind]);
     t.nxchunk = (g.nxx - 8 + t.xw - 1) / t.xw;
     return t;
 }
@@ -474,7 +475,11 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     if (r == FWI_OK) r = dalloc(&c->d_tmp, n_int, c->stream);
     if (r == FWI_OK) r = dalloc(&c->d_shot0, 1, c->stream);
     if (r == FWI_OK) r = dalloc(&c->d_wav, cfg->nt, c->stream);
-    for (int k = 0; k < 8 && r == FWI_OK; ++k) r = dalloc(&c->d_f[k], S * c->fstride_max, c->stream);
+    // every buffer a step kernel reads carries kGuard zeroed floats after its
+    // last slot: the lanes of a wave past the last row of the last column
+    // load there (the loads are not masked per lane)
+    for (int k = 0; k < 8 && r == FWI_OK; ++k)
+        r = dalloc(&c->d_f[k], S * c->fstride_max + kGuard, c->stream);
     for (int kind = 0; kind < 2 && r == FWI_OK; ++kind)
         if (c->grid[kind].on) r = dalloc(&c->grid[kind].vpad, c->grid[kind].g.fstride, c->stream);
     if (r == FWI_OK && c->grid[0].on) {
@@ -484,13 +489,13 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
         if (r == FWI_OK)
             r = hipMemcpy(c->d_b1d, b1d.data(), b1d.size() * 4, hipMemcpyHostToDevice) == hipSuccess
                     ? FWI_OK : fail(FWI_ERR_HIP, "b1d upload");
-        if (r == FWI_OK) r = dalloc(&c->d_cabc, c->grid[0].g.fstride, c->stream);
+        if (r == FWI_OK) r = dalloc(&c->d_cabc, c->grid[0].g.fstride + kGuard, c->stream);
     }
     if (r == FWI_OK && c->grid[1].on) {
         const long fs = c->grid[1].g.fstride;
-        r = dalloc(&c->d_crb, S * fs, c->stream);
-        if (r == FWI_OK) r = dalloc(&c->d_acc_slot, S * fs, c->stream);
-        if (r == FWI_OK) r = dalloc(&c->d_ill_slot, S * fs, c->stream);
+        r = dalloc(&c->d_crb, S * fs + kGuard, c->stream);
+        if (r == FWI_OK) r = dalloc(&c->d_acc_slot, S * fs + kGuard, c->stream);
+        if (r == FWI_OK) r = dalloc(&c->d_ill_slot, S * fs + kGuard, c->stream);
         if (r == FWI_OK) r = dalloc(&c->d_image, fs, c->stream);
         if (r == FWI_OK) r = dalloc(&c->d_grad, fs, c->stream);
         if (r == FWI_OK) r = dalloc(&c->d_illum, fs, c->stream);

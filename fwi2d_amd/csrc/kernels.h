@@ -83,7 +83,7 @@ hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t
 hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
                            hipStream_t s);
 // z segments of a launch shape (two-step launches write 64*vz - 8 rows per wave)
-int zsegments(int ldz, int vz, int steps);
+int zsegments(int nzz, int vz, int steps);
 
 // Adds idh2*w[0] at each slot's source cell of `field` (before step 0).
 hipError_t launch_fwd_init(const FwdStep &a, float *field, int nslots, hipStream_t s);

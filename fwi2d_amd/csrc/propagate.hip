@@ -74,39 +74,68 @@ __device__ __forceinline__ void vzero(float (&r)[VZ])
     for (int k = 0; k < VZ; ++k) r[k] = 0.f;
 }
 
-// ext[0 .. VZ+7] = rows zb-4 .. zb+VZ+3 of one column: the lane's own VZ
-// rows, 4 above and 4 below taken from neighbouring lanes by shuffles; lanes
-// whose neighbour is outside the wave read the row from memory (0 outside
-// the array).
+// Whole-wave lane shifts (GFX9 DPP wave_shr:1 / wave_shl:1): lane i takes
+// lane i-1's (shr) or lane i+1's (shl) value; the lane with no source (0 or
+// 63) takes `edge`.  Plain VALU moves, no LDS traffic.
+__device__ __forceinline__ float wave_shr1(float src, float edge)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(src),
+                                                      0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float wave_shl1(float src, float edge)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(src),
+                                                      0x130, 0xF, 0xF, false));
+}
+
+// The 8 rows around a wave's 64*VZ rows [z0, z0 + 64*VZ) of one column:
+// lanes 0-31 get rows z0-4 .. z0-1, lanes 32-63 rows z0+64*VZ .. +3 (one
+// aligned float4 per lane; z0 is a multiple of 4).  Zero outside [lo, hi).
 template <int VZ>
-__device__ __forceinline__ void zext(float (&ext)[VZ + 8], const float (&own)[VZ],
-                                     const float *col, int zb, int lane, int ldz)
+__device__ __forceinline__ float4 ldhalo(const float *base, int col, int lo, int hi, int ldz,
+                                         int z0, int lane)
+{
+    if (col >= lo && col < hi)
+        return *reinterpret_cast<const float4 *>(base + (long)col * ldz +
+                                                 (lane < 32 ? z0 - 4 : z0 + 64 * VZ));
+    return make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ext[0 .. VZ+7] = rows zb-4 .. zb+VZ+3 of one column: the lane's own VZ
+// rows and 4 rows above and below, shifted in from the neighbouring lanes;
+// lane 0 and lane 63 take the rows beyond the wave from the halo vector h.
+template <int VZ>
+__device__ __forceinline__ void zext(float (&ext)[VZ + 8], const float (&own)[VZ], float4 h)
 {
 #pragma unroll
     for (int r = 0; r < VZ; ++r) ext[4 + r] = own[r];
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        const int k = kk - 4;                       // row offset -4 .. -1
-        const int o = -((-k + VZ - 1) / VZ);        // source lane offset (< 0)
-        const int e = k - o * VZ;                   // element of that lane
-        float v = __shfl(own[e], lane + o, 64);
-        if (lane + o < 0) {
-            const int row = zb + k;
-            v = (row >= 0) ? col[row] : 0.f;
-        }
-        ext[kk] = v;
-    }
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        const int k = VZ + kk;                      // row offset VZ .. VZ+3
-        const int o = k / VZ;                       // source lane offset (> 0)
-        const int e = k - o * VZ;
-        float v = __shfl(own[e], lane + o, 64);
-        if (lane + o > 63) {
-            const int row = zb + k;
-            v = (row < ldz) ? col[row] : 0.f;
-        }
-        ext[VZ + 4 + kk] = v;
+    if constexpr (VZ == 4) {
+        ext[0] = wave_shr1(own[0], h.x);
+        ext[1] = wave_shr1(own[1], h.y);
+        ext[2] = wave_shr1(own[2], h.z);
+        ext[3] = wave_shr1(own[3], h.w);
+        ext[8] = wave_shl1(own[0], h.x);
+        ext[9] = wave_shl1(own[1], h.y);
+        ext[10] = wave_shl1(own[2], h.z);
+        ext[11] = wave_shl1(own[3], h.w);
+    } else if constexpr (VZ == 2) {
+        ext[2] = wave_shr1(own[0], h.z);
+        ext[3] = wave_shr1(own[1], h.w);
+        ext[0] = wave_shr1(ext[2], h.x);
+        ext[1] = wave_shr1(ext[3], h.y);
+        ext[6] = wave_shl1(own[0], h.x);
+        ext[7] = wave_shl1(own[1], h.y);
+        ext[8] = wave_shl1(ext[6], h.z);
+        ext[9] = wave_shl1(ext[7], h.w);
+    } else {
+        ext[3] = wave_shr1(own[0], h.w);
+        ext[2] = wave_shr1(ext[3], h.z);
+        ext[1] = wave_shr1(ext[2], h.y);
+        ext[0] = wave_shr1(ext[1], h.x);
+        ext[5] = wave_shl1(own[0], h.x);
+        ext[6] = wave_shl1(ext[5], h.y);
+        ext[7] = wave_shl1(ext[6], h.z);
+        ext[8] = wave_shl1(ext[7], h.w);
     }
 }
 
@@ -198,33 +227,58 @@ __device__ __forceinline__ float leapfrog(float p, float o, float c, float lap, 
 }
 
 // z-extension of a field that exists only in registers (the inner field of a
-// two-step launch): rows outside the wave are never used for a written cell,
-// so the wrapped shuffle values stand in for them.
+// two-step launch): rows beyond the wave are never used for a written cell,
+// so the edge lanes take zeros (DPP bound_ctrl).
+__device__ __forceinline__ float wave_shr1z(float src)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(src), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float wave_shl1z(float src)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(src), 0x130, 0xF, 0xF, true));
+}
+
 template <int VZ>
-__device__ __forceinline__ void zext_local(float (&ext)[VZ + 8], const float (&own)[VZ], int lane)
+__device__ __forceinline__ void zext_local(float (&ext)[VZ + 8], const float (&own)[VZ])
 {
 #pragma unroll
     for (int r = 0; r < VZ; ++r) ext[4 + r] = own[r];
+    if constexpr (VZ == 4) {
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        const int k = kk - 4;
-        const int o = -((-k + VZ - 1) / VZ);
-        ext[kk] = __shfl(own[k - o * VZ], lane + o, 64);
-    }
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        const int k = VZ + kk;
-        const int o = k / VZ;
-        ext[VZ + 4 + kk] = __shfl(own[k - o * VZ], lane + o, 64);
+        for (int k = 0; k < 4; ++k) {
+            ext[k] = wave_shr1z(own[k]);
+            ext[8 + k] = wave_shl1z(own[k]);
+        }
+    } else if constexpr (VZ == 2) {
+        ext[2] = wave_shr1z(own[0]);
+        ext[3] = wave_shr1z(own[1]);
+        ext[0] = wave_shr1z(ext[2]);
+        ext[1] = wave_shr1z(ext[3]);
+        ext[6] = wave_shl1z(own[0]);
+        ext[7] = wave_shl1z(own[1]);
+        ext[8] = wave_shl1z(ext[6]);
+        ext[9] = wave_shl1z(ext[7]);
+    } else {
+        ext[3] = wave_shr1z(own[0]);
+        ext[2] = wave_shr1z(ext[3]);
+        ext[1] = wave_shr1z(ext[2]);
+        ext[0] = wave_shr1z(ext[1]);
+        ext[5] = wave_shl1z(own[0]);
+        ext[6] = wave_shl1z(ext[5]);
+        ext[7] = wave_shl1z(ext[6]);
+        ext[8] = wave_shl1z(ext[7]);
     }
 }
 
-// Column `col` of a slot field into registers; zero outside the grid.
+// Column `col` of a slot field into registers; zero outside the grid's
+// columns.  The load is not masked per lane: rows at or past ldz read the
+// next column's first rows (zero rim rows of a wavefield) or the zeroed
+// guard after each buffer, and only feed rim rows, which are never written.
 template <int VZ>
 __device__ __forceinline__ void ldcol(float (&r)[VZ], const float *base, int col, int nxx,
-                                      int ldz, int zb, bool active)
+                                      int ldz, int zb)
 {
-    if (active && col >= 0 && col < nxx) vload<VZ>(r, base + (long)col * ldz + zb);
+    if (col >= 0 && col < nxx) vload<VZ>(r, base + (long)col * ldz + zb);
     else vzero<VZ>(r);
 }
 
@@ -399,7 +453,7 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
     if (!tile_of(nzseg, nxchunk, nslots, id)) return;
     const Grid g = a.g;
     const int lane = threadIdx.x;
-    const int zb = id.zseg * 64 * VZ + lane * VZ;
+    const int zb = 4 + id.zseg * 64 * VZ + lane * VZ;   // rows 0-3 are rim rows
     const bool active = zb < g.ldz;
     const int j0 = 4 + id.xc * xw;
     const int j1 = min(j0 + xw, g.nxx - 4);
@@ -425,11 +479,13 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
 
     float q[9][VZ];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) ldcol<VZ>(q[k], cur, j0 - 4 + k, nxx, ldz, zb, active);
+    for (int k = 0; k < 9; ++k) ldcol<VZ>(q[k], cur, j0 - 4 + k, nxx, ldz, zb);
     float on[VZ], cn[VZ], pn[VZ];
-    ldcol<VZ>(on, old, j0, nxx, ldz, zb, active);
-    ldcol<VZ>(cn, cc, j0, nxx, ldz, zb, active);
+    ldcol<VZ>(on, old, j0, nxx, ldz, zb);
+    ldcol<VZ>(cn, cc, j0, nxx, ldz, zb);
     vzero<VZ>(pn);
+    const int z0 = zb - lane * VZ;                   // the wave's first row
+    float4 h = ldhalo<VZ>(cur, j0, 4, nxx - 4, ldz, z0, lane);
 
     for (int j = j0; j < j1; ++j) {
         const bool inx = (j >= g.nb) && (j < nxx - g.nb);
@@ -442,12 +498,13 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o[r] = on[r]; c[r] = cn[r]; }
         if (j + 1 < j1) {
-            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb, active);
-            ldcol<VZ>(on, old, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb);
+            ldcol<VZ>(on, old, j + 1, nxx, ldz, zb);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb);
         }
         float ext[VZ + 8];
-        zext<VZ>(ext, q[4], cur + (long)j * ldz, zb, lane, ldz);
+        zext<VZ>(ext, q[4], h);
+        h = ldhalo<VZ>(cur, j + 1, 4, nxx - 4, ldz, z0, lane);
         float nv[VZ], dv[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) {
@@ -489,9 +546,9 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
     const Grid g = a.g;
     const int lane = threadIdx.x;
     const int ldz = g.ldz, nxx = g.nxx;
-    const int zs = id.zseg * (64 * VZ - 8) - 4;     // first computed row
+    const int zs = id.zseg * (64 * VZ - 8);         // first computed row; owned from zs + 4
     const int zb = zs + lane * VZ;
-    const bool active = (zb >= 0) && (zb < ldz);
+    const bool active = zb < ldz;
     // a lane owns all or none of its rows (the bounds are multiples of VZ)
     const bool own = active && (zb >= zs + 4) && (zb < zs + 64 * VZ - 4);
     const int x0 = 4 + id.xc * xw;
@@ -523,22 +580,23 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
     float q2[9][VZ];     // u_{t+1}, columns j-8 .. j
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        ldcol<VZ>(q1[k], cur, x0 - 8 + k, nxx, ldz, zb, active);
+        ldcol<VZ>(q1[k], cur, x0 - 8 + k, nxx, ldz, zb);
         vzero<VZ>(q2[k]);
     }
     float on[VZ], cn[VZ], pn[VZ];
-    ldcol<VZ>(on, old, x0 - 4, nxx, ldz, zb, active);
-    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb, active);
+    ldcol<VZ>(on, old, x0 - 4, nxx, ldz, zb);
+    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb);
     vzero<VZ>(pn);
+    float4 h = ldhalo<VZ>(cur, x0 - 4, 4, nxx - 4, ldz, zs, lane);
 
     for (int j = x0 - 4; j < x1 + 4; ++j) {
         float o1[VZ], cj[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
         if (j + 1 < x1 + 4) {
-            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb, active);
-            ldcol<VZ>(on, old, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb);
+            ldcol<VZ>(on, old, j + 1, nxx, ldz, zb);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb);
         }
         // ---- step 1: u_{t+1} at column j (0 on the rim columns)
         float n1[VZ];
@@ -551,7 +609,8 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
                 dxj = staged ? st.dx[j - xa] : pad_factor(a.b1d, j, nxx, g.nb, in);
             }
             float ext[VZ + 8], dv[VZ];
-            zext<VZ>(ext, q1[4], cur + (long)j * ldz, zb, lane, ldz);
+            zext<VZ>(ext, q1[4], h);
+            h = ldhalo<VZ>(cur, j + 1, 4, nxx - 4, ldz, zs, lane);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
@@ -568,6 +627,8 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
                     else sample_receivers<VZ, ABC>(a, n1, j, zb, seis, t + 1 - a.tlag);
                 }
             }
+        } else {
+            h = ldhalo<VZ>(cur, j + 1, 4, nxx - 4, ldz, zs, lane);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k)
@@ -586,9 +647,9 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
                 dxj = staged ? st.dx[jj - xa] : pad_factor(a.b1d, jj, nxx, g.nb, in);
             }
             float c2[VZ];                               // c at column jj (read 4 columns ago)
-            ldcol<VZ>(c2, cc, jj, nxx, ldz, zb, active);
+            ldcol<VZ>(c2, cc, jj, nxx, ldz, zb);
             float ext[VZ + 8], dv[VZ];
-            zext_local<VZ>(ext, q2[4], lane);
+            zext_local<VZ>(ext, q2[4]);
             float n2[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
@@ -630,7 +691,7 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
     if (!tile_of(nzseg, nxchunk, nslots, id)) return;
     const Grid g = a.g;
     const int lane = threadIdx.x;
-    const int zb = id.zseg * 64 * VZ + lane * VZ;
+    const int zb = 4 + id.zseg * 64 * VZ + lane * VZ;   // rows 0-3 are rim rows
     const bool active = zb < g.ldz;
     const int j0 = 4 + id.xc * xw;
     const int j1 = min(j0 + xw, g.nxx - 4);
@@ -649,7 +710,7 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
     const bool inj_on = (k + 1 < a.nt);
     const int drow = a.data_by_shot ? (*a.shot0 + id.slot) : id.slot;
     const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
-    const int zw = id.zseg * 64 * VZ;
+    const int zw = 4 + id.zseg * 64 * VZ;
     __shared__ RevStage st;
     const bool staged = stage_rev(a, dat, j0, j1 - j0, zw, zw + 64 * VZ, inj_on, a.nt - 2 - k,
                                   false, 0, st, lane);
@@ -657,16 +718,18 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
     float qs[9][VZ], qr[9][VZ];
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-        ldcol<VZ>(qs[m], scur, j0 - 4 + m, nxx, ldz, zb, active);
-        ldcol<VZ>(qr[m], rcur, j0 - 4 + m, nxx, ldz, zb, active);
+        ldcol<VZ>(qs[m], scur, j0 - 4 + m, nxx, ldz, zb);
+        ldcol<VZ>(qr[m], rcur, j0 - 4 + m, nxx, ldz, zb);
     }
     float osn[VZ], orn[VZ], cn[VZ], an[VZ], in[VZ], psn[VZ], prn[VZ];
-    ldcol<VZ>(osn, sold, j0, nxx, ldz, zb, active);
-    ldcol<VZ>(orn, rold, j0, nxx, ldz, zb, active);
-    ldcol<VZ>(cn, cc, j0, nxx, ldz, zb, active);
-    ldcol<VZ>(an, acc, j0, nxx, ldz, zb, active);
-    ldcol<VZ>(in, ill, j0, nxx, ldz, zb, active);
+    ldcol<VZ>(osn, sold, j0, nxx, ldz, zb);
+    ldcol<VZ>(orn, rold, j0, nxx, ldz, zb);
+    ldcol<VZ>(cn, cc, j0, nxx, ldz, zb);
+    ldcol<VZ>(an, acc, j0, nxx, ldz, zb);
+    ldcol<VZ>(in, ill, j0, nxx, ldz, zb);
     vzero<VZ>(psn); vzero<VZ>(prn);
+    float4 hs = ldhalo<VZ>(scur, j0, 4, nxx - 4, ldz, zw, lane);
+    float4 hr = ldhalo<VZ>(rcur, j0, 4, nxx - 4, ldz, zw, lane);
 
     for (int j = j0; j < j1; ++j) {
         float os[VZ], orr[VZ], c[VZ], ac[VZ], il[VZ];
@@ -675,17 +738,19 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
             os[r] = osn[r]; orr[r] = orn[r]; c[r] = cn[r]; ac[r] = an[r]; il[r] = in[r];
         }
         if (j + 1 < j1) {
-            ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb, active);
-            ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb, active);
-            ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(an, acc, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(in, ill, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb);
+            ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb);
+            ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb);
+            ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb);
+            ldcol<VZ>(an, acc, j + 1, nxx, ldz, zb);
+            ldcol<VZ>(in, ill, j + 1, nxx, ldz, zb);
         }
         float es[VZ + 8], er[VZ + 8];
-        zext<VZ>(es, qs[4], scur + (long)j * ldz, zb, lane, ldz);
-        zext<VZ>(er, qr[4], rcur + (long)j * ldz, zb, lane, ldz);
+        zext<VZ>(es, qs[4], hs);
+        zext<VZ>(er, qr[4], hr);
+        hs = ldhalo<VZ>(scur, j + 1, 4, nxx - 4, ldz, zw, lane);
+        hr = ldhalo<VZ>(rcur, j + 1, 4, nxx - 4, ldz, zw, lane);
         float ns[VZ], nr[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) {
@@ -734,9 +799,9 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
     const Grid g = a.g;
     const int lane = threadIdx.x;
     const int ldz = g.ldz, nxx = g.nxx;
-    const int zs = id.zseg * (64 * VZ - 8) - 4;
+    const int zs = id.zseg * (64 * VZ - 8);
     const int zb = zs + lane * VZ;
-    const bool active = (zb >= 0) && (zb < ldz);
+    const bool active = zb < ldz;
     const bool own = active && (zb >= zs + 4) && (zb < zs + 64 * VZ - 4);
     const int x0 = 4 + id.xc * xw;
     const int x1 = min(x0 + xw, nxx - 4);
@@ -767,17 +832,19 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
     float iq[5][VZ], aq[5][VZ];      // step-1 illum / image contributions
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-        ldcol<VZ>(qs1[m], scur, x0 - 8 + m, nxx, ldz, zb, active);
-        ldcol<VZ>(qr1[m], rcur, x0 - 8 + m, nxx, ldz, zb, active);
+        ldcol<VZ>(qs1[m], scur, x0 - 8 + m, nxx, ldz, zb);
+        ldcol<VZ>(qr1[m], rcur, x0 - 8 + m, nxx, ldz, zb);
         vzero<VZ>(qs2[m]); vzero<VZ>(qr2[m]);
     }
 #pragma unroll
     for (int m = 0; m < 5; ++m) { vzero<VZ>(iq[m]); vzero<VZ>(aq[m]); }
     float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ];
-    ldcol<VZ>(osn, sold, x0 - 4, nxx, ldz, zb, active);
-    ldcol<VZ>(orn, rold, x0 - 4, nxx, ldz, zb, active);
-    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb, active);
+    ldcol<VZ>(osn, sold, x0 - 4, nxx, ldz, zb);
+    ldcol<VZ>(orn, rold, x0 - 4, nxx, ldz, zb);
+    ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb);
     vzero<VZ>(psn); vzero<VZ>(prn);
+    float4 hs = ldhalo<VZ>(scur, x0 - 4, 4, nxx - 4, ldz, zs, lane);
+    float4 hr = ldhalo<VZ>(rcur, x0 - 4, 4, nxx - 4, ldz, zs, lane);
 
     for (int j = x0 - 4; j < x1 + 4; ++j) {
         float os[VZ], orr[VZ], cj[VZ];
@@ -785,19 +852,21 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
         for (int r = 0; r < VZ; ++r) { os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r]; }
         const int jj = j - 4;
         if (j + 1 < x1 + 4) {
-            ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb, active);
-            ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb, active);
-            ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb, active);
-            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb, active);
+            ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb);
+            ldcol<VZ>(prn, rcur, j + 5, nxx, ldz, zb);
+            ldcol<VZ>(osn, sold, j + 1, nxx, ldz, zb);
+            ldcol<VZ>(orn, rold, j + 1, nxx, ldz, zb);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zb);
         }
         // ---- step 1 at column j
         float ns[VZ], nr[VZ], ci[VZ], ca[VZ];
         vzero<VZ>(ns); vzero<VZ>(nr); vzero<VZ>(ci); vzero<VZ>(ca);
         if (j >= 4 && j < nxx - 4) {
             float es[VZ + 8], er[VZ + 8];
-            zext<VZ>(es, qs1[4], scur + (long)j * ldz, zb, lane, ldz);
-            zext<VZ>(er, qr1[4], rcur + (long)j * ldz, zb, lane, ldz);
+            zext<VZ>(es, qs1[4], hs);
+            zext<VZ>(er, qr1[4], hr);
+            hs = ldhalo<VZ>(scur, j + 1, 4, nxx - 4, ldz, zs, lane);
+            hr = ldhalo<VZ>(rcur, j + 1, 4, nxx - 4, ldz, zs, lane);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
@@ -821,6 +890,9 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
                 vstore<VZ>(sout1 + (long)j * ldz + zb, ns);
                 vstore<VZ>(rout1 + (long)j * ldz + zb, nr);
             }
+        } else {
+            hs = ldhalo<VZ>(scur, j + 1, 4, nxx - 4, ldz, zs, lane);
+            hr = ldhalo<VZ>(rcur, j + 1, 4, nxx - 4, ldz, zs, lane);
         }
 #pragma unroll
         for (int m = 0; m < 8; ++m)
@@ -838,12 +910,12 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
         // ---- step 2 at column jj
         if (jj >= x0) {
             float ac[VZ], il[VZ], cjj[VZ];
-            ldcol<VZ>(ac, acc, jj, nxx, ldz, zb, own);
-            ldcol<VZ>(il, ill, jj, nxx, ldz, zb, own);
-            ldcol<VZ>(cjj, cc, jj, nxx, ldz, zb, active);   // c at jj (read 4 columns ago)
+            ldcol<VZ>(ac, acc, jj, nxx, ldz, zb);
+            ldcol<VZ>(il, ill, jj, nxx, ldz, zb);
+            ldcol<VZ>(cjj, cc, jj, nxx, ldz, zb);   // c at jj (read 4 columns ago)
             float es[VZ + 8], er[VZ + 8];
-            zext_local<VZ>(es, qs2[4], lane);
-            zext_local<VZ>(er, qr2[4], lane);
+            zext_local<VZ>(es, qs2[4]);
+            zext_local<VZ>(er, qr2[4]);
             float ns2[VZ], nr2[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
@@ -1143,10 +1215,12 @@ hipError_t rev_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, i
 
 }  // namespace
 
-int zsegments(int ldz, int vz, int steps)
+int zsegments(int nzz, int vz, int steps)
 {
+    // waves tile the updated rows 4 .. nzz-5; the rim rows stay zero
     const int rows = (steps == 2) ? 64 * vz - 8 : 64 * vz;
-    return (ldz + rows - 1) / rows;
+    const int n = (nzz - 8 + rows - 1) / rows;
+    return n > 1 ? n : 1;
 }
 
 hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
