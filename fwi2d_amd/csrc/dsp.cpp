@@ -1,5 +1,5 @@
 // dsp.cpp -- host-side numerics of the engine: the source wavelet (own FP64
-// DFT; the reference uses FFTW, src/modeling/modeling.cu:154-184), the Cerjan
+// DFT; the reference uses FFTW, src/modeling/modeling.cu:50-80), the Cerjan
 // table, image / gradient post-processing and the parabola vertex.  Plain C++.
 #include "dsp.h"
 
@@ -15,13 +15,13 @@ constexpr double kPi = 3.14159265358979323846;
 // Ricker 1e5 (1 - 2a) e^{-a}, a = pi^3 fc^2 (n dt - t0)^2, t0 = 2 sqrt(pi)/fmax,
 // fc = fmax/(3 sqrt(pi)), then multiplied in frequency by (i 2 pi f)^{1/2}
 // (principal root: sqrt(2 pi f) e^{i pi/4}, DC -> 0) and transformed back
-// with 1/nt (modeling.cu:136-194).  Only bins 0..nt/2 enter the inverse,
+// with 1/nt (modeling.cu:32-90).  Only bins 0..nt/2 enter the inverse,
 // as in a real-to-complex / complex-to-real transform pair.
 int ricker_half_derivative(int nt, double dt, double fmax, std::vector<double> &w)
 {
     const double t0 = 2.0 * std::sqrt(kPi) / fmax;
     const double fc = fmax / (3.0 * std::sqrt(kPi));
-    // tlag in single precision, as the reference computes it (modeling.cu:140-143)
+    // tlag in single precision, as the reference computes it (modeling.cu:36-39)
     const float t0f = 2.0f * std::sqrt((float)kPi) / (float)fmax;
     const int tlag = (int)(t0f / (float)dt) + 1;
     std::vector<double> s(nt);
@@ -72,7 +72,7 @@ int ricker_half_derivative(int nt, double dt, double fmax, std::vector<double> &
     return tlag;
 }
 
-// b1d[k] = exp(-(factor (nb - k))^2) (modeling.cu:229-232)
+// b1d[k] = exp(-(factor (nb - k))^2) (modeling.cu:125-128)
 void cerjan_table(int nb, double factor, std::vector<float> &b1d)
 {
     b1d.assign(nb > 0 ? nb : 1, 1.0f);
@@ -83,7 +83,7 @@ void cerjan_table(int nb, double factor, std::vector<float> &b1d)
 }
 
 // q = num/den per cell; out = -(q[i-1] - 2 q[i] + q[i+1]) / dh^2 for rows
-// row_lo < i < nz-1, else 0 (migration.cu:144-157, inversion.cu:144-158).
+// row_lo < i < nz-1, else 0 (migration.cu:98-111, inversion.cu:144-158).
 void second_derivative_z(const float *num, const float *den, int nz, int nx, double dh,
                          int row_lo, std::vector<double> &out)
 {

@@ -552,7 +552,7 @@ int fwi_set_wavelet(fwi_ctx *c, const float *w, int32_t tlag)
     if (!c || !w) return fail(FWI_ERR_ARG, "null argument");
     if (tlag < 1) return fail(FWI_ERR_ARG, "tlag must be >= 1");
     std::vector<float> h(c->cfg.nt);
-    for (int k = 0; k < c->cfg.nt; ++k) h[k] = c->idh2 * w[k];   // idh2 * w[t] (modeling.cu:421)
+    for (int k = 0; k < c->cfg.nt; ++k) h[k] = c->idh2 * w[k];   // idh2 * w[t] (modeling.cu:317)
     HIP_TRY(hipMemcpy(c->d_wav, h.data(), h.size() * 4, hipMemcpyHostToDevice));
     if (tlag != c->tlag) drop_graphs(c);
     c->tlag = tlag;

@@ -1,7 +1,7 @@
 // propagate.hip -- CDNA4 (gfx950) kernels of the 2-D acoustic engine.
 //
 // Time stepping is the reference's 8th-order-space / 2nd-order-time
-// leapfrog (src/modeling/modeling.cu:405-449, src/migration/migration.cu:177-226,
+// leapfrog (src/modeling/modeling.cu:301-345, src/migration/migration.cu:131-180,
 // src/inversion/inversion.cu:395-444), re-laid out for the hardware:
 //
 //  * z (the contiguous axis of every reference array) maps to lanes: a
@@ -28,9 +28,9 @@
 //
 // Source injection is folded into the write of the field it enters:
 // the field written at step t receives idh2*w[t+1], which is what the
-// reference's step t+1 adds before reading it (modeling.cu:420-421).  With
+// reference's step t+1 adds before reading it (modeling.cu:316-317).  With
 // the sponge on, the reference damps the current field in place
-// (modeling.cu:441-447); here the stored field is undamped and the factor is
+// (modeling.cu:337-343); here the stored field is undamped and the factor is
 // applied when it is read back as the previous field one step later:
 //   u_{t+1} = d * ((u_t + (u_t - d*u_{t-1})) + c*lap u_t)  [+ idh2 w[t+1]].
 #include "kernels.h"
@@ -140,7 +140,7 @@ __device__ __forceinline__ void zext(float (&ext)[VZ + 8], const float (&own)[VZ
 }
 
 // Cerjan factor of cell (i,j): 1 inside, b1d on the edges, b1d[a]+b1d[b]-1
-// in the corners (modeling.cu:459-506).  i is clamped into the grid.
+// in the corners (modeling.cu:355-402).  i is clamped into the grid.
 __device__ __forceinline__ float damper(const float *b1d, int i, int j, const Grid &g)
 {
     i = min(max(i, 0), g.nzz - 1);
@@ -200,7 +200,7 @@ __device__ __forceinline__ float add_nc(float a, float b)
 
 // Sponge factors split by axis: dz(i) = b1d of the row's edge distance in
 // the z pads (1 elsewhere), dx(j) likewise; the reference's lookup
-// (modeling.cu:459-506) is then d = dz (x inside), dx (z inside),
+// (modeling.cu:355-402) is then d = dz (x inside), dx (z inside),
 // (dz + dx) - 1 (corners), 1 (interior).
 __device__ __forceinline__ float pad_factor(const float *b1d, int k, int n, int nb, bool &in)
 {
@@ -1035,7 +1035,7 @@ __device__ __forceinline__ float rb_draw(uint64_t key, uint64_t cell, int k)
 
 // Seeded random boundary: pad cells take the edge value plus a ramped
 // uniform perturbation, once for the x pad and once for the z pad, clamped to
-// [vlo, vhi] (the structure of random_boundary_bg, modeling.cu:591-628, with
+// [vlo, vhi] (the structure of random_boundary_bg, modeling.cu:487-524, with
 // a counter-based draw).  FMA contraction is off in this kernel and every
 // float operation is one correctly rounded IEEE operation (the division
 // included), so the result is bit-identical to a sequential C statement of

@@ -1,4 +1,4 @@
-// modeling: drop-in for the reference's modeling.exe (src/modeling_main.cpp:3-30).
+// modeling: drop-in for the reference's modeling.exe (src/modeling_main.cpp:3-29).
 // Usage: modeling parameters.txt
 #include <chrono>
 #include <iostream>

@@ -140,7 +140,7 @@ void Migration::set_parameters()
     create_engine(rbc_nb, rbc_varVp, rbc_varVp_rel, seed);
     const long per = (long)geometry.nrec * nt;
     std::vector<float> data(per);
-    for (int s = 0; s < geometry.nsrc; ++s) {   // migration.cu:129-134, once per shot
+    for (int s = 0; s < geometry.nsrc; ++s) {   // migration.cu:83-88, once per shot
         import_binary_float(input_folder + input_prefix + std::to_string(s + 1) + ".bin",
                             data.data(), per);
         check(fwi_set_observed(ctx, s, 1, data.data()), "set_observed");

@@ -22,9 +22,9 @@ public:
     std::string parameters;
     Geometry geometry;
 
-    void set_parameters();        // modeling.cu:107-119
-    void forward_solver();        // all shots; modeling.cu:390-403
-    void get_seismogram();        // one file per shot; modeling.cu:383-388
+    void set_parameters();        // modeling.cu:3-15
+    void forward_solver();        // all shots; modeling.cu:286-299
+    void get_seismogram();        // one file per shot; modeling.cu:279-284
     void show_information() const;
 
 protected:
@@ -39,17 +39,17 @@ protected:
     std::string data_folder, title;
     fwi_ctx *ctx = nullptr;
 
-    void set_main_parameters();     // modeling.cu:121-134
-    void set_wavelet();             // modeling.cu:136-194
-    void set_properties();          // modeling.cu:256-292
+    void set_main_parameters();     // modeling.cu:17-30
+    void set_wavelet();             // modeling.cu:32-90
+    void set_properties();          // modeling.cu:152-188
     void create_engine(int rbc_nb, float varVp, float varVp_rel, uint64_t seed);
 };
 
 class Migration : public Modeling {
 public:
-    void set_parameters();          // migration.cu:49-82
+    void set_parameters();          // migration.cu:3-36
     void run_migration();           // forward_propagation + backward_propagation, all shots
-    void export_seismic();          // migration.cu:136-161
+    void export_seismic();          // migration.cu:90-115
 
 private:
     float rbc_varVp = 0, rbc_varVp_rel = 0;

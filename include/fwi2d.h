@@ -60,43 +60,43 @@ const char *fwi_last_error(void);
 const char *fwi_version(void);
 
 /* Creates a context and its stream.  Replaces the set_parameters chains of
- * Modeling (src/modeling/modeling.cu:107-119), Migration
- * (src/migration/migration.cu:49-82) and Inversion (src/inversion/inversion.cu:3-53)
+ * Modeling (src/modeling/modeling.cu:3-15), Migration
+ * (src/migration/migration.cu:3-36) and Inversion (src/inversion/inversion.cu:3-53)
  * for everything that is not read from files. */
 int fwi_create(const fwi_config *cfg, fwi_ctx **out);
 int fwi_destroy(fwi_ctx *ctx);
 
 /* Host helper: Ricker + half-derivative wavelet, FP64 DFT, rounded to FP32.
- * Replaces Modeling::set_wavelet (modeling.cu:136-194); returns tlag. */
+ * Replaces Modeling::set_wavelet (modeling.cu:32-90); returns tlag. */
 int fwi_ricker_wavelet(int32_t nt, float dt, float fmax, float *w_out, int32_t *tlag_out);
 
 /* Uploads the source wavelet w[nt]; T = nt + tlag steps per pass.
- * Replaces the cudaMemcpy of Modeling::set_wavelet (modeling.cu:186-188). */
+ * Replaces the cudaMemcpy of Modeling::set_wavelet (modeling.cu:82-84). */
 int fwi_set_wavelet(fwi_ctx *ctx, const float *w, int32_t tlag);
 
 /* Source / receiver coordinates in metres.  Grid indices follow
  * (int)((x + 0.5 dh)/dh) + nb per padded grid.  Replaces
  * Geometry::set_parameters (src/geometry/geometry.cpp:3-43) and
- * Modeling::initialization (modeling.cu:356-381). */
+ * Modeling::initialization (modeling.cu:252-277). */
 int fwi_set_geometry(fwi_ctx *ctx, int32_t nsrc, const float *xsrc, const float *zsrc,
                      int32_t nrec, const float *xrec, const float *zrec);
 
 /* Interior velocity model vp[nx][nz] (m/s).  Padded on the device by edge
  * replication for both grids.  Replaces Modeling::set_properties
- * (modeling.cu:256-292) and the model uploads of Inversion::linesearch /
+ * (modeling.cu:152-188) and the model uploads of Inversion::linesearch /
  * update_model (inversion.cu:310-321, 344-354). */
 int fwi_set_model(fwi_ctx *ctx, const float *vp);
 
 /* Observed (or migration input) data of shots [shot0, shot0+nshots):
  * data[nshots][nrec][nt], kept resident on the device.  Replaces
- * Migration::set_seismic_source (migration.cu:129-134) and
+ * Migration::set_seismic_source (migration.cu:83-88) and
  * Inversion::set_obs_data (inversion.cu:168-172). */
 int fwi_set_observed(fwi_ctx *ctx, int32_t shot0, int32_t nshots, const float *data);
 
 /* Forward modeling of shots [shot0, shot0+nshots) on the chosen boundary;
  * if seis_out != NULL it receives [nshots][nrec][nt] (sample 0 = 0).
  * Replaces Modeling::forward_solver + compute_pressure + compute_seismogram
- * + get_seismogram (modeling.cu:383-457). */
+ * + get_seismogram (modeling.cu:279-353). */
 int fwi_modeling(fwi_ctx *ctx, int32_t boundary, int32_t shot0, int32_t nshots,
                  float *seis_out);
 
@@ -104,7 +104,7 @@ int fwi_modeling(fwi_ctx *ctx, int32_t boundary, int32_t shot0, int32_t nshots,
  * random-boundary forward, time reversal and receiver back-propagation,
  * image += S*R and illum += S^2 accumulated on the device.  Replaces
  * Migration::forward_propagation / backward_propagation, inject_seismogram
- * and cross_correlation (migration.cu:95-127, 163-226). */
+ * and cross_correlation (migration.cu:49-81, 117-180). */
 int fwi_migration(fwi_ctx *ctx, int32_t shot0, int32_t nshots);
 
 /* FWI gradient of shots [shot0, shot0+nshots): sponge-boundary modeling,
@@ -128,7 +128,7 @@ int fwi_reset(fwi_ctx *ctx, int32_t what);
 
 /* RTM output: image_out[nx][nz] = -d2/dz2 (image / illum), zero on the first
  * and last rows; raw_image / raw_illum (cropped, may be NULL).  Replaces
- * Migration::export_seismic (migration.cu:136-161). */
+ * Migration::export_seismic (migration.cu:90-115). */
 int fwi_get_image(fwi_ctx *ctx, float *image_out, float *raw_image, float *raw_illum);
 
 /* FWI search direction: g = -d2/dz2 (grad / illum) below zmask, divided by

@@ -13,8 +13,8 @@
  * undefined, the oracle fixes one order and says which:
  *   - forward step: inject the source, stencil on the undamped current
  *     field, damp both fields, then sample receivers
- *     (src/modeling/modeling.cu:405-457);
- *   - seismogram sample 0 is 0 (modeling.cu:455 never writes it);
+ *     (src/modeling/modeling.cu:301-353);
+ *   - seismogram sample 0 is 0 (modeling.cu:351 never writes it);
  *   - the random boundary is this project's seeded generator
  *     (oracle_rb.c), not the reference's clock-seeded cuRAND one.
  *
@@ -32,7 +32,7 @@ extern "C" {
 
 /* ---- closed-form pieces (oracle_misc.c) -------------------------------- */
 
-/* Ricker + half-derivative source wavelet; modeling.cu:136-194.
+/* Ricker + half-derivative source wavelet; modeling.cu:32-90.
  * w[nt] out (FP64), returns tlag = (int)(t0/dt)+1 with t0 = 2*sqrt(pi)/fmax. */
 int or_wavelet(int nt, double dt, double fmax, double *w);
 
@@ -40,19 +40,19 @@ int or_wavelet(int nt, double dt, double fmax, double *w);
 void or_dft_r2c(int n, const double *x, double *re, double *im);    /* k = 0..n/2 */
 void or_dft_c2r(int n, const double *re, const double *im, double *x); /* unnormalised */
 
-/* Cerjan sponge table b1d[k] = exp(-(factor*(nb-k))^2); modeling.cu:217-254 */
+/* Cerjan sponge table b1d[k] = exp(-(factor*(nb-k))^2); modeling.cu:113-150 */
 void or_abc_b1d(int nb, double factor, double *b1d);
-/* Region lookup of the sponge factor at padded cell (i,j); modeling.cu:459-506 */
+/* Region lookup of the sponge factor at padded cell (i,j); modeling.cu:355-402 */
 double or_damper(const double *b1d, int i, int j, int nzz, int nxx, int nb);
 
-/* Edge-replication padding and crop; modeling.cu:294-334 */
+/* Edge-replication padding and crop; modeling.cu:190-230 */
 void or_expand(const float *in, int nz, int nx, int nb, float *out);
 void or_reduce(const float *in, int nz, int nx, int nb, float *out);
 
-/* Source / receiver index rule (int)((x+0.5dh)/dh)+nb; modeling.cu:356-381 */
+/* Source / receiver index rule (int)((x+0.5dh)/dh)+nb; modeling.cu:252-277 */
 int or_grid_index(float x, float dh, int nb);
 
-/* -d2/dz2 of image/illum, zero on the top/bottom rows; migration.cu:136-161.
+/* -d2/dz2 of image/illum, zero on the top/bottom rows; migration.cu:90-115.
  * image, illum: cropped [nx][nz]; out [nx][nz] */
 void or_image_post(const double *image, const double *illum, int nz, int nx,
                    double dh, double *out);
@@ -76,7 +76,7 @@ void or_laplacian_f64(int nzz, int nxx, double dh, const double *P, double *out)
 void or_laplacian_f32(int nzz, int nxx, double dh, const float *P, float *out);
 
 /*
- * Forward pass (Modeling::forward_solver, modeling.cu:390-403, with
+ * Forward pass (Modeling::forward_solver, modeling.cu:286-299, with
  * compute_pressure :405-449 and compute_seismogram :451-457).
  *   vp     padded velocity [nxx][nzz] (float, as the reference stores it)
  *   w      wavelet[nt] (already in the working precision)
@@ -98,7 +98,7 @@ void or_forward_f32(int nzz, int nxx, int nb, double dh, double dt, int nt, int 
 
 /*
  * Reverse pass: source-wavefield reversal + receiver back-propagation
- * + imaging (mode 0, migration.cu:107-127,163-226) or gradient (mode 1,
+ * + imaging (mode 0, migration.cu:61-81,117-180) or gradient (mode 1,
  * inversion.cu:230-248,381-444).
  *   S_cur  = u_T   (reference d_P after the forward pass)
  *   S_old  = u_{T-1} (reference d_Pold)

@@ -49,16 +49,16 @@ void or_dft_c2r(int n, const double *re, const double *im, double *x)
     }
 }
 
-/* Modeling::set_wavelet, src/modeling/modeling.cu:136-194: Ricker wavelet
+/* Modeling::set_wavelet, src/modeling/modeling.cu:32-90: Ricker wavelet
  * 1e5 (1 - 2a) e^{-a}, a = pi^3 fc^2 (n dt - t0)^2, filtered in frequency by
  * (i 2 pi f)^{1/2} (DC -> 0) and transformed back with 1/nt. */
 int or_wavelet(int nt, double dt_in, double fmax_in, double *w)
 {
-    /* the reference parses dt and fmax with std::stof (modeling.cu:129-131) */
+    /* the reference parses dt and fmax with std::stof (modeling.cu:25-27) */
     const double dt = (double)(float)dt_in, fmax = (double)(float)fmax_in;
     const double t0 = 2.0 * sqrt(M_PI) / fmax;
     const double fc = fmax / (3.0 * sqrt(M_PI));
-    /* tlag in single precision, as modeling.cu:140-143 computes it */
+    /* tlag in single precision, as modeling.cu:36-39 computes it */
     const float t0f = 2.0f * sqrtf((float)M_PI) / (float)fmax;
     const int tlag = (int)(t0f / (float)dt) + 1;
     double *s = (double *)malloc((size_t)nt * sizeof(double));
@@ -85,7 +85,7 @@ int or_wavelet(int nt, double dt_in, double fmax_in, double *w)
     return tlag;
 }
 
-/* Modeling::set_abc_dampers, modeling.cu:229-232 */
+/* Modeling::set_abc_dampers, modeling.cu:125-128 */
 void or_abc_b1d(int nb, double factor, double *b1d)
 {
     for (int k = 0; k < nb; k++) {
@@ -94,7 +94,7 @@ void or_abc_b1d(int nb, double factor, double *b1d)
     }
 }
 
-/* get_boundary_damper, modeling.cu:459-506: 1 inside, b1d on the four edges,
+/* get_boundary_damper, modeling.cu:355-402: 1 inside, b1d on the four edges,
  * b2d = b1d[a] + b1d[b] - 1 in the corners (b2d built at :234-244). */
 double or_damper(const double *b1d, int i, int j, int nzz, int nxx, int nb)
 {
@@ -108,7 +108,7 @@ double or_damper(const double *b1d, int i, int j, int nzz, int nxx, int nb)
     return b1d[a] + b1d[b] - 1.0;
 }
 
-/* Modeling::expand_boundary, modeling.cu:294-322 */
+/* Modeling::expand_boundary, modeling.cu:190-218 */
 void or_expand(const float *in, int nz, int nx, int nb, float *out)
 {
     const int nzz = nz + 2 * nb, nxx = nx + 2 * nb;
@@ -127,7 +127,7 @@ void or_expand(const float *in, int nz, int nx, int nb, float *out)
         }
 }
 
-/* Modeling::reduce_boundary, modeling.cu:324-334 */
+/* Modeling::reduce_boundary, modeling.cu:220-230 */
 void or_reduce(const float *in, int nz, int nx, int nb, float *out)
 {
     const int nzz = nz + 2 * nb;
@@ -136,13 +136,13 @@ void or_reduce(const float *in, int nz, int nx, int nb, float *out)
             out[i + (long)j * nz] = in[(i + nb) + (long)(j + nb) * nzz];
 }
 
-/* Modeling::initialization, modeling.cu:361-373 (float arithmetic) */
+/* Modeling::initialization, modeling.cu:257-269 (float arithmetic) */
 int or_grid_index(float x, float dh, int nb)
 {
     return (int)((x + 0.5f * dh) / dh) + nb;
 }
 
-/* Migration::export_seismic, migration.cu:144-157 */
+/* Migration::export_seismic, migration.cu:98-111 */
 void or_image_post(const double *image, const double *illum, int nz, int nx,
                    double dh_in, double *out)
 {

@@ -5,11 +5,11 @@
  * IEEE-754 operation, so the GPU generator can be checked bit for bit.
  *
  * The reference draws its pad velocity from a clock-seeded cuRAND normal per
- * cell (src/modeling/modeling.cu:591-635) plus Poisson-disk blobs
- * (:535-589, src/admin/admin.cpp:238-329) and is not reproducible.  This
+ * cell (src/modeling/modeling.cu:487-531) plus Poisson-disk blobs
+ * (modeling.cu:431-485, src/admin/admin.cpp:115-206) and is not reproducible.  This
  * generator keeps its structure -- clamp-to-edge base value, a linear ramp
  * from the interior edge (0) to the outer edge (1) applied once for the x pad
- * and once for the z pad (modeling.cu:601-624), pad cells only, clamped to a
+ * and once for the z pad (modeling.cu:497-520), pad cells only, clamped to a
  * velocity range (:586-587) -- with a counter-based draw:
  *
  *   key(seed, stream)       = mix64(seed ^ mix64(stream + G))

@@ -136,7 +136,7 @@ def test_sponge_absorbs():
 
 def brute_force_forward(vpad, dh, dt, nt, tlag, w, s, rec, d=None):
     """Independent vectorised statement of compute_pressure / compute_seismogram
-    (modeling.cu:405-457): inject, stencil on the current field, damp, sample."""
+    (modeling.cu:301-353): inject, stencil on the current field, damp, sample."""
     nxx, nzz = vpad.shape
     dh, dt = float(np.float32(dh)), float(np.float32(dt))    # float parameters
     P = np.zeros((nxx, nzz))
