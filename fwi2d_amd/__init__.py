@@ -7,8 +7,9 @@ reference's ``parameters.txt``.  This Python package only binds the ABI for
 tests and benchmarks.
 """
 from .abi import (ABC, RBC, RESET_GRADIENT, RESET_ILLUM, RESET_IMAGE, Engine, FwiError, Net,
-                  lib, mfma_probe, parabola_step, ricker_wavelet, stream_copy)
+                  guarded_step, lib, mfma_probe, parabola_step, ricker_wavelet,
+                  stream_copy)
 
 __all__ = ["ABC", "RBC", "RESET_IMAGE", "RESET_GRADIENT", "RESET_ILLUM", "Engine",
-           "FwiError", "Net", "lib", "mfma_probe", "parabola_step", "ricker_wavelet",
+           "FwiError", "Net", "guarded_step", "lib", "mfma_probe", "parabola_step", "ricker_wavelet",
            "stream_copy"]

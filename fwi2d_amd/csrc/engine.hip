@@ -809,6 +809,43 @@ int fwi_parabola_step(const double *alpha, const double *f, double *step)
     return FWI_OK;
 }
 
+int fwi_guarded_step(const double *alpha, const double *f, const float *m, const float *g,
+                     int64_t n, double *step, int32_t *rule)
+{
+    if (!alpha || !f || !m || !g || !step || !rule || n < 1)
+        return fail(FWI_ERR_ARG, "bad arguments");
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(f[k])) return fail(FWI_ERR_NUMERIC, "non-finite misfit");
+    if (!(alpha[0] < alpha[1] && alpha[1] < alpha[2]))
+        return fail(FWI_ERR_ARG, "trial steps must increase");
+    auto positive = [&](double s) {
+        for (int64_t k = 0; k < n; ++k)
+            if (!((float)(m[k] - (float)s * g[k]) > 0.f)) return false;
+        return true;
+    };
+    // curvature of the parabola through the three points (> 0: convex)
+    const double s01 = (f[1] - f[0]) / (alpha[1] - alpha[0]);
+    const double s02 = (f[2] - f[0]) / (alpha[2] - alpha[0]);
+    const double curv = (s02 - s01) / (alpha[2] - alpha[1]);
+    double v = 0.0;
+    if (curv > 0.0 && fwi::parabola_vertex(alpha, f, &v) && v > 0.0 && v <= 4.0 * alpha[2] &&
+        positive(v)) {
+        *step = v;
+        *rule = 0;
+        return FWI_OK;
+    }
+    const int order[2] = {(f[2] < f[1]) ? 2 : 1, (f[2] < f[1]) ? 1 : 2};
+    for (int kb : order)
+        if (f[kb] < f[0] && positive(alpha[kb])) {
+            *step = alpha[kb];
+            *rule = 1;
+            return FWI_OK;
+        }
+    *step = 0.0;
+    *rule = 2;
+    return FWI_OK;
+}
+
 int fwi_rb_velocity(fwi_ctx *c, int32_t shot, uint32_t iteration, float *out)
 {
     TRY(check_ready(c));

@@ -249,16 +249,20 @@ void Inversion::optimization()
     f[0] = residuo.back();
     f[1] = linesearch((float)a[1]);
     f[2] = linesearch((float)a[2]);
-    check(fwi_parabola_step(a, f, &step), "parabola");
+    int32_t rule = 0;
+    check(fwi_guarded_step(a, f, model.data(), gradient.data(), (int64_t)model.size(), &step,
+                           &rule),
+          "step");
+    static const char *names[3] = {"parabola vertex", "best trial step", "no decrease, no step"};
+    std::cout << "line search: f = " << f[0] << ", " << f[1] << ", " << f[2] << " -> step "
+              << step << " (" << names[rule] << ")\n";
 }
 
 void Inversion::update_model()
 {
     std::vector<float> v(model.size());
     for (size_t k = 0; k < model.size(); ++k) {
-        model[k] = model[k] - (float)step * gradient[k];
-        if (!(model[k] > 0.f))
-            throw std::runtime_error("model update produced a non-positive slowness^2");
+        model[k] = model[k] - (float)step * gradient[k];   // positive by the step guard
         v[k] = 1.0f / std::sqrt(model[k]);
     }
     vp = v;

@@ -143,6 +143,18 @@ int fwi_get_gradient(fwi_ctx *ctx, float zmask, float *grad_out, float *raw_grad
  * FWI_ERR_NUMERIC for a degenerate parabola. */
 int fwi_parabola_step(const double *alpha, const double *f, double *step);
 
+/* Step length for the update m - step * g with a guard the reference lacks
+ * (it applies the vertex of inversion.cu:304-307 unconditionally at
+ * inversion.cu:349, which gives NaN velocities when m turns non-positive).
+ * Rule 0: the parabola vertex, when the parabola through (alpha_k, f_k) is
+ * convex, the vertex lies in (0, 4 alpha_2] and every m[k] - step g[k] stays
+ * positive.  Rule 1: otherwise the trial alpha_1 or alpha_2 with the lower
+ * misfit below f_0 that keeps m positive (the other one next).  Rule 2:
+ * otherwise step = 0.  m and g hold n
+ * values; *rule receives the rule taken. */
+int fwi_guarded_step(const double *alpha, const double *f, const float *m, const float *g,
+                     int64_t n, double *step, int32_t *rule);
+
 /* Test hook: the random-boundary velocity [nxx][nzz] (rbc grid) that a pass
  * of shot `shot` in FWI iteration `iteration` uses (migration: iteration 0). */
 int fwi_rb_velocity(fwi_ctx *ctx, int32_t shot, uint32_t iteration, float *vp_padded_out);

@@ -54,6 +54,35 @@ def test_host_parabola_step():
         assert e.value.code == abi.FWI_ERR_NUMERIC
 
 
+def test_guarded_step_rules():
+    a = np.array([0.0, 0.02, 0.05])
+    m = np.full(50, 1.0, np.float32)
+    g = np.linspace(-1.0, 1.0, 50).astype(np.float32)
+    # convex, vertex inside (0, 4 alpha_2]: the vertex itself (rule 0)
+    f = 3.0 * (a - 0.031) ** 2 + 1.0
+    s, r = F.guarded_step(a, f, m, g)
+    assert r == 0 and s == F.parabola_step(a, f)
+    # convex but the vertex beyond 4 alpha_2: best trial (rule 1)
+    f = 3.0 * (a - 0.5) ** 2
+    s, r = F.guarded_step(a, f, m, g)
+    assert (s, r) == (0.05, 1)
+    # concave (the vertex is a maximum): best trial below f0 (rule 1)
+    f = np.array([1.0, 0.9, 0.5])
+    s, r = F.guarded_step(a, f, m, g)
+    assert (s, r) == (0.05, 1)
+    # concave and nothing decreases: no step (rule 2)
+    f = np.array([1.0, 1.5, 1.6])
+    assert F.guarded_step(a, f, m, g) == (0.0, 2)
+    # the vertex would make some m non-positive: fall back
+    f = 3.0 * (a - 0.04) ** 2 + 1.0
+    m2 = m.copy()
+    m2[-1] = 0.03                                   # m - 0.04 * 1 < 0 at the last cell
+    s, r = F.guarded_step(a, f, m2, g)
+    assert r == 1 and s == 0.02 and np.all(m2 - np.float32(s) * g > 0)
+    with pytest.raises(F.FwiError):
+        F.guarded_step([0.0, 0.05, 0.02], [1.0, 0.9, 0.8], m, g)   # alphas not increasing
+
+
 def test_wavelet_rejects_bad_arguments():
     with pytest.raises(F.FwiError) as e:
         F.ricker_wavelet(1, 1e-3, 30.0)

@@ -1,0 +1,117 @@
+"""GPU end-to-end test of the drop-in executables: bin/modeling, bin/migration
+and bin/inversion, each driven by a parameters.txt in the reference's format
+(run/parameters.txt), on the reference's overthrust model.
+
+The executables and the library path share every kernel, so the modeling
+seismograms and the migration image written by the executables must equal,
+bit for bit, what the C ABI returns for the same inputs.  The inversion must
+write the reference's output files (model per iteration, final model,
+convergence text) and lower the data residual.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import fwi2d_amd as F
+from fwi2d_amd import configs as C
+from cases import overthrust
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "fwi2d_amd", "bin")
+
+NT, DT, FMAX = 1001, 1e-3, 30.0
+SURVEY = C.Survey("overthrust", 81, 201, 25.0, NT, DT, FMAX, 41, 4, 51, 100.0, 0.0,
+                  varVp_rel=0.0, varVp=1000.0, abc_nb=51, abc_factor=0.004, seed=77)
+XS = np.array([1000.0, 2000.0, 3000.0, 4000.0], np.float32)
+XR = np.linspace(0.0, 5000.0, 51).astype(np.float32)
+
+
+def run(exe, params):
+    print(f"running {exe} {params}", flush=True)
+    r = subprocess.run([os.path.join(BIN, exe), params], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-3000:])
+    print(r.stderr[-3000:])
+    assert r.returncode == 0, (exe, r.returncode)
+    return r.stdout
+
+
+def setup_files(tmp):
+    v = overthrust()
+    vsmooth = C.smooth_slowness(v, 6.0)
+    true_file, init_file = os.path.join(tmp, "true.bin"), os.path.join(tmp, "init.bin")
+    v.tofile(true_file)
+    vsmooth.tofile(init_file)
+    sps, rps = os.path.join(tmp, "sps.txt"), os.path.join(tmp, "rps.txt")
+    zs = np.full(len(XS), SURVEY.zsrc, np.float32)
+    zr = np.full(len(XR), SURVEY.zrec, np.float32)
+    C.write_geometry(sps, rps, XS, zs, XR, zr)
+    folders = C.ensure_dirs(tmp)
+    prefix = f"seismogram_nt{NT}_nr{len(XR)}_{int(1e6 * DT)}us_shot_"
+    params = {}
+    for name, model in (("mod", true_file), ("mig", init_file), ("inv", init_file)):
+        p = os.path.join(tmp, f"parameters_{name}.txt")
+        C.write_parameters(p, SURVEY, model, sps, rps, folders, max_iteration=3,
+                           depth_mask=100.0, mig_prefix=prefix, inv_prefix=prefix)
+        params[name] = p
+    return v, vsmooth, zs, zr, folders, prefix, params
+
+
+def test_three_mains_end_to_end(tmp_path):
+    tmp = str(tmp_path)
+    v, vsmooth, zs, zr, folders, prefix, params = setup_files(tmp)
+    s = SURVEY
+    nrec = len(XR)
+
+    # ---- modeling: one seismogram file per shot, equal to the library path
+    run("modeling", params["mod"])
+    files = [os.path.join(folders["data"], f"{prefix}{k + 1}.bin") for k in range(len(XS))]
+    obs = np.stack([np.fromfile(f, np.float32).reshape(nrec, NT) for f in files])
+    w, tlag = F.ricker_wavelet(NT, DT, FMAX)
+    eng = F.Engine(s.nz, s.nx, s.dh, NT, DT, abc_nb=s.abc_nb, abc_factor=s.abc_factor,
+                   rbc_nb=s.nb, rbc_varVp=s.varVp, rbc_varVp_rel=s.varVp_rel, rbc_seed=s.seed,
+                   max_batch=len(XS))
+    eng.set_wavelet(w, tlag)
+    eng.set_geometry(XS, zs, XR, zr)
+    eng.set_model(v)
+    ref = eng.modeling(F.ABC)
+    assert np.array_equal(obs, ref)
+    assert np.all(obs[:, :, 0] == 0.0) and np.abs(obs).max() > 0
+
+    # ---- migration: the RTM section equals the library's post-processed image
+    run("migration", params["mig"])
+    img_file = os.path.join(folders["seismic"], f"RTM_section_{int(FMAX)}Hz_{s.nz}x{s.nx}_"
+                                                f"{int(s.dh)}m.bin")
+    img = np.fromfile(img_file, np.float32).reshape(s.nx, s.nz)
+    eng.set_model(vsmooth)
+    eng.set_observed(obs)
+    eng.reset()
+    eng.migration()
+    ref_img, _, _ = eng.get_image()
+    assert np.array_equal(img, ref_img)
+    assert np.isfinite(img).all() and np.abs(img).max() > 0
+
+    # ---- inversion: 3 iterations, the reference's outputs, residual goes down
+    out = run("inversion", params["inv"])
+    suffix = f"{int(FMAX)}Hz_{s.nz}x{s.nx}_{int(s.dh)}m.bin"
+    for k in range(1, 4):
+        m = np.fromfile(os.path.join(folders["models"], f"model_FWI_iteration_{k}_{suffix}"),
+                        np.float32)
+        assert m.size == s.nx * s.nz and np.isfinite(m).all()
+    final = np.fromfile(os.path.join(folders["models"], f"final_model_FWI_{suffix}"),
+                        np.float32).reshape(s.nx, s.nz)
+    res = np.loadtxt(os.path.join(folders["residuo"], "convergence_3_iterations.txt"))
+    print("residuo", res)
+    assert res.shape == (4,)
+    assert res[-1] < res[0]
+    # the model moved towards the true model
+    e0 = np.linalg.norm(vsmooth - v)
+    e1 = np.linalg.norm(final - v)
+    print(f"model error {e0:.1f} -> {e1:.1f}")
+    assert e1 < e0
+    assert "residuo" in out
+    eng.close()
