@@ -155,6 +155,42 @@ def config3_models(seed=3):
     return v, smooth_slowness(v, 30.0)
 
 
+def config4_models(seed, nz=256, nx=768, sigma=None):
+    """Training family (config-2 style): 4-7 seeded dipping layers from 1500 m/s
+    to 3500-4500 m/s, 0-3 Gaussian lenses (+-300 m/s) and, every other seed, a
+    salt-like body; starting model = smoothed slowness (sigma defaults to 20
+    cells at nz = 301, scaled with nz)."""
+    rng = np.random.default_rng(seed)
+    v = dipping_layers(nz, nx, int(rng.integers(4, 8)), 1500.0,
+                       float(rng.uniform(3500.0, 4500.0)), seed)
+    k = nz / 301.0
+    for _ in range(int(rng.integers(0, 4))):
+        gaussian_lens(v, rng.uniform(0.25, 0.85) * nz, rng.uniform(0.1, 0.9) * nx,
+                      rng.uniform(10.0, 25.0) * k, float(rng.choice([-300.0, 300.0])))
+    if seed % 2 == 0:
+        salt_polygon(v, rng.uniform(0.45, 0.7) * nz, rng.uniform(0.25, 0.75) * nx,
+                     0.1 * nz, 0.08 * nx, 4500.0, seed + 11)
+    np.clip(v, 1500.0, 4500.0, out=v)
+    return v, smooth_slowness(v, sigma if sigma else 20.0 * k)
+
+
+def config5_models(seed=5, nz=401, nx=1601, sigma=None):
+    """Comparison model: layered 1500-4500 m/s, two salt-like bodies and five
+    Gaussian lenses, all seeded; starting model = smoothed slowness (sigma 40
+    cells at nz = 401, scaled with nz)."""
+    rng = np.random.default_rng(seed)
+    k = nz / 401.0
+    v = flat_layers(nz, nx, list(np.linspace(1500.0, 4500.0, 7)))
+    for j in range(5):
+        gaussian_lens(v, rng.uniform(0.2, 0.8) * nz, rng.uniform(0.1, 0.9) * nx,
+                      rng.uniform(15.0, 30.0) * k, 300.0 if j % 2 == 0 else -300.0)
+    for j in range(2):
+        salt_polygon(v, rng.uniform(0.45, 0.7) * nz, (0.3 + 0.4 * j) * nx,
+                     0.09 * nz, 0.07 * nx, 4500.0, seed + 17 * (j + 1))
+    np.clip(v, 1500.0, 4500.0, out=v)
+    return v, smooth_slowness(v, sigma if sigma else 40.0 * k)
+
+
 def mute_direct_wave(data, xs, zs, xr, zr, dt, velocity, delay, gstd=30.0, normalise=True):
     """Direct-wave mute of one shot (run/generate_input_data.py:22-46):
     data[nrec, nt] (t fastest) is divided by its max |.| and the samples

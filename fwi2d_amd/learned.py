@@ -1,0 +1,351 @@
+"""Learned-preconditioner FWI on top of the engine and the network ABI.
+
+Three tools:
+
+* ``make_samples`` runs plain FWI (the reference's algorithm:
+  inversion.cu:111-358, with the accepted model used next and the guarded
+  step) on seeded synthetic models and records, per iteration k, the input
+  (raw gradient, illumination, model m_k) and the target m_end - m_k: the
+  accepted updates that the plain run still makes after iteration k.  A
+  single accepted update -step*g is collinear with the post-processed
+  gradient, so it would only teach the network the post-processing; the
+  accumulated updates carry what the remaining iterations add.
+* ``train`` fits the network (BF16 MFMA kernels, Adam) to those samples; every
+  channel and the target are normalised by their maximum magnitude per sample.
+* ``run_fwi`` runs plain or learned FWI.  A learned iteration computes the
+  gradient, the plain two-point line search and its guarded step, then tries
+  the network's direction d at two multiples of the plain step length; the
+  better trial is taken only when its forward-modeled misfit is below the
+  misfit the plain line search predicts for its own step, otherwise the plain
+  step is taken.  With a network that outputs zeros the learned run is the
+  plain run, step for step.
+
+Multiscale bands rebuild the wavelet and re-model the observed data per band
+from the true model (a synthetic inverse-crime setup).
+"""
+import dataclasses
+import json
+import os
+import time
+
+import numpy as np
+
+from . import abi as A
+from . import configs as C
+
+
+@dataclasses.dataclass(frozen=True)
+class FwiSetup:
+    nz: int
+    nx: int
+    dh: float
+    nt: int
+    dt: float
+    nsrc: int
+    nrec: int
+    zsrc: float
+    zrec: float
+    rbc_nb: int
+    abc_nb: int
+    varVp_rel: float
+    seed: int
+    peaks: tuple            # Ricker peak of each band [Hz]; max_frequency = 3 x peak
+    band_iters: tuple       # iterations per band
+    zmask: float            # gradient depth mask [m]
+
+    @property
+    def iterations(self):
+        return int(sum(self.band_iters))
+
+    @property
+    def abc_factor(self):
+        return 0.35 / self.abc_nb        # edge factor exp(-0.35^2) for any pad width
+
+    def band(self, it):
+        acc = 0
+        for b, n in enumerate(self.band_iters):
+            acc += n
+            if it < acc:
+                return b
+        return len(self.band_iters) - 1
+
+    def acquisition(self):
+        xmax = (self.nx - 1) * self.dh
+        xs = np.linspace(0.02 * xmax, 0.98 * xmax, self.nsrc).astype(np.float32)
+        xr = np.linspace(0.0, xmax, self.nrec).astype(np.float32)
+        return (xs, np.full(self.nsrc, self.zsrc, np.float32), xr,
+                np.full(self.nrec, self.zrec, np.float32))
+
+
+# Reduced-size versions of the training (config 4) and comparison (config 5)
+# workloads: same model families, bands and rules, smaller grids and shot
+# counts so that generation, training and both comparison runs fit in minutes.
+TRAIN_SMALL = FwiSetup(nz=128, nx=384, dh=10.0, nt=1875, dt=8e-4, nsrc=16, nrec=192,
+                       zsrc=20.0, zrec=10.0, rbc_nb=60, abc_nb=60, varVp_rel=0.4, seed=4,
+                       peaks=(4.0, 8.0, 12.0), band_iters=(3, 3, 2), zmask=60.0)
+COMPARE_SMALL = dataclasses.replace(TRAIN_SMALL, nsrc=32, nrec=384, seed=5,
+                                    band_iters=(7, 7, 6))
+
+
+def velocity(m):
+    return (1.0 / np.sqrt(m)).astype(np.float32)
+
+
+def slowness2(v):
+    v = np.asarray(v, np.float64)
+    return (1.0 / (v * v)).astype(np.float32)
+
+
+class Fwi:
+    """One survey on one engine: observed data per band, the gradient pass,
+    the misfit of a model and the plain line search."""
+
+    def __init__(self, setup, v_true, v_init):
+        s = self.setup = setup
+        self.eng = A.Engine(s.nz, s.nx, s.dh, s.nt, s.dt, abc_nb=s.abc_nb,
+                            abc_factor=s.abc_factor, rbc_nb=s.rbc_nb, rbc_varVp=0.0,
+                            rbc_varVp_rel=s.varVp_rel, rbc_seed=s.seed, max_batch=s.nsrc)
+        self.eng.set_geometry(*s.acquisition())
+        self.eng.set_model(v_true)
+        self.bands = []
+        for peak in s.peaks:
+            w, tlag = A.ricker_wavelet(s.nt, s.dt, 3.0 * peak)
+            self.eng.set_wavelet(w, tlag)
+            self.bands.append((w, tlag, self.eng.modeling(A.ABC)))
+        self.v_true = np.asarray(v_true, np.float32)
+        self.m0 = slowness2(v_init)
+        self.dm = float(1.0 / float(np.min(v_init)) ** 2 - 1.0 / float(np.max(v_init)) ** 2)
+        self.alphas = np.array([0.0, 0.02 * self.dm, 0.05 * self.dm])
+        self.current = None
+
+    def close(self):
+        self.eng.close()
+
+    def use_band(self, b):
+        if b != self.current:
+            w, tlag, obs = self.bands[b]
+            self.eng.set_wavelet(w, tlag)
+            self.eng.set_observed(obs)
+            self.current = b
+
+    def misfit(self, m):
+        self.eng.set_model(velocity(m))
+        return float(np.sqrt(self.eng.misfit()))
+
+    def gradient(self, m, it):
+        self.eng.set_model(velocity(m))
+        self.eng.reset(A.RESET_GRADIENT)
+        s2 = self.eng.gradient(iteration=it)
+        g, raw, ill, _ = self.eng.get_gradient(self.setup.zmask)
+        return float(np.sqrt(s2)), g, raw, ill
+
+    def plain_search(self, m, g, f0):
+        """Two trial misfits, the guarded step and the misfit the line search
+        predicts for it (parabola value for the vertex, the trial's own misfit
+        otherwise)."""
+        a = self.alphas
+        f = np.array([f0, self.misfit(m - np.float32(a[1]) * g),
+                      self.misfit(m - np.float32(a[2]) * g)])
+        step, rule = A.guarded_step(a, f, m, g)
+        if rule == 0:
+            coef = np.polyfit(a, f, 2)
+            pred = float(np.polyval(coef, step))
+        elif rule == 1:
+            pred = float(f[int(np.argmin(np.abs(a - step)))])
+        else:
+            pred = float(f0)
+        return step, rule, pred, f
+
+
+def normalise_inputs(raw, ill, m):
+    x = np.stack([raw, ill, m]).astype(np.float32)
+    for c in range(3):
+        s = float(np.max(np.abs(x[c])))
+        if s > 0:
+            x[c] /= s
+    return x
+
+
+class Direction:
+    """The network as a search-direction generator for [nx, nz] grids."""
+
+    def __init__(self, nx, nz, params=None, seed=0, batch=1):
+        if nx % 4 or nz % 4:
+            raise ValueError("grid sides must be multiples of 4 (two 2x2 pools)")
+        self.net = A.Net(batch, nx, nz, seed)
+        if params is not None:
+            self.net.set_params(params)
+
+    def close(self):
+        self.net.close()
+
+    def __call__(self, raw, ill, m):
+        d = self.net.forward(normalise_inputs(raw, ill, m)[None])[0]
+        s = float(np.max(np.abs(d)))
+        return d / s if s > 0 else d
+
+
+def run_fwi(fwi, direction=None, multiples=(2.0, 4.0), iterations=None, log=None):
+    """Plain FWI (direction=None) or learned FWI with the acceptance rule.
+    Returns a history list with the misfit f0 of every iteration (at its
+    band), the wall time since the start, the rule taken and the model error,
+    plus the final misfit and the final model."""
+    s = fwi.setup
+    its = s.iterations if iterations is None else iterations
+    m = fwi.m0.copy()
+    fwi.eng.reset()                       # illumination accumulates from here on
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(its):
+        fwi.use_band(s.band(it))
+        f0, g, raw, ill = fwi.gradient(m, it)
+        step, rule, pred, f = fwi.plain_search(m, g, f0)
+        taken = "plain"
+        new = m - np.float32(step) * g
+        if direction is not None and step > 0.0:
+            d = direction(raw, ill, m)
+            best = None
+            for k in multiples:
+                trial = m + np.float32(k * step) * d
+                if not np.all(trial > 0):
+                    continue
+                fl = fwi.misfit(trial)
+                if best is None or fl < best[0]:
+                    best = (fl, trial)
+            if best is not None and best[0] < pred:
+                new, taken = best[1], "learned"
+        m = new
+        err = float(np.sqrt(np.mean((velocity(m) - fwi.v_true) ** 2)))
+        hist.append({"iteration": it, "band": s.band(it), "misfit": f0,
+                     "time": time.perf_counter() - t0, "step": step, "rule": rule,
+                     "taken": taken, "model_rms_error": err})
+        if log:
+            log(hist[-1])
+    fwi.use_band(s.band(its - 1))
+    fend = fwi.misfit(m)
+    hist.append({"iteration": its, "band": s.band(its - 1), "misfit": fend,
+                 "time": time.perf_counter() - t0, "step": 0.0, "rule": -1, "taken": "end",
+                 "model_rms_error": float(np.sqrt(np.mean((velocity(m) - fwi.v_true) ** 2)))})
+    return hist, m
+
+
+def make_samples(setup, seeds, log=None):
+    """Plain-FWI samples: X [N, 3, nx, nz] (normalised raw gradient,
+    illumination, model), Y [N, nx, nz] (normalised m_end - m_k), and the
+    per-sample scale max|m_end - m_k| / dm."""
+    xs, ys, scales = [], [], []
+    for seed in seeds:
+        v, v0 = C.config4_models(seed, setup.nz, setup.nx)
+        fwi = Fwi(setup, v, v0)
+        m = fwi.m0.copy()
+        fwi.eng.reset()
+        inputs = []
+        for it in range(setup.iterations):
+            fwi.use_band(setup.band(it))
+            f0, g, raw, ill = fwi.gradient(m, it)
+            step, _, _, _ = fwi.plain_search(m, g, f0)
+            inputs.append((raw, ill, m.copy()))
+            m = m - np.float32(step) * g
+        for raw, ill, mk in inputs:
+            t = (m - mk).astype(np.float32)
+            s = float(np.max(np.abs(t)))
+            if s <= 0:
+                continue
+            xs.append(normalise_inputs(raw, ill, mk))
+            ys.append(t / s)
+            scales.append(s / fwi.dm)
+        fwi.close()
+        if log:
+            log({"seed": seed, "samples": len(xs)})
+    return np.stack(xs), np.stack(ys), np.array(scales)
+
+
+def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, log=None):
+    """Adam on the GPU; returns (params, history of (epoch, train loss,
+    held-out loss))."""
+    n, _, h, w = x.shape
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(n)
+    nval = max(1, int(round(holdout * n)))
+    val, tr = order[:nval], order[nval:]
+    net = A.Net(batch, h, w, seed)
+    hist = []
+    for ep in range(epochs):
+        rng.shuffle(tr)
+        losses = []
+        for b0 in range(0, len(tr) - batch + 1, batch):
+            idx = tr[b0:b0 + batch]
+            losses.append(net.train_step(x[idx], y[idx], lr))
+        vl = []
+        for b0 in range(0, nval, batch):
+            idx = val[b0:b0 + batch]
+            vl.append(net.gradient(x[idx], y[idx])[0] * len(idx))
+        hist.append((ep, float(np.mean(losses)), float(np.sum(vl) / nval)))
+        if log:
+            log({"epoch": ep, "train_loss": hist[-1][1], "holdout_loss": hist[-1][2]})
+    p = net.get_params()
+    net.close()
+    return p, hist, val
+
+
+def cosine(a, b):
+    a, b = a.ravel().astype(np.float64), b.ravel().astype(np.float64)
+    return float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
+
+
+def time_to_misfit(hist, target):
+    """Wall time at which a run's misfit first reaches `target` (the misfit at
+    an iteration counts at the time that iteration ends), or None."""
+    for h in hist:
+        if h["misfit"] <= target:
+            return h["time"]
+    return None
+
+
+def main(argv=None):
+    import argparse
+    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    p.add_argument("--out", default="learned_run")
+    p.add_argument("--train-models", type=int, default=48)
+    p.add_argument("--epochs", type=int, default=40)
+    p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--compare-seed", type=int, default=10005)
+    a = p.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+    log = lambda d: print(json.dumps(d), flush=True)
+
+    t0 = time.perf_counter()
+    x, y, scales = make_samples(TRAIN_SMALL, range(1, 1 + a.train_models), log)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    params, th, _ = train(x, y, epochs=a.epochs, lr=a.lr, log=log)
+    t_train = time.perf_counter() - t0
+    params.tofile(os.path.join(a.out, "preconditioner_params.bin"))
+
+    s = COMPARE_SMALL
+    v, v0 = C.config5_models(a.compare_seed, s.nz, s.nx)
+    fwi = Fwi(s, v, v0)
+    plain, _ = run_fwi(fwi, None, log=log)
+    d = Direction(s.nx, s.nz, params)
+    learned, _ = run_fwi(fwi, d, log=log)
+    d.close()
+    fwi.close()
+    target = plain[-1]["misfit"]
+    t_plain = plain[-1]["time"]
+    t_learn = time_to_misfit(learned, target)
+    report = {"samples": int(len(x)), "generation_s": t_gen, "training_s": t_train,
+              "train_loss_first_last": [th[0][1], th[-1][1]],
+              "holdout_loss_first_last": [th[0][2], th[-1][2]],
+              "plain_final_misfit": target, "plain_time_s": t_plain,
+              "learned_final_misfit": learned[-1]["misfit"], "learned_time_s": learned[-1]["time"],
+              "learned_time_to_plain_misfit_s": t_learn,
+              "time_ratio": (t_learn / t_plain) if t_learn else None,
+              "learned_steps_taken": sum(h["taken"] == "learned" for h in learned),
+              "plain": plain, "learned": learned}
+    with open(os.path.join(a.out, "report.json"), "w") as f:
+        json.dump(report, f, indent=1)
+    log({k: v for k, v in report.items() if k not in ("plain", "learned")})
+
+
+if __name__ == "__main__":
+    main()
