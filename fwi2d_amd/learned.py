@@ -179,19 +179,27 @@ class Direction:
     def close(self):
         self.net.close()
 
-    def __call__(self, raw, ill, m):
+    def __call__(self, raw, ill, m, g):
         d = self.net.forward(normalise_inputs(raw, ill, m)[None])[0]
         s = float(np.max(np.abs(d)))
         return d / s if s > 0 else d
 
 
-def run_fwi(fwi, direction=None, multiples=(2.0, 4.0), iterations=None, log=None):
-    """Plain FWI (direction=None) or learned FWI with the acceptance rule.
+def plain_direction(raw, ill, m, g):
+    """Control: the plain search direction -g (max |g| = 1) in place of the
+    network, with the same trial multiples and acceptance rule."""
+    return -g
+
+
+def run_fwi(fwi, direction=None, multiples=(2.0, 4.0), band_iters=None, log=None):
+    """Plain FWI (direction=None) or FWI with a trial direction and the
+    acceptance rule.  band_iters overrides the setup's iterations per band.
     Returns a history list with the misfit f0 of every iteration (at its
     band), the wall time since the start, the rule taken and the model error,
     plus the final misfit and the final model."""
-    s = fwi.setup
-    its = s.iterations if iterations is None else iterations
+    s = fwi.setup if band_iters is None else dataclasses.replace(fwi.setup,
+                                                                 band_iters=tuple(band_iters))
+    its = s.iterations
     m = fwi.m0.copy()
     fwi.eng.reset()                       # illumination accumulates from here on
     hist = []
@@ -203,7 +211,7 @@ def run_fwi(fwi, direction=None, multiples=(2.0, 4.0), iterations=None, log=None
         taken = "plain"
         new = m - np.float32(step) * g
         if direction is not None and step > 0.0:
-            d = direction(raw, ill, m)
+            d = direction(raw, ill, m, g)
             best = None
             for k in multiples:
                 trial = m + np.float32(k * step) * d
@@ -310,6 +318,8 @@ def main(argv=None):
     p.add_argument("--epochs", type=int, default=40)
     p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--compare-seed", type=int, default=10005)
+    p.add_argument("--short-bands", default="",
+                   help="iterations per band of extra, shorter control and learned runs")
     a = p.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     log = lambda d: print(json.dumps(d), flush=True)
@@ -325,26 +335,34 @@ def main(argv=None):
     s = COMPARE_SMALL
     v, v0 = C.config5_models(a.compare_seed, s.nz, s.nx)
     fwi = Fwi(s, v, v0)
-    plain, _ = run_fwi(fwi, None, log=log)
     d = Direction(s.nx, s.nz, params)
-    learned, _ = run_fwi(fwi, d, log=log)
+    runs = {"plain": run_fwi(fwi, None, log=log)[0],
+            "control": run_fwi(fwi, plain_direction, log=log)[0],
+            "learned": run_fwi(fwi, d, log=log)[0]}
+    if a.short_bands:
+        bi = tuple(int(k) for k in a.short_bands.split(","))
+        runs["control_short"] = run_fwi(fwi, plain_direction, band_iters=bi, log=log)[0]
+        runs["learned_short"] = run_fwi(fwi, d, band_iters=bi, log=log)[0]
     d.close()
     fwi.close()
-    target = plain[-1]["misfit"]
-    t_plain = plain[-1]["time"]
-    t_learn = time_to_misfit(learned, target)
+    target = runs["plain"][-1]["misfit"]
+    t_plain = runs["plain"][-1]["time"]
+    summary = {}
+    for name, h in runs.items():
+        t = time_to_misfit(h, target) if h[-1]["band"] == runs["plain"][-1]["band"] else None
+        summary[name] = {"final_misfit": h[-1]["misfit"], "time_s": h[-1]["time"],
+                         "final_model_rms_error": h[-1]["model_rms_error"],
+                         "time_to_plain_final_misfit_s": t,
+                         "time_ratio": (t / t_plain) if t else None,
+                         "trial_steps_taken": sum(e["taken"] == "learned" for e in h)}
     report = {"samples": int(len(x)), "generation_s": t_gen, "training_s": t_train,
               "train_loss_first_last": [th[0][1], th[-1][1]],
               "holdout_loss_first_last": [th[0][2], th[-1][2]],
               "plain_final_misfit": target, "plain_time_s": t_plain,
-              "learned_final_misfit": learned[-1]["misfit"], "learned_time_s": learned[-1]["time"],
-              "learned_time_to_plain_misfit_s": t_learn,
-              "time_ratio": (t_learn / t_plain) if t_learn else None,
-              "learned_steps_taken": sum(h["taken"] == "learned" for h in learned),
-              "plain": plain, "learned": learned}
+              "summary": summary, "runs": runs}
     with open(os.path.join(a.out, "report.json"), "w") as f:
         json.dump(report, f, indent=1)
-    log({k: v for k, v in report.items() if k not in ("plain", "learned")})
+    log({k: v for k, v in report.items() if k != "runs"})
 
 
 if __name__ == "__main__":
