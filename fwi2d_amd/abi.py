@@ -69,7 +69,7 @@ SIGNATURES = [
     ("fwi_get_image", C.c_int, [_ctx, _vp, _vp, _vp]),
     ("fwi_get_gradient", C.c_int, [_ctx, C.c_float, _vp, _vp, _vp, C.POINTER(C.c_double)]),
     ("fwi_parabola_step", C.c_int, [_f64p, _f64p, C.POINTER(C.c_double)]),
-    ("fwi_guarded_step", C.c_int, [_f64p, _f64p, _f32p, _f32p, C.c_int64,
+    ("fwi_guarded_step", C.c_int, [_f64p, _f64p, _f32p, _f32p, C.c_int64, C.c_double,
                                    C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     ("fwi_rb_velocity", C.c_int, [_ctx, C.c_int32, C.c_uint32, _f32p]),
     ("fwi_grid_shape", C.c_int, [_ctx, C.c_int32, C.POINTER(C.c_int32),
@@ -151,7 +151,7 @@ def parabola_step(alpha, f):
     return s.value
 
 
-def guarded_step(alpha, f, m, g):
+def guarded_step(alpha, f, m, g, m_min=0.0):
     """(step, rule) of fwi_guarded_step for the update m - step * g."""
     m = np.ascontiguousarray(m, np.float32).ravel()
     g = np.ascontiguousarray(g, np.float32).ravel()
@@ -160,7 +160,7 @@ def guarded_step(alpha, f, m, g):
     s, r = C.c_double(0.0), C.c_int32(0)
     _check(lib().fwi_guarded_step(np.ascontiguousarray(alpha, np.float64),
                                   np.ascontiguousarray(f, np.float64), m, g, m.size,
-                                  C.byref(s), C.byref(r)))
+                                  float(m_min), C.byref(s), C.byref(r)))
     return s.value, r.value
 
 

@@ -810,33 +810,40 @@ int fwi_parabola_step(const double *alpha, const double *f, double *step)
 }
 
 int fwi_guarded_step(const double *alpha, const double *f, const float *m, const float *g,
-                     int64_t n, double *step, int32_t *rule)
+                     int64_t n, double m_min, double *step, int32_t *rule)
 {
-    if (!alpha || !f || !m || !g || !step || !rule || n < 1)
+    if (!alpha || !f || !m || !g || !step || !rule || n < 1 || !(m_min >= 0.0))
         return fail(FWI_ERR_ARG, "bad arguments");
-    for (int k = 0; k < 3; ++k)
-        if (!std::isfinite(f[k])) return fail(FWI_ERR_NUMERIC, "non-finite misfit");
+    if (!std::isfinite(f[0])) return fail(FWI_ERR_NUMERIC, "non-finite misfit at alpha_0");
+    for (int k = 1; k < 3; ++k)
+        if (std::isnan(f[k]) || f[k] < 0.0) return fail(FWI_ERR_NUMERIC, "invalid trial misfit");
     if (!(alpha[0] < alpha[1] && alpha[1] < alpha[2]))
         return fail(FWI_ERR_ARG, "trial steps must increase");
-    auto positive = [&](double s) {
-        for (int64_t k = 0; k < n; ++k)
-            if (!((float)(m[k] - (float)s * g[k]) > 0.f)) return false;
+    const float lo = (float)m_min;
+    auto admissible = [&](double s) {
+        for (int64_t k = 0; k < n; ++k) {
+            const float mk = m[k] - (float)s * g[k];
+            if (!(mk > 0.f && mk >= lo)) return false;
+        }
         return true;
     };
-    // curvature of the parabola through the three points (> 0: convex)
-    const double s01 = (f[1] - f[0]) / (alpha[1] - alpha[0]);
-    const double s02 = (f[2] - f[0]) / (alpha[2] - alpha[0]);
-    const double curv = (s02 - s01) / (alpha[2] - alpha[1]);
-    double v = 0.0;
-    if (curv > 0.0 && fwi::parabola_vertex(alpha, f, &v) && v > 0.0 && v <= 4.0 * alpha[2] &&
-        positive(v)) {
-        *step = v;
-        *rule = 0;
-        return FWI_OK;
+    const bool finite = std::isfinite(f[1]) && std::isfinite(f[2]);
+    if (finite) {
+        // curvature of the parabola through the three points (> 0: convex)
+        const double s01 = (f[1] - f[0]) / (alpha[1] - alpha[0]);
+        const double s02 = (f[2] - f[0]) / (alpha[2] - alpha[0]);
+        const double curv = (s02 - s01) / (alpha[2] - alpha[1]);
+        double v = 0.0;
+        if (curv > 0.0 && fwi::parabola_vertex(alpha, f, &v) && v > 0.0 && v <= 4.0 * alpha[2] &&
+            admissible(v)) {
+            *step = v;
+            *rule = 0;
+            return FWI_OK;
+        }
     }
     const int order[2] = {(f[2] < f[1]) ? 2 : 1, (f[2] < f[1]) ? 1 : 2};
     for (int kb : order)
-        if (f[kb] < f[0] && positive(alpha[kb])) {
+        if (f[kb] < f[0] && admissible(alpha[kb])) {
             *step = alpha[kb];
             *rule = 1;
             return FWI_OK;

@@ -1,6 +1,8 @@
 // survey.cpp -- see survey.hpp.
 #include "survey.hpp"
 
+#include <limits>
+
 #include <algorithm>
 #include <cmath>
 #include <fstream>
@@ -226,13 +228,22 @@ void Inversion::check_convergence()
     std::cout << "residuo[" << residuo.size() - 1 << "] = " << residuo.back() << "\n";
 }
 
+double Inversion::min_slowness2() const
+{
+    // 1/v^2 of the fastest velocity the propagator accepts (with a margin)
+    const double vmax_stable = 0.999 * 0.5546 * (double)dh / (double)dt;
+    return 1.0 / (vmax_stable * vmax_stable);
+}
+
 double Inversion::linesearch(float alpha)
 {
+    // a trial model that cannot be propagated (m <= 0 or faster than the
+    // stability limit) has no misfit: +inf, which the step rule skips
+    const float m_min = (float)min_slowness2();
     std::vector<float> trial(model.size());
     for (size_t k = 0; k < model.size(); ++k) {
         const float m = model[k] - alpha * gradient[k];
-        if (!(m > 0.f))
-            throw std::runtime_error("line search: non-positive slowness^2 at a trial step");
+        if (!(m > 0.f && m >= m_min)) return std::numeric_limits<double>::infinity();
         trial[k] = 1.0f / std::sqrt(m);
     }
     check(fwi_set_model(ctx, trial.data()), "set_model (trial)");
@@ -250,8 +261,8 @@ void Inversion::optimization()
     f[1] = linesearch((float)a[1]);
     f[2] = linesearch((float)a[2]);
     int32_t rule = 0;
-    check(fwi_guarded_step(a, f, model.data(), gradient.data(), (int64_t)model.size(), &step,
-                           &rule),
+    check(fwi_guarded_step(a, f, model.data(), gradient.data(), (int64_t)model.size(),
+                           min_slowness2(), &step, &rule),
           "step");
     static const char *names[3] = {"parabola vertex", "best trial step", "no decrease, no step"};
     std::cout << "line search: f = " << f[0] << ", " << f[1] << ", " << f[2] << " -> step "

@@ -81,7 +81,8 @@ private:
     double sum_res = 0, step = 0;
     float vmin0 = 0, vmax0 = 0;     // extremes of the initial model (dm of the line search)
     std::string input_folder, input_prefix, output_folder, residuo_folder;
-    double linesearch(float alpha);
+    double linesearch(float alpha);  // inversion.cu:310-342
+    double min_slowness2() const;
 };
 
 }  // namespace fwi

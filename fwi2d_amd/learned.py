@@ -116,6 +116,8 @@ class Fwi:
         self.m0 = slowness2(v_init)
         self.dm = float(1.0 / float(np.min(v_init)) ** 2 - 1.0 / float(np.max(v_init)) ** 2)
         self.alphas = np.array([0.0, 0.02 * self.dm, 0.05 * self.dm])
+        vstable = 0.999 * 0.5546 * s.dh / s.dt     # fastest velocity the propagator accepts
+        self.m_min = 1.0 / (vstable * vstable)
         self.current = None
 
     def close(self):
@@ -129,6 +131,9 @@ class Fwi:
             self.current = b
 
     def misfit(self, m):
+        """Misfit of model m, +inf when m cannot be propagated."""
+        if not np.all((m > 0) & (m >= np.float32(self.m_min))):
+            return float("inf")
         self.eng.set_model(velocity(m))
         return float(np.sqrt(self.eng.misfit()))
 
@@ -146,7 +151,7 @@ class Fwi:
         a = self.alphas
         f = np.array([f0, self.misfit(m - np.float32(a[1]) * g),
                       self.misfit(m - np.float32(a[2]) * g)])
-        step, rule = A.guarded_step(a, f, m, g)
+        step, rule = A.guarded_step(a, f, m, g, self.m_min)
         if rule == 0:
             coef = np.polyfit(a, f, 2)
             pred = float(np.polyval(coef, step))
@@ -215,12 +220,10 @@ def run_fwi(fwi, direction=None, multiples=(2.0, 4.0), band_iters=None, log=None
             best = None
             for k in multiples:
                 trial = m + np.float32(k * step) * d
-                if not np.all(trial > 0):
-                    continue
                 fl = fwi.misfit(trial)
                 if best is None or fl < best[0]:
                     best = (fl, trial)
-            if best is not None and best[0] < pred:
+            if best is not None and np.isfinite(best[0]) and best[0] < pred:
                 new, taken = best[1], "learned"
         m = new
         err = float(np.sqrt(np.mean((velocity(m) - fwi.v_true) ** 2)))
@@ -318,18 +321,23 @@ def main(argv=None):
     p.add_argument("--epochs", type=int, default=40)
     p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--compare-seed", type=int, default=10005)
+    p.add_argument("--params", default="", help="trained parameters (skips generation and training)")
     p.add_argument("--short-bands", default="",
                    help="iterations per band of extra, shorter control and learned runs")
     a = p.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     log = lambda d: print(json.dumps(d), flush=True)
 
-    t0 = time.perf_counter()
-    x, y, scales = make_samples(TRAIN_SMALL, range(1, 1 + a.train_models), log)
-    t_gen = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    params, th, _ = train(x, y, epochs=a.epochs, lr=a.lr, log=log)
-    t_train = time.perf_counter() - t0
+    if a.params:
+        params = np.fromfile(a.params, np.float32)
+        x, th, t_gen, t_train = np.zeros((0,)), [(0, None, None)], None, None
+    else:
+        t0 = time.perf_counter()
+        x, y, scales = make_samples(TRAIN_SMALL, range(1, 1 + a.train_models), log)
+        t_gen = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        params, th, _ = train(x, y, epochs=a.epochs, lr=a.lr, log=log)
+        t_train = time.perf_counter() - t0
     params.tofile(os.path.join(a.out, "preconditioner_params.bin"))
 
     s = COMPARE_SMALL

@@ -146,14 +146,16 @@ int fwi_parabola_step(const double *alpha, const double *f, double *step);
 /* Step length for the update m - step * g with a guard the reference lacks
  * (it applies the vertex of inversion.cu:304-307 unconditionally at
  * inversion.cu:349, which gives NaN velocities when m turns non-positive).
+ * "Admissible" means every m[k] - step g[k] >= m_min (m_min = 1/v^2 of the
+ * fastest velocity the caller can propagate, or 0 for positivity only).
  * Rule 0: the parabola vertex, when the parabola through (alpha_k, f_k) is
- * convex, the vertex lies in (0, 4 alpha_2] and every m[k] - step g[k] stays
- * positive.  Rule 1: otherwise the trial alpha_1 or alpha_2 with the lower
- * misfit below f_0 that keeps m positive (the other one next).  Rule 2:
- * otherwise step = 0.  m and g hold n
+ * convex, the vertex lies in (0, 4 alpha_2] and is admissible.  Rule 1:
+ * otherwise the trial alpha_1 or alpha_2 with the lower misfit below f_0 that
+ * is admissible (the other one next).  Rule 2: otherwise step = 0.  f_1 or
+ * f_2 may be +inf for a trial that could not be modeled.  m and g hold n
  * values; *rule receives the rule taken. */
 int fwi_guarded_step(const double *alpha, const double *f, const float *m, const float *g,
-                     int64_t n, double *step, int32_t *rule);
+                     int64_t n, double m_min, double *step, int32_t *rule);
 
 /* Test hook: the random-boundary velocity [nxx][nzz] (rbc grid) that a pass
  * of shot `shot` in FWI iteration `iteration` uses (migration: iteration 0). */

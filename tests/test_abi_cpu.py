@@ -81,6 +81,15 @@ def test_guarded_step_rules():
     assert r == 1 and s == 0.02 and np.all(m2 - np.float32(s) * g > 0)
     with pytest.raises(F.FwiError):
         F.guarded_step([0.0, 0.05, 0.02], [1.0, 0.9, 0.8], m, g)   # alphas not increasing
+    # a trial that could not be modeled (+inf) is no decrease
+    s, r = F.guarded_step(a, [1.0, 0.9, np.inf], m, g)
+    assert (s, r) == (0.02, 1)
+    assert F.guarded_step(a, [1.0, np.inf, np.inf], m, g) == (0.0, 2)
+    # m_min bounds the update from below, not just at zero
+    f = 3.0 * (a - 0.03) ** 2 + 1.0
+    s, r = F.guarded_step(a, f, m, g, m_min=0.975)    # the vertex would reach 0.97
+    assert (s, r) == (0.02, 1)
+    assert np.all(m - np.float32(s) * g >= 0.975)
 
 
 def test_wavelet_rejects_bad_arguments():
