@@ -5,20 +5,25 @@ Three tools:
 * ``make_samples`` runs plain FWI (the reference's algorithm:
   inversion.cu:111-358, with the accepted model used next and the guarded
   step) on seeded synthetic models and records, per iteration k, the input
-  (raw gradient, illumination, model m_k) and the target m_end - m_k: the
-  accepted updates that the plain run still makes after iteration k.  A
-  single accepted update -step*g is collinear with the post-processed
-  gradient, so it would only teach the network the post-processing; the
-  accumulated updates carry what the remaining iterations add.
+  (raw gradient, illumination, model m_k) and the accepted updates that the
+  plain run still makes after iteration k, m_end - m_k.  A single accepted
+  update -step*g is collinear with the post-processed gradient g, so it would
+  only teach the network the post-processing; the accumulated updates carry
+  what the remaining iterations add.
+* The network is a learned correction of the plain direction: the search
+  direction is d = -g + net(x) (then scaled to max |d| = 1), and the training
+  target is the normalised accumulated update minus -g.  A network that
+  outputs zeros gives d = -g.
 * ``train`` fits the network (BF16 MFMA kernels, Adam) to those samples; every
-  channel and the target are normalised by their maximum magnitude per sample.
-* ``run_fwi`` runs plain or learned FWI.  A learned iteration computes the
-  gradient, the plain two-point line search and its guarded step, then tries
-  the network's direction d at two multiples of the plain step length; the
-  better trial is taken only when its forward-modeled misfit is below the
+  input channel and the accumulated update are normalised by their maximum
+  magnitude per sample.
+* ``run_fwi`` runs plain FWI or FWI with a trial direction.  Such an iteration
+  computes the gradient, the plain two-point line search and its guarded
+  step, then tries the direction d at two multiples of the plain step length;
+  the better trial is taken only when its forward-modeled misfit is below the
   misfit the plain line search predicts for its own step, otherwise the plain
-  step is taken.  With a network that outputs zeros the learned run is the
-  plain run, step for step.
+  step is taken.  ``plain_direction`` (d = -g) is the control that isolates
+  what the network adds to the longer trial steps.
 
 Multiscale bands rebuild the wavelet and re-model the observed data per band
 from the true model (a synthetic inverse-crime setup).
@@ -185,7 +190,7 @@ class Direction:
         self.net.close()
 
     def __call__(self, raw, ill, m, g):
-        d = self.net.forward(normalise_inputs(raw, ill, m)[None])[0]
+        d = self.net.forward(normalise_inputs(raw, ill, m)[None])[0] - g
         s = float(np.max(np.abs(d)))
         return d / s if s > 0 else d
 
@@ -242,9 +247,10 @@ def run_fwi(fwi, direction=None, multiples=(2.0, 4.0), band_iters=None, log=None
 
 def make_samples(setup, seeds, log=None):
     """Plain-FWI samples: X [N, 3, nx, nz] (normalised raw gradient,
-    illumination, model), Y [N, nx, nz] (normalised m_end - m_k), and the
+    illumination, model), Y [N, nx, nz] (the network's target: normalised
+    m_end - m_k plus the plain direction g), G [N, nx, nz] (g) and the
     per-sample scale max|m_end - m_k| / dm."""
-    xs, ys, scales = [], [], []
+    xs, ys, gs, scales = [], [], [], []
     for seed in seeds:
         v, v0 = C.config4_models(seed, setup.nz, setup.nx)
         fwi = Fwi(setup, v, v0)
@@ -255,20 +261,21 @@ def make_samples(setup, seeds, log=None):
             fwi.use_band(setup.band(it))
             f0, g, raw, ill = fwi.gradient(m, it)
             step, _, _, _ = fwi.plain_search(m, g, f0)
-            inputs.append((raw, ill, m.copy()))
+            inputs.append((raw, ill, m.copy(), g))
             m = m - np.float32(step) * g
-        for raw, ill, mk in inputs:
+        for raw, ill, mk, gk in inputs:
             t = (m - mk).astype(np.float32)
             s = float(np.max(np.abs(t)))
             if s <= 0:
                 continue
             xs.append(normalise_inputs(raw, ill, mk))
-            ys.append(t / s)
+            ys.append(t / s + gk)
+            gs.append(gk)
             scales.append(s / fwi.dm)
         fwi.close()
         if log:
             log({"seed": seed, "samples": len(xs)})
-    return np.stack(xs), np.stack(ys), np.array(scales)
+    return np.stack(xs), np.stack(ys), np.stack(gs), np.array(scales)
 
 
 def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, log=None):
@@ -299,16 +306,32 @@ def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, log=None):
     return p, hist, val
 
 
+def holdout_cosines(params, x, y, g, idx):
+    """Mean cosine between the accumulated update (y - g) and the learned
+    direction, and between it and the plain direction -g, over samples idx."""
+    n, _, h, w = x.shape
+    d = Direction(h, w, params)
+    cl, cp = [], []
+    for i in idx:
+        t = y[i] - g[i]
+        cl.append(cosine(d.net.forward(x[i][None])[0] - g[i], t))
+        cp.append(cosine(-g[i], t))
+    d.close()
+    return float(np.mean(cl)), float(np.mean(cp))
+
+
 def cosine(a, b):
     a, b = a.ravel().astype(np.float64), b.ravel().astype(np.float64)
     return float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
 
 
-def time_to_misfit(hist, target):
-    """Wall time at which a run's misfit first reaches `target` (the misfit at
-    an iteration counts at the time that iteration ends), or None."""
+def time_to_misfit(hist, target, band):
+    """Wall time at which a run's misfit in `band` first reaches `target`
+    (misfits of different bands measure different data and are not
+    compared), or None.  The misfit of an iteration's starting model counts
+    at the end of that iteration (its gradient pass measures it)."""
     for h in hist:
-        if h["misfit"] <= target:
+        if h["band"] == band and h["misfit"] <= target:
             return h["time"]
     return None
 
@@ -331,13 +354,16 @@ def main(argv=None):
     if a.params:
         params = np.fromfile(a.params, np.float32)
         x, th, t_gen, t_train = np.zeros((0,)), [(0, None, None)], None, None
+        cos = (None, None)
     else:
         t0 = time.perf_counter()
-        x, y, scales = make_samples(TRAIN_SMALL, range(1, 1 + a.train_models), log)
+        x, y, g, scales = make_samples(TRAIN_SMALL, range(1, 1 + a.train_models), log)
         t_gen = time.perf_counter() - t0
         t0 = time.perf_counter()
-        params, th, _ = train(x, y, epochs=a.epochs, lr=a.lr, log=log)
+        params, th, val = train(x, y, epochs=a.epochs, lr=a.lr, log=log)
         t_train = time.perf_counter() - t0
+        cos = holdout_cosines(params, x, y, g, val)
+        log({"holdout_cosine_learned": cos[0], "holdout_cosine_plain": cos[1]})
     params.tofile(os.path.join(a.out, "preconditioner_params.bin"))
 
     s = COMPARE_SMALL
@@ -357,13 +383,14 @@ def main(argv=None):
     t_plain = runs["plain"][-1]["time"]
     summary = {}
     for name, h in runs.items():
-        t = time_to_misfit(h, target) if h[-1]["band"] == runs["plain"][-1]["band"] else None
+        t = time_to_misfit(h, target, runs["plain"][-1]["band"])
         summary[name] = {"final_misfit": h[-1]["misfit"], "time_s": h[-1]["time"],
                          "final_model_rms_error": h[-1]["model_rms_error"],
                          "time_to_plain_final_misfit_s": t,
                          "time_ratio": (t / t_plain) if t else None,
                          "trial_steps_taken": sum(e["taken"] == "learned" for e in h)}
     report = {"samples": int(len(x)), "generation_s": t_gen, "training_s": t_train,
+              "holdout_cosine": {"learned": cos[0], "plain_direction": cos[1]},
               "train_loss_first_last": [th[0][1], th[-1][1]],
               "holdout_loss_first_last": [th[0][2], th[-1][2]],
               "plain_final_misfit": target, "plain_time_s": t_plain,

@@ -33,10 +33,14 @@ def test_normalise_inputs_per_channel():
 
 
 def test_time_to_misfit_and_cosine():
-    hist = [{"misfit": 5.0, "time": 1.0}, {"misfit": 3.0, "time": 2.0},
-            {"misfit": 2.5, "time": 3.0}]
-    assert L.time_to_misfit(hist, 3.0) == 2.0
-    assert L.time_to_misfit(hist, 1.0) is None
+    hist = [{"band": 0, "misfit": 1.0, "time": 1.0, "taken": "plain"},
+            {"band": 1, "misfit": 5.0, "time": 2.0, "taken": "plain"},
+            {"band": 1, "misfit": 3.0, "time": 3.0, "taken": "plain"},
+            {"band": 1, "misfit": 2.5, "time": 3.5, "taken": "end"}]
+    assert L.time_to_misfit(hist, 3.0, 1) == 3.0
+    assert L.time_to_misfit(hist, 2.5, 1) == 3.5
+    assert L.time_to_misfit(hist, 1.0, 1) is None      # band 0's misfit is not compared
+    assert L.time_to_misfit(hist, 1.0, 0) == 1.0
     a = np.array([1.0, 2.0, 3.0])
     assert abs(L.cosine(a, 2 * a) - 1.0) < 1e-12
     assert abs(L.cosine(a, -a) + 1.0) < 1e-12
