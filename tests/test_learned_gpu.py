@@ -1,0 +1,73 @@
+"""GPU tests of the learned-preconditioner FWI (fwi2d_amd.learned):
+
+* a network with zero weights gives the control direction -g, so the
+  learned run equals the control run step for step;
+* an ascent direction (+g) never passes the acceptance rule, so the run
+  equals plain FWI step for step ("never worse than the plain step");
+* the trained network (tests/golden/preconditioner_48models.bin, made by
+  tests/golden/make_preconditioner.py) gains over plain FWI and over the
+  control on a held-out model of the comparison family.
+"""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+from fwi2d_amd import abi as A
+from fwi2d_amd import configs as C
+from fwi2d_amd import learned as L
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SMALL = dataclasses.replace(L.TRAIN_SMALL, nx=256, nsrc=12, nrec=128, seed=9,
+                            band_iters=(2, 2, 2))
+
+
+def survey(seed):
+    v, v0 = C.config5_models(seed, SMALL.nz, SMALL.nx)
+    return L.Fwi(SMALL, v, v0)
+
+
+def trace(hist):
+    return [(h["misfit"], h["step"], h["taken"]) for h in hist]
+
+
+def test_zero_network_equals_control():
+    fwi = survey(20001)
+    n = A.Net(1, SMALL.nx, SMALL.nz, 0).nparams
+    d = L.Direction(SMALL.nx, SMALL.nz, np.zeros(n, np.float32))
+    a, ma = L.run_fwi(fwi, d, band_iters=(1, 1, 1))
+    b, mb = L.run_fwi(fwi, L.plain_direction, band_iters=(1, 1, 1))
+    d.close()
+    fwi.close()
+    assert trace(a) == trace(b)
+    assert np.array_equal(ma, mb)
+
+
+def test_ascent_direction_never_beats_the_plain_step():
+    fwi = survey(20002)
+    up, _ = L.run_fwi(fwi, lambda raw, ill, m, g: g, band_iters=(1, 1, 1))
+    plain, _ = L.run_fwi(fwi, None, band_iters=(1, 1, 1))
+    fwi.close()
+    assert all(h["taken"] in ("plain", "end") for h in up)
+    assert [h["misfit"] for h in up] == [h["misfit"] for h in plain]
+
+
+def test_trained_network_gains_over_plain_and_control():
+    params = np.fromfile(os.path.join(GOLDEN, "preconditioner_48models.bin"), np.float32)
+    fwi = survey(20003)
+    d = L.Direction(SMALL.nx, SMALL.nz, params)
+    runs = {"plain": L.run_fwi(fwi, None)[0], "control": L.run_fwi(fwi, L.plain_direction)[0],
+            "learned": L.run_fwi(fwi, d)[0]}
+    d.close()
+    fwi.close()
+    for k, h in runs.items():
+        print(f"{k:8s} final misfit {h[-1]['misfit']:.4e}  model rms error "
+              f"{h[-1]['model_rms_error']:.1f} m/s  time {h[-1]['time']:.2f} s  "
+              f"trial steps {sum(e['taken'] == 'learned' for e in h)}")
+    f = {k: h[-1]["misfit"] for k, h in runs.items()}
+    e = {k: h[-1]["model_rms_error"] for k, h in runs.items()}
+    assert f["learned"] < f["plain"] and e["learned"] < e["plain"]
+    assert f["learned"] < f["control"]
