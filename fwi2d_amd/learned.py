@@ -22,7 +22,7 @@ Three tools:
   step, then tries the direction d at two multiples of the plain step length;
   the better trial is taken only when its forward-modeled misfit is below the
   misfit the plain line search predicts for its own step, otherwise the plain
-  step is taken.  ``plain_direction`` (d = -g) is the control that isolates
+  step is taken.  Trial directions get the gradient's depth mask.  ``plain_direction`` (d = -g) is the control that isolates
   what the network adds to the longer trial steps.
 
 Multiscale bands rebuild the wavelet and re-model the observed data per band
@@ -222,6 +222,12 @@ def run_fwi(fwi, direction=None, multiples=(2.0, 4.0), band_iters=None, log=None
         new = m - np.float32(step) * g
         if direction is not None and step > 0.0:
             d = direction(raw, ill, m, g)
+            # the gradient's depth mask (rows izm < i < nz-1 only) applies to
+            # any trial direction: the shallow zone stays as the plain FWI keeps it
+            izm = int(np.float32(s.zmask) / np.float32(s.dh))
+            d = d.copy()
+            d[:, :izm + 1] = 0.0
+            d[:, s.nz - 1:] = 0.0
             best = None
             for k in multiples:
                 trial = m + np.float32(k * step) * d
@@ -278,9 +284,11 @@ def make_samples(setup, seeds, log=None):
     return np.stack(xs), np.stack(ys), np.stack(gs), np.array(scales)
 
 
-def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, log=None):
+def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, log=None):
     """Adam on the GPU; returns (params, history of (epoch, train loss,
-    held-out loss))."""
+    held-out loss), held-out indices).  With flip, each training sample is
+    mirrored along x with probability 1/2 (the wave equation is symmetric
+    under x -> -x, so a mirrored sample is a valid sample)."""
     n, _, h, w = x.shape
     rng = np.random.default_rng(seed)
     order = rng.permutation(n)
@@ -293,7 +301,13 @@ def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, log=None):
         losses = []
         for b0 in range(0, len(tr) - batch + 1, batch):
             idx = tr[b0:b0 + batch]
-            losses.append(net.train_step(x[idx], y[idx], lr))
+            xb, yb = x[idx], y[idx]
+            if flip:
+                f = rng.random(len(idx)) < 0.5
+                xb, yb = xb.copy(), yb.copy()
+                xb[f] = xb[f][:, :, ::-1, :]
+                yb[f] = yb[f][:, ::-1, :]
+            losses.append(net.train_step(xb, yb, lr))
         vl = []
         for b0 in range(0, nval, batch):
             idx = val[b0:b0 + batch]
