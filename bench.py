@@ -54,6 +54,9 @@ def parse():
     p.add_argument("--graph", type=int, default=-1)
     p.add_argument("--no-traffic", action="store_true",
                    help="skip the rocprofv3 counter passes (roofline.traffic = null)")
+    p.add_argument("--workload", default="rtm64",
+                   choices=("rtm64", "forward1", "fwi128", "grad256"),
+                   help="rtm64 is the headline; the others show the rate's dependence on shape")
     p.add_argument("--pmc-child", action="store_true",
                    help="internal: the short run profiled by the counter passes")
     return p.parse_args()
@@ -95,6 +98,102 @@ def cpu_baseline(s, vmig, xs, zs, xr, zr, w, tlag, obs, threads, nshots):
             "sample": f"{nshots} of {s.nsrc} shots, full RTM each (RB forward + reverse "
                       f"imaging), {nzz}x{nxx} cells x {T} steps, FP32 OpenMP, "
                       f"{el:.1f} s of CPU time"}
+
+
+def roof(ms, n, cells, bpc, copy_gbs):
+    """Roofline record of one kind of step launch from the engine's event
+    profile: ms over n launches that advanced `cells` cell-updates."""
+    avg_s = ms / n * 1e-3
+    per_launch = cells / n
+    gbs = per_launch * bpc / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": per_launch * bpc,
+            "kernel_avg_us": round(avg_s * 1e6, 2),
+            "gcell_per_s": round(per_launch / avg_s / 1e9, 2),
+            "frac_of_measured_copy": round(gbs / copy_gbs, 4)}
+
+
+def other_workload(a):
+    """The other benchmark configurations, for the sensitivity of the rate to
+    the problem shape (same JSON keys; not the headline):
+      forward1  1 shot, 121x401 + 100-cell random boundary, nt 2001, 1 ms,
+                15 Hz peak, 401 receivers (the field fits in the caches)
+      fwi128    one plain FWI iteration (gradient pass + two line-search
+                misfits), 128 shots, 301x1001 + 150 (random) / 100 (sponge),
+                nt 4001, 0.8 ms, 8 Hz peak, 1001 receivers
+      grad256   one FWI gradient pass, 256 shots, 401x1601 + 200 (random) /
+                100 (sponge), nt 5001, 0.6 ms, 12 Hz peak, 1601 receivers"""
+    import dataclasses
+    import fwi2d_amd as F
+    from fwi2d_amd import configs as C
+    copy_gbs = F.abi.stream_copy(1 << 30, reps=20)
+    if a.workload == "forward1":
+        s = C.CONFIG1
+        true_v = C.config1_model()
+        init_v = None
+        s = dataclasses.replace(s, seed=2024)
+    elif a.workload == "fwi128":
+        s = C.CONFIG3
+        true_v, init_v = C.config3_models()
+    else:
+        s = C.CONFIG5
+        true_v, init_v = C.config5_models(5, s.nz, s.nx)
+    xs, zs, xr, zr = s.acquisition()
+    w, tlag = F.ricker_wavelet(s.nt, s.dt, s.fmax)
+    T = s.nt + tlag
+    eng = F.Engine(s.nz, s.nx, s.dh, s.nt, s.dt, abc_nb=0 if init_v is None else 100,
+                   abc_factor=0.0 if init_v is None else 0.0035, rbc_nb=s.nb,
+                   rbc_varVp=s.varVp, rbc_varVp_rel=s.varVp_rel, rbc_seed=s.seed,
+                   max_batch=s.nsrc)
+    eng.set_wavelet(w, tlag)
+    eng.set_geometry(xs, zs, xr, zr)
+    eng.set_model(true_v)
+    nzz, nxx, _ = eng.grid_shape(F.RBC)
+    if init_v is not None:
+        eng.set_observed(eng.modeling(F.ABC))
+        eng.set_model(init_v)
+        azz, axx, _ = eng.grid_shape(F.ABC)
+    if not a.no_autotune:
+        eng.autotune(s.nsrc, reps=8)
+
+    def step():
+        if a.workload == "forward1":
+            eng.modeling(F.RBC)
+            return s.nsrc * nzz * nxx * T
+        eng.reset(F.RESET_GRADIENT)
+        eng.gradient(iteration=1)
+        cells = s.nsrc * (azz * axx + nzz * nxx * 3.0) * T
+        if a.workload == "fwi128":
+            for scale in (0.99, 0.98):               # the two line-search trial models
+                eng.set_model((init_v * scale).astype(np.float32))
+                eng.misfit()
+            eng.set_model(init_v)
+            cells += 2.0 * s.nsrc * azz * axx * T
+        return cells
+
+    for _ in range(a.warmup):
+        step()
+    eng.synchronize()
+    eng.profile_enable(True)
+    t0 = time.perf_counter()
+    cells = sum(step() for _ in range(a.steps))
+    eng.synchronize()
+    el = time.perf_counter() - t0
+    fwd = eng.profile_read(0)
+    rev = eng.profile_read(1)
+    eng.profile_enable(False)
+    eng.close()
+    out = {"metric": "Gcell-updates/s", "value": round(cells / el / 1e9, 2),
+           "unit": "Gcell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": round(el / a.steps * 1e3, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": a.workload, "shots": s.nsrc, "nzz": nzz, "nxx": nxx,
+                      "steps_per_pass": T, "measured_copy_GBs": round(copy_gbs, 1)},
+           "roofline_forward": roof(*fwd, BYTES_FWD, copy_gbs)}
+    if rev[1]:
+        out["roofline"] = roof(*rev, BYTES_REV, copy_gbs)
+    print(json.dumps(out), flush=True)
 
 
 def kernel_key(kind, steps, vz):
@@ -217,6 +316,8 @@ def main():
         sys.exit(2)
     if a.pmc_child:
         return pmc_child(a)
+    if a.workload != "rtm64":
+        return other_workload(a)
     import fwi2d_amd as F
     from fwi2d_amd import configs as C
 
@@ -279,19 +380,8 @@ def main():
     cells_step = nshots * nzz * nxx * T * 3.0
     value = cells_step * a.steps / el / 1e9
 
-    def roof(ms, n, cells, bpc):
-        avg_s = ms / n * 1e-3
-        per_launch = cells / n
-        gbs = per_launch * bpc / avg_s / 1e9
-        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes_per_launch": per_launch * bpc,
-                "kernel_avg_us": round(avg_s * 1e6, 2),
-                "gcell_per_s": round(per_launch / avg_s / 1e9, 2),
-                "frac_of_measured_copy": round(gbs / copy_gbs, 4)}
-
-    rev_roof = roof(rev_ms, rev_n, rev_cells, BYTES_REV)
-    fwd_roof = roof(fwd_ms, fwd_n, fwd_cells, BYTES_FWD)
+    rev_roof = roof(rev_ms, rev_n, rev_cells, BYTES_REV, copy_gbs)
+    fwd_roof = roof(fwd_ms, fwd_n, fwd_cells, BYTES_FWD, copy_gbs)
     rev_roof["kernel"] = "rev_step_kernel (fused reverse + imaging)"
     fwd_roof["kernel"] = "fwd_step_kernel (random boundary)"
     stf, vzf, xwf, g = eng.get_tuning(0)
