@@ -940,16 +940,32 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
     for (int k = 0; k < 8; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
     TRY(set_shot0(c, 0));
     const int vzs[3] = {1, 2, 4};
-    const int xws[5] = {16, 32, 64, 96, 128};
+    const long cells = (long)nslots * gb.g.nzz * gb.g.nxx;
+    const int ncols = gb.g.nxx - 8;
     double best[2] = {1e30, 1e30};
     for (int kind = 0; kind < 2; ++kind) {
         int bst = c->nsteps[kind], bvz = c->vz[kind], bxw = c->xw[kind];
         for (int st = 1; st <= 2; ++st)
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 5; ++b) {
+            for (int a = 0; a < 3; ++a) {
+                // fixed widths, narrow ones for small batches (more waves in
+                // flight), and widths that make the launch about 1-4 full
+                // rounds of the waves the device holds at once
+                std::vector<int> xws = {16, 32, 64, 96, 128};
+                if (cells < (16L << 20)) { xws.push_back(4); xws.push_back(8); }
+                const int tiles_per_chunk = fwi::zsegments(gb.g.nzz, vzs[a], st) * nslots;
+                const int resident = fwi::resident_waves(kind, st, vzs[a]);
+                for (int k = 1; k <= 4 && resident > 0; ++k) {
+                    const int nchunk = (k * resident) / tiles_per_chunk;
+                    if (nchunk < 1) continue;
+                    const int xw = (ncols + nchunk - 1) / nchunk;
+                    if (xw >= 4 && xw <= 256 &&
+                        std::find(xws.begin(), xws.end(), xw) == xws.end())
+                        xws.push_back(xw);
+                }
+                for (int xwv : xws) {
                     c->nsteps[kind] = st;
                     c->vz[kind] = vzs[a];
-                    c->xw[kind] = xws[b];
+                    c->xw[kind] = xwv;
                     const fwi::Tiling tl = tiling_of(c, gb.g, kind);
                     fwi::FwdStep fa = fwd_args(c, FWI_BOUNDARY_RBC, false);
                     fa.cur = c->d_f[1]; fa.old = c->d_f[0];
@@ -975,9 +991,10 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
                         best[kind] = us_per_step;
                         bst = st;
                         bvz = vzs[a];
-                        bxw = xws[b];
+                        bxw = xwv;
                     }
                 }
+            }
         c->nsteps[kind] = bst;
         c->vz[kind] = bvz;
         c->xw[kind] = bxw;

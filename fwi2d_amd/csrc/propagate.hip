@@ -1245,6 +1245,37 @@ hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k
     }
 }
 
+namespace {
+template <class K>
+int resident_of(K kernel)
+{
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return per_cu * cus;
+}
+
+template <int VZ>
+int resident_vz(int kind, int steps)
+{
+    if (kind == 0)
+        return steps == 2 ? resident_of(fwd2_kernel<VZ, false>) : resident_of(fwd1_kernel<VZ, false>);
+    return steps == 2 ? resident_of(rev2_kernel<VZ, 0>) : resident_of(rev1_kernel<VZ, 0>);
+}
+}  // namespace
+
+int resident_waves(int kind, int steps, int vz)
+{
+    switch (vz) {
+    case 1: return resident_vz<1>(kind, steps);
+    case 2: return resident_vz<2>(kind, steps);
+    case 4: return resident_vz<4>(kind, steps);
+    default: return 0;
+    }
+}
+
 hipError_t launch_fwd_init(const FwdStep &a, float *field, int nslots, hipStream_t s)
 {
     hipLaunchKernelGGL(fwd_init_kernel, dim3((nslots + 63) / 64), dim3(64), 0, s, a, field,

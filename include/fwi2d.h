@@ -179,7 +179,11 @@ int fwi_get_tuning(fwi_ctx *ctx, int32_t kernel, int32_t *steps, int32_t *vz, in
 
 /* Autotuner: times every candidate launch shape of the forward and reverse
  * kernels on the random-boundary grid with nslots resident shots (`reps`
- * launches each) and keeps the fastest; returns the time per step (us). */
+ * launches each) and keeps the fastest; returns the time per step (us).
+ * Candidates: 1 or 2 steps per launch x 1, 2 or 4 rows per lane x chunk
+ * widths 16-128 columns, 4 and 8 for small batches, and the widths that make
+ * a launch 1-4 full rounds of the waves the device holds at once.  No
+ * reference counterpart (the reference launches one fixed shape). */
 int fwi_autotune(fwi_ctx *ctx, int32_t nslots, int32_t reps, double *fwd_us, double *rev_us);
 
 /* Measured device bandwidth (GB/s, read + write) of a plain float4 copy of
