@@ -72,6 +72,8 @@ SIGNATURES = [
     ("fwi_guarded_step", C.c_int, [_f64p, _f64p, _f32p, _f32p, C.c_int64, C.c_double,
                                    C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     ("fwi_rb_velocity", C.c_int, [_ctx, C.c_int32, C.c_uint32, _f32p]),
+    ("fwi_get_accumulator", C.c_int, [_ctx, C.c_int32, _f32p]),
+    ("fwi_set_accumulator", C.c_int, [_ctx, C.c_int32, _f32p]),
     ("fwi_grid_shape", C.c_int, [_ctx, C.c_int32, C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("fwi_synchronize", C.c_int, [_ctx]),
@@ -252,6 +254,16 @@ class Engine:
         gm = C.c_double(0.0)
         _check(lib().fwi_get_gradient(self._h, zmask, _p(g), _p(raw), _p(ill), C.byref(gm)))
         return g, raw, ill, gm.value
+
+    def get_accumulator(self, what):
+        """Shot-sum image, gradient or illumination (what = RESET_*), [nxx, nzz]."""
+        nzz, nxx, _ = self.grid_shape(RBC)
+        out = np.zeros((nxx, nzz), np.float32)
+        _check(lib().fwi_get_accumulator(self._h, what, out))
+        return out
+
+    def set_accumulator(self, what, values):
+        _check(lib().fwi_set_accumulator(self._h, what, np.ascontiguousarray(values, np.float32)))
 
     def rb_velocity(self, shot, iteration=0):
         nzz, nxx, _ = self.grid_shape(RBC)

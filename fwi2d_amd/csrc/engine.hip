@@ -809,6 +809,41 @@ int fwi_parabola_step(const double *alpha, const double *f, double *step)
     return FWI_OK;
 }
 
+static int accumulator(fwi_ctx *c, int32_t what, float **dev)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    if (!c->grid[1].on) return fail(FWI_ERR_ARG, "random-boundary grid not configured");
+    if (what == FWI_RESET_IMAGE) *dev = c->d_image;
+    else if (what == FWI_RESET_GRADIENT) *dev = c->d_grad;
+    else if (what == FWI_RESET_ILLUM) *dev = c->d_illum;
+    else return fail(FWI_ERR_ARG, "what must be one of image, gradient, illumination");
+    return FWI_OK;
+}
+
+int fwi_get_accumulator(fwi_ctx *c, int32_t what, float *out)
+{
+    float *dev = nullptr;
+    TRY(accumulator(c, what, &dev));
+    if (!out) return fail(FWI_ERR_ARG, "null output");
+    const fwi::Grid &g = c->grid[1].g;
+    HIP_TRY(hipMemcpy2DAsync(out, (size_t)g.nzz * 4, dev, (size_t)g.ldz * 4, (size_t)g.nzz * 4,
+                             g.nxx, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FWI_OK;
+}
+
+int fwi_set_accumulator(fwi_ctx *c, int32_t what, const float *in)
+{
+    float *dev = nullptr;
+    TRY(accumulator(c, what, &dev));
+    if (!in) return fail(FWI_ERR_ARG, "null input");
+    const fwi::Grid &g = c->grid[1].g;
+    HIP_TRY(hipMemcpy2DAsync(dev, (size_t)g.ldz * 4, in, (size_t)g.nzz * 4, (size_t)g.nzz * 4,
+                             g.nxx, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FWI_OK;
+}
+
 int fwi_guarded_step(const double *alpha, const double *f, const float *m, const float *g,
                      int64_t n, double m_min, double *step, int32_t *rule)
 {

@@ -1,6 +1,8 @@
 // admin.cpp -- see admin.hpp.
 #include "admin.hpp"
 
+#include <cctype>
+
 #include <algorithm>
 #include <fstream>
 #include <iostream>
@@ -86,6 +88,16 @@ uint64_t param_u64(const std::string &key, const std::string &file, uint64_t fal
 {
     std::string v = catch_parameter(key, file, "");
     return v.empty() ? fallback : (uint64_t)std::stoull(v, nullptr, 0);
+}
+
+bool param_bool(const std::string &key, const std::string &file, bool fallback)
+{
+    std::string v = catch_parameter(key, file, "");
+    if (v.empty()) return fallback;
+    for (char &ch : v) ch = (char)std::tolower((unsigned char)ch);
+    if (v == "true") return true;
+    if (v == "false") return false;
+    throw std::runtime_error("Error: " + key + " must be true or false, not " + v);
 }
 
 void import_binary_float(const std::string &path, float *array, long n)

@@ -22,6 +22,9 @@ float param_float(const std::string &key, const std::string &file);
 int param_int(const std::string &key, const std::string &file, int fallback);
 float param_float(const std::string &key, const std::string &file, float fallback);
 uint64_t param_u64(const std::string &key, const std::string &file, uint64_t fallback);
+// "true" / "false" in any letter case (src/admin/admin.cpp:3-11); any other
+// value is an error.
+bool param_bool(const std::string &key, const std::string &file, bool fallback);
 
 // Raw little-endian float32 arrays (admin.cpp:13-37).  import throws when
 // the file is missing or shorter than n values.

@@ -23,6 +23,7 @@ int main(int argc, char **argv)
             if (inversion.converged) break;
             inversion.optimization();
             inversion.update_model();
+            inversion.save_checkpoint();
         }
         inversion.export_convergence();
         inversion.export_final_model();

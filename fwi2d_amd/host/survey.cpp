@@ -2,6 +2,8 @@
 #include "survey.hpp"
 
 #include <limits>
+#include <cstdio>
+#include <cstring>
 
 #include <algorithm>
 #include <cmath>
@@ -183,6 +185,10 @@ void Inversion::set_parameters()
     input_prefix = catch_parameter("inv_input_prefix", parameters);
     output_folder = catch_parameter("inv_output_folder", parameters, "");
     residuo_folder = catch_parameter("inv_residuo_folder", parameters, "");
+    checkpoint_file = catch_parameter("checkpoint_file", parameters, "");
+    resume = param_bool("resume", parameters, false);
+    if (resume && checkpoint_file.empty())
+        throw std::runtime_error("Error: resume = true needs a checkpoint_file");
     set_wavelet();
     set_properties();
     vmin0 = vmin;
@@ -201,6 +207,72 @@ void Inversion::set_parameters()
     gradient.assign(vp.size(), 0.f);
     // the illumination accumulates over every gradient pass (inversion.cu:51, 441)
     check(fwi_reset(ctx, FWI_RESET_GRADIENT | FWI_RESET_ILLUM), "reset");
+    if (resume && load_checkpoint())
+        std::cout << "resumed from " << checkpoint_file << " at iteration " << iteration << "\n";
+}
+
+namespace {
+const char kCheckpointMagic[8] = {'F', 'W', 'I', '2', 'D', 'C', 'K', '1'};
+}
+
+void Inversion::save_checkpoint() const
+{
+    if (checkpoint_file.empty()) return;
+    int32_t nzz = 0, nxx = 0, nb = 0;
+    check(fwi_grid_shape(ctx, FWI_BOUNDARY_RBC, &nzz, &nxx, &nb), "grid_shape");
+    std::vector<float> illum((size_t)nzz * nxx);
+    check(fwi_get_accumulator(ctx, FWI_RESET_ILLUM, illum.data()), "get illumination");
+    const std::string tmp = checkpoint_file + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+        if (!f.is_open()) throw std::runtime_error("Error: " + tmp + " could not be opened!");
+        const int32_t head[6] = {nz, nx, nzz, nxx, iteration, (int32_t)residuo.size()};
+        f.write(kCheckpointMagic, 8);
+        f.write(reinterpret_cast<const char *>(head), sizeof head);
+        f.write(reinterpret_cast<const char *>(&vmin0), 4);
+        f.write(reinterpret_cast<const char *>(&vmax0), 4);
+        f.write(reinterpret_cast<const char *>(residuo.data()), residuo.size() * 8);
+        f.write(reinterpret_cast<const char *>(model.data()), model.size() * 4);
+        f.write(reinterpret_cast<const char *>(illum.data()), illum.size() * 4);
+        if (!f) throw std::runtime_error("Error: writing " + tmp + " failed");
+    }
+    if (std::rename(tmp.c_str(), checkpoint_file.c_str()) != 0)
+        throw std::runtime_error("Error: could not replace " + checkpoint_file);
+}
+
+bool Inversion::load_checkpoint()
+{
+    std::ifstream f(checkpoint_file, std::ios::binary);
+    if (!f.is_open()) return false;               // nothing to resume: start from scratch
+    char magic[8];
+    int32_t head[6];
+    f.read(magic, 8);
+    f.read(reinterpret_cast<char *>(head), sizeof head);
+    if (!f || std::memcmp(magic, kCheckpointMagic, 8) != 0)
+        throw std::runtime_error("Error: " + checkpoint_file + " is not a checkpoint");
+    int32_t nzz = 0, nxx = 0, nb = 0;
+    check(fwi_grid_shape(ctx, FWI_BOUNDARY_RBC, &nzz, &nxx, &nb), "grid_shape");
+    if (head[0] != nz || head[1] != nx || head[2] != nzz || head[3] != nxx)
+        throw std::runtime_error("Error: " + checkpoint_file + " belongs to another grid");
+    if (head[4] < 0 || head[5] != head[4])
+        throw std::runtime_error("Error: " + checkpoint_file + " is inconsistent");
+    float v0[2];
+    f.read(reinterpret_cast<char *>(v0), 8);
+    std::vector<double> res((size_t)head[5]);
+    f.read(reinterpret_cast<char *>(res.data()), res.size() * 8);
+    std::vector<float> m(model.size()), illum((size_t)nzz * nxx);
+    f.read(reinterpret_cast<char *>(m.data()), m.size() * 4);
+    f.read(reinterpret_cast<char *>(illum.data()), illum.size() * 4);
+    if (!f) throw std::runtime_error("Error: " + checkpoint_file + " is truncated");
+    vmin0 = v0[0];
+    vmax0 = v0[1];
+    residuo = res;
+    iteration = head[4];
+    model = m;
+    for (size_t k = 0; k < model.size(); ++k) vp[k] = 1.0f / std::sqrt(model[k]);
+    check(fwi_set_model(ctx, vp.data()), "set_model (resume)");
+    check(fwi_set_accumulator(ctx, FWI_RESET_ILLUM, illum.data()), "set illumination");
+    return true;
 }
 
 void Inversion::compute_gradient()

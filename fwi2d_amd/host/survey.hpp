@@ -67,6 +67,13 @@ public:
     void update_model();            // inversion.cu:344-358
     void export_convergence();      // inversion.cu:360-372
     void export_final_model();      // inversion.cu:374-379
+    // Checkpoint / resume (no reference counterpart): after every model
+    // update the state that the next iteration needs -- iteration, residual
+    // history, model, the initial model's vmin / vmax and the accumulated
+    // illumination -- goes to `checkpoint_file`; with `resume = true` an
+    // existing checkpoint is loaded before the first iteration.
+    void save_checkpoint() const;
+    bool load_checkpoint();
 
     int iteration = 0;
     int max_iteration = 0;
@@ -81,6 +88,8 @@ private:
     double sum_res = 0, step = 0;
     float vmin0 = 0, vmax0 = 0;     // extremes of the initial model (dm of the line search)
     std::string input_folder, input_prefix, output_folder, residuo_folder;
+    std::string checkpoint_file;
+    bool resume = false;
     double linesearch(float alpha);  // inversion.cu:310-342
     double min_slowness2() const;
 };

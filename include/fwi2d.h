@@ -157,6 +157,15 @@ int fwi_parabola_step(const double *alpha, const double *f, double *step);
 int fwi_guarded_step(const double *alpha, const double *f, const float *m, const float *g,
                      int64_t n, double m_min, double *step, int32_t *rule);
 
+/* Shot-sum accumulators on the random-boundary grid as [nxx][nzz] (z
+ * fastest, pads included): what = FWI_RESET_IMAGE, FWI_RESET_GRADIENT or
+ * FWI_RESET_ILLUM (one of them).  For checkpoint / resume of an inversion,
+ * whose illumination accumulates over every gradient pass
+ * (inversion.cu:51,441).  No reference counterpart: the reference cannot
+ * resume. */
+int fwi_get_accumulator(fwi_ctx *ctx, int32_t what, float *out);
+int fwi_set_accumulator(fwi_ctx *ctx, int32_t what, const float *in);
+
 /* Test hook: the random-boundary velocity [nxx][nzz] (rbc grid) that a pass
  * of shot `shot` in FWI iteration `iteration` uses (migration: iteration 0). */
 int fwi_rb_velocity(fwi_ctx *ctx, int32_t shot, uint32_t iteration, float *vp_padded_out);

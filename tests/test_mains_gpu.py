@@ -115,3 +115,37 @@ def test_three_mains_end_to_end(tmp_path):
     assert e1 < e0
     assert "residuo" in out
     eng.close()
+
+
+def test_inversion_resume_equals_straight_run(tmp_path):
+    """Three iterations in one run equal two iterations, a checkpoint and a
+    resumed third iteration, bit for bit (model and residual history)."""
+    tmp = str(tmp_path)
+    _, _, _, _, folders, prefix, params = setup_files(tmp)
+    run("modeling", params["mod"])
+    init_file = os.path.join(tmp, "init.bin")
+    sps, rps = os.path.join(tmp, "sps.txt"), os.path.join(tmp, "rps.txt")
+
+    def inv_params(name, iters, extra):
+        f = C.ensure_dirs(os.path.join(tmp, name))
+        f["data"] = folders["data"]
+        p = os.path.join(tmp, f"parameters_{name}_{iters}.txt")
+        C.write_parameters(p, SURVEY, init_file, sps, rps, f, max_iteration=iters,
+                           depth_mask=100.0, inv_prefix=prefix, extra=extra)
+        return p, f
+
+    ck = os.path.join(tmp, "fwi.ckpt")
+    pa, fa = inv_params("straight", 3, {"checkpoint_file": os.path.join(tmp, "straight.ckpt")})
+    pb1, fb = inv_params("split", 2, {"checkpoint_file": ck})
+    pb2, _ = inv_params("split", 3, {"checkpoint_file": ck, "resume": "True"})
+    run("inversion", pa)
+    run("inversion", pb1)
+    out = run("inversion", pb2)
+    assert "resumed from" in out
+    suffix = f"{int(FMAX)}Hz_{SURVEY.nz}x{SURVEY.nx}_{int(SURVEY.dh)}m.bin"
+    a = np.fromfile(os.path.join(fa["models"], f"final_model_FWI_{suffix}"), np.float32)
+    b = np.fromfile(os.path.join(fb["models"], f"final_model_FWI_{suffix}"), np.float32)
+    assert np.array_equal(a, b)
+    ra = open(os.path.join(fa["residuo"], "convergence_3_iterations.txt")).read()
+    rb = open(os.path.join(fb["residuo"], "convergence_3_iterations.txt")).read()
+    assert ra == rb
