@@ -167,9 +167,11 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
 }
 
 // Laplacian (times dh^2) from a 9-column register queue whose column j-4+m
-// sits in slot (m + OFF) % 9, and the z extension of the centre column,
-// summed as centre differences; the weight sum is one explicit FMA chain so
-// that every kernel rounds it identically.
+// sits in slot (m + OFF) % 9, and the z extension of the centre column.  The
+// four nearest neighbours (weight 1.6) enter as differences from the centre,
+// which keeps the FP32 cancellation against the centre term small; the
+// farther rings (weights 0.2 and below) as sums minus 4u.  The weight sum is
+// one explicit FMA chain so that every kernel rounds it identically.
 template <int VZ, int OFF>
 __device__ __forceinline__ float lap8r(const float (&q)[9][VZ], const float (&ext)[VZ + 8],
                                        int r)
@@ -179,9 +181,9 @@ __device__ __forceinline__ float lap8r(const float (&q)[9][VZ], const float (&ex
                   P6 = (6 + OFF) % 9, P7 = (7 + OFF) % 9, P8 = (8 + OFF) % 9;
     const float p = q[P4][r];
     const float s1 = ((q[P3][r] - p) + (q[P5][r] - p)) + ((ext[r + 3] - p) + (ext[r + 5] - p));
-    const float s2 = ((q[P2][r] - p) + (q[P6][r] - p)) + ((ext[r + 2] - p) + (ext[r + 6] - p));
-    const float s3 = ((q[P1][r] - p) + (q[P7][r] - p)) + ((ext[r + 1] - p) + (ext[r + 7] - p));
-    const float s4 = ((q[P0][r] - p) + (q[P8][r] - p)) + ((ext[r + 0] - p) + (ext[r + 8] - p));
+    const float s2 = fmaf(-4.f, p, (q[P2][r] + q[P6][r]) + (ext[r + 2] + ext[r + 6]));
+    const float s3 = fmaf(-4.f, p, (q[P1][r] + q[P7][r]) + (ext[r + 1] + ext[r + 7]));
+    const float s4 = fmaf(-4.f, p, (q[P0][r] + q[P8][r]) + (ext[r + 0] + ext[r + 8]));
     return fmaf(kC4, s1, fmaf(-kC3, s2, fmaf(kC2, s3, -kC1 * s4)));
 }
 
