@@ -590,11 +590,17 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
     ldcol<VZ>(cn, cc, x0 - 4, nxx, ldz, zb);
     vzero<VZ>(pn);
     float4 h = ldhalo<VZ>(cur, x0 - 4, 4, nxx - 4, ldz, zs, lane);
+    // c of the last five columns, each lane its own rows: step 2 at column
+    // j - 4 reads what step 1 parked at column j - 4 instead of HBM
+    __shared__ float cring[5][64 * VZ];
+    int slot = 0;
 
     for (int j = x0 - 4; j < x1 + 4; ++j) {
         float o1[VZ], cj[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
+        vstore<VZ>(&cring[slot][lane * VZ], cj);
+        slot = (slot == 4) ? 0 : slot + 1;          // = the slot of column j - 4
         if (j + 1 < x1 + 4) {
             ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zb);
             ldcol<VZ>(on, old, j + 1, nxx, ldz, zb);
@@ -648,8 +654,8 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
                 bool in;
                 dxj = staged ? st.dx[jj - xa] : pad_factor(a.b1d, jj, nxx, g.nb, in);
             }
-            float c2[VZ];                               // c at column jj (read 4 columns ago)
-            ldcol<VZ>(c2, cc, jj, nxx, ldz, zb);
+            float c2[VZ];                               // c at column jj (parked 4 columns ago)
+            vload<VZ>(c2, &cring[slot][lane * VZ]);
             float ext[VZ + 8], dv[VZ];
             zext_local<VZ>(ext, q2[4]);
             float n2[VZ];
@@ -847,11 +853,15 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
     vzero<VZ>(psn); vzero<VZ>(prn);
     float4 hs = ldhalo<VZ>(scur, x0 - 4, 4, nxx - 4, ldz, zs, lane);
     float4 hr = ldhalo<VZ>(rcur, x0 - 4, 4, nxx - 4, ldz, zs, lane);
+    __shared__ float cring[5][64 * VZ];              // c of the last five columns
+    int slot = 0;
 
     for (int j = x0 - 4; j < x1 + 4; ++j) {
         float os[VZ], orr[VZ], cj[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r]; }
+        vstore<VZ>(&cring[slot][lane * VZ], cj);
+        slot = (slot == 4) ? 0 : slot + 1;          // = the slot of column j - 4
         const int jj = j - 4;
         if (j + 1 < x1 + 4) {
             ldcol<VZ>(psn, scur, j + 5, nxx, ldz, zb);
@@ -914,7 +924,7 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
             float ac[VZ], il[VZ], cjj[VZ];
             ldcol<VZ>(ac, acc, jj, nxx, ldz, zb);
             ldcol<VZ>(il, ill, jj, nxx, ldz, zb);
-            ldcol<VZ>(cjj, cc, jj, nxx, ldz, zb);   // c at jj (read 4 columns ago)
+            vload<VZ>(cjj, &cring[slot][lane * VZ]);   // c at jj (parked 4 columns ago)
             float es[VZ + 8], er[VZ + 8];
             zext_local<VZ>(es, qs2[4]);
             zext_local<VZ>(er, qr2[4]);
