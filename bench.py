@@ -386,6 +386,7 @@ def main():
     fwd_roof["kernel"] = "fwd_step_kernel (random boundary)"
     stf, vzf, xwf, g = eng.get_tuning(0)
     str_, vzr, xwr, _ = eng.get_tuning(1)
+    sts, vzs, xws, _ = eng.get_tuning(2)
     out = {
         "metric": "Gcell-updates/s", "value": round(value, 2), "unit": "Gcell-updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -397,6 +398,7 @@ def main():
                    "shots": nshots, "nzz": nzz, "nxx": nxx, "steps_per_pass": T,
                    "tuning": {"forward": {"steps_per_launch": stf, "vz": vzf, "xw": xwf},
                               "reverse": {"steps_per_launch": str_, "vz": vzr, "xw": xwr},
+                              "forward_sponge": {"steps_per_launch": sts, "vz": vzs, "xw": xws},
                               "graph": g},
                    "measured_copy_GBs": round(copy_gbs, 1)},
         "roofline": rev_roof,

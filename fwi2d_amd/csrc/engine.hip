@@ -117,8 +117,9 @@ struct fwi_ctx {
     float vmin = 0.f, vmax = 0.f;
     Quad fwd{};                // final (u_T, u_{T-1}) of the last forward pass
 
-    int vz[2] = {4, 2}, xw[2] = {32, 32};   // [0] forward kernel, [1] reverse kernel
-    int nsteps[2] = {2, 2};                 // time steps per launch
+    // launch shapes: [0] random-boundary forward, [1] reverse, [2] sponge forward
+    int vz[3] = {2, 1, 2}, xw[3] = {64, 64, 64};
+    int nsteps[3] = {2, 2, 2};              // time steps per launch
     int use_graph = 1;
     std::map<long, hipGraphExec_t> graphs;
 
@@ -217,7 +218,8 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
 {
     const int T = T_of(c);
     const fwi::Grid &g = fa ? fa->g : ra->g;
-    const int kind = fa ? 0 : 1;
+    const int kind = fa ? (fa->abc ? 2 : 0) : 1;    // tuning slot
+    const int pkind = fa ? 0 : 1;                   // profile slot: forward / reverse
     const fwi::Tiling tl = tiling_of(c, g, kind);
     const int per = c->nsteps[kind];
     Quad s0 = sq, r0 = rq ? *rq : sq;
@@ -300,9 +302,9 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
         HIP_TRY(hipEventSynchronize(c->ev1));
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        c->prof_ms[kind] += ms;
-        c->prof_launches[kind] += launches;
-        c->prof_cells[kind] += (double)nslots * g.nzz * g.nxx * T * (kind ? 2.0 : 1.0);
+        c->prof_ms[pkind] += ms;
+        c->prof_launches[pkind] += launches;
+        c->prof_cells[pkind] += (double)nslots * g.nzz * g.nxx * T * (pkind ? 2.0 : 1.0);
     }
     return FWI_OK;
 }
@@ -503,12 +505,7 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     if (r == FWI_OK) r = dalloc(&c->d_partials, c->npartials, c->stream);
     if (r == FWI_OK) r = dalloc(&c->d_sum, 1, c->stream);
     if (r != FWI_OK) { fwi_destroy(c); return r; }
-    // default launch shape (the autotuner refines it): two steps per launch,
-    // 2 rows per lane forward, 1 reverse (register pressure), 64-column chunks
-    c->nsteps[0] = c->nsteps[1] = 2;
-    c->vz[0] = 2;
-    c->vz[1] = 1;
-    c->xw[0] = c->xw[1] = 64;
+    // default launch shapes (the autotuner refines them, see the fields)
     *out = c;
     return FWI_OK;
 }
@@ -932,10 +929,10 @@ int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_
                    int32_t use_graph)
 {
     if (!c) return fail(FWI_ERR_ARG, "null context");
-    if (kernel < -1 || kernel > 1) return fail(FWI_ERR_ARG, "kernel must be -1, 0 or 1");
+    if (kernel < -1 || kernel > 2) return fail(FWI_ERR_ARG, "kernel must be -1, 0, 1 or 2");
     if (vz > 0 && vz != 1 && vz != 2 && vz != 4) return fail(FWI_ERR_ARG, "vz must be 1, 2 or 4");
     if (steps > 2) return fail(FWI_ERR_ARG, "steps per launch must be 1 or 2");
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 3; ++k) {
         if (kernel != -1 && kernel != k) continue;
         if (steps > 0) c->nsteps[k] = steps;
         if (vz > 0) c->vz[k] = vz;
@@ -950,7 +947,7 @@ int fwi_get_tuning(fwi_ctx *c, int32_t kernel, int32_t *steps, int32_t *vz, int3
                    int32_t *use_graph)
 {
     if (!c) return fail(FWI_ERR_ARG, "null context");
-    if (kernel < 0 || kernel > 1) return fail(FWI_ERR_ARG, "kernel must be 0 or 1");
+    if (kernel < 0 || kernel > 2) return fail(FWI_ERR_ARG, "kernel must be 0, 1 or 2");
     if (steps) *steps = c->nsteps[kernel];
     if (vz) *vz = c->vz[kernel];
     if (xw) *xw = c->xw[kernel];
@@ -960,25 +957,29 @@ int fwi_get_tuning(fwi_ctx *c, int32_t kernel, int32_t *steps, int32_t *vz, int3
 
 // Times `reps` launches of every candidate launch shape (steps per launch,
 // rows per lane, chunk width) of the forward and the reverse kernels on the
-// random-boundary grid with nslots slots and keeps the fastest per time step.
-// The launches use step index T (no injection, no sampling) on zeroed
-// fields, and everything is zeroed again afterwards, so no pass state changes.
+// random-boundary grid (and the sponge forward on its grid) with nslots slots
+// and keeps the fastest per time step.  The launches run on zeroed fields,
+// and everything is zeroed again afterwards, so no pass state changes.
 int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, double *rev_us)
 {
     TRY(check_ready(c));
     if (!c->grid[1].on) return fail(FWI_ERR_ARG, "autotune runs on the random-boundary grid");
     if (nslots < 1 || nslots > c->cfg.max_batch) return fail(FWI_ERR_ARG, "bad slot count");
     if (reps < 1) reps = 1;
-    const GridBuf &gb = c->grid[1];
     const int T = T_of(c);
-    const long nf = (long)nslots * gb.g.fstride;
+    const long nf = (long)nslots * c->fstride_max;
     for (int k = 0; k < 8; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
     TRY(set_shot0(c, 0));
     const int vzs[3] = {1, 2, 4};
-    const long cells = (long)nslots * gb.g.nzz * gb.g.nxx;
-    const int ncols = gb.g.nxx - 8;
-    double best[2] = {1e30, 1e30};
-    for (int kind = 0; kind < 2; ++kind) {
+    double best[3] = {1e30, 1e30, 1e30};
+    for (int kind = 0; kind < 3; ++kind) {
+        // kind 2, the sponge forward, runs on its own grid with the receivers
+        // sampled (step index past tlag); the others at step T (no injection)
+        if (kind == 2 && !c->grid[0].on) continue;
+        const GridBuf &gb = c->grid[kind == 2 ? 0 : 1];
+        const int tstep = (kind == 2 && c->tlag + 3 <= T) ? c->tlag + 1 : T;
+        const long cells = (long)nslots * gb.g.nzz * gb.g.nxx;
+        const int ncols = gb.g.nxx - 8;
         int bst = c->nsteps[kind], bvz = c->vz[kind], bxw = c->xw[kind];
         for (int st = 1; st <= 2; ++st)
             for (int a = 0; a < 3; ++a) {
@@ -1002,7 +1003,8 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
                     c->vz[kind] = vzs[a];
                     c->xw[kind] = xwv;
                     const fwi::Tiling tl = tiling_of(c, gb.g, kind);
-                    fwi::FwdStep fa = fwd_args(c, FWI_BOUNDARY_RBC, false);
+                    fwi::FwdStep fa = kind == 2 ? fwd_args(c, FWI_BOUNDARY_ABC, true)
+                                                : fwd_args(c, FWI_BOUNDARY_RBC, false);
                     fa.cur = c->d_f[1]; fa.old = c->d_f[0];
                     fa.out1 = (st == 2) ? c->d_f[2] : c->d_f[0]; fa.out2 = c->d_f[3];
                     fwi::RevStep ra = rev_args(c, 0);
@@ -1011,8 +1013,9 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
                     ra.rcur = c->d_f[5]; ra.rold = c->d_f[4];
                     ra.rout1 = (st == 2) ? c->d_f[6] : c->d_f[4]; ra.rout2 = c->d_f[7];
                     auto launch = [&]() {
-                        return kind == 0 ? fwi::launch_fwd_step(fa, tl, nslots, T, st, c->stream)
-                                         : fwi::launch_rev_step(ra, tl, nslots, T, st, c->stream);
+                        return kind != 1
+                                   ? fwi::launch_fwd_step(fa, tl, nslots, tstep, st, c->stream)
+                                   : fwi::launch_rev_step(ra, tl, nslots, T, st, c->stream);
                     };
                     for (int w = 0; w < 2; ++w) HIP_TRY(launch());
                     HIP_TRY(hipEventRecord(c->ev0, c->stream));

@@ -84,8 +84,9 @@ hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k
                            hipStream_t s);
 // z segments of a launch shape (two-step launches write 64*vz - 8 rows per wave)
 int zsegments(int nzz, int vz, int steps);
-// Waves of the random-boundary forward (kind 0) or imaging reverse (kind 1)
-// step kernel that the current device holds at once (0 if unknown).
+// Waves of the random-boundary forward (kind 0), imaging reverse (kind 1) or
+// sponge forward (kind 2) step kernel that the current device holds at once
+// (0 if unknown).
 int resident_waves(int kind, int steps, int vz);
 
 // Adds idh2*w[0] at each slot's source cell of `field` (before step 0).

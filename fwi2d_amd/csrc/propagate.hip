@@ -1274,6 +1274,8 @@ int resident_vz(int kind, int steps)
 {
     if (kind == 0)
         return steps == 2 ? resident_of(fwd2_kernel<VZ, false>) : resident_of(fwd1_kernel<VZ, false>);
+    if (kind == 2)
+        return steps == 2 ? resident_of(fwd2_kernel<VZ, true>) : resident_of(fwd1_kernel<VZ, true>);
     return steps == 2 ? resident_of(rev2_kernel<VZ, 0>) : resident_of(rev1_kernel<VZ, 0>);
 }
 }  // namespace

@@ -176,7 +176,8 @@ int fwi_grid_shape(fwi_ctx *ctx, int32_t boundary, int32_t *nzz, int32_t *nxx, i
 /* Blocks until the context's stream is idle. */
 int fwi_synchronize(fwi_ctx *ctx);
 
-/* Launch tuning of the step kernels (kernel 0 forward, 1 reverse, -1 both):
+/* Launch tuning of the step kernels (kernel 0 random-boundary forward, 1
+ * reverse, 2 sponge-boundary forward, -1 all):
  * time steps per launch (1, or 2 = temporal blocking), cells per lane along
  * z (1, 2 or 4), columns per wave, and hipGraph capture of the time loops
  * (1 on, 0 off).  0 / negative keeps the current value.  Every shape gives
@@ -187,8 +188,11 @@ int fwi_get_tuning(fwi_ctx *ctx, int32_t kernel, int32_t *steps, int32_t *vz, in
                    int32_t *use_graph);
 
 /* Autotuner: times every candidate launch shape of the forward and reverse
- * kernels on the random-boundary grid with nslots resident shots (`reps`
- * launches each) and keeps the fastest; returns the time per step (us).
+ * kernels on the random-boundary grid, and of the sponge-boundary forward
+ * kernel (receivers sampled) on its grid when that grid is configured, with
+ * nslots resident shots (`reps` launches each), and keeps the fastest;
+ * returns the time per step (us) of the random-boundary forward and of the
+ * reverse kernel.
  * Candidates: 1 or 2 steps per launch x 1, 2 or 4 rows per lane x chunk
  * widths 16-128 columns, 4 and 8 for small batches, and the widths that make
  * a launch 1-4 full rounds of the waves the device holds at once.  No
