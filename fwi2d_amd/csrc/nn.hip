@@ -133,85 +133,97 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvFwd a)
 
 // ---------------------------------------------------------------- conv wgrad
 
-// Block (chunk, tap): for each 32-pixel step the block stages dy (32 x coutp)
-// and the tap-shifted input (32 x cin16) transposed in LDS, so that the MFMA
-// operands (8 consecutive pixels of one channel) are contiguous.  Wave w owns
-// output-channel block w (co 16w .. 16w+15) and all ci blocks.
-constexpr int kPixPad = 40;   // LDS row length (32 pixels + 8 padding)
+// dW[co][tap][ci] = sum_p dy[p][co] x[p + offset(tap)][ci] as MFMAs with the
+// pixels as the K dimension.  A block walks tiles of 32 consecutive pixels of
+// one image row.  Per tile it stages dy (32 x coutp) transposed, and the
+// input rows h-1 .. h+1, columns w0-1 .. w0+32, transposed and stored once
+// per column shift, so that every MFMA operand (8 consecutive pixels of one
+// channel) is an aligned 16-byte LDS read.  Wave w owns output channels
+// 16w .. 16w+15 and accumulates all 9 taps x all input-channel blocks over
+// its tiles; the per-block partial sums are reduced by wgrad_reduce.
+constexpr int kPixRow = 40;   // LDS row: 32 pixels + 8 padding
 
+template <int CIB>
 __global__ __launch_bounds__(256) void conv_wgrad_kernel(ConvWgrad a)
 {
-    __shared__ __attribute__((aligned(16))) bf16 sdy[64 * kPixPad];
-    __shared__ __attribute__((aligned(16))) bf16 sx[64 * kPixPad];
-    __shared__ float sbias[64];
-    const int chunk = blockIdx.x, tap = blockIdx.y;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int CI = CIB * 16;
+    __shared__ __attribute__((aligned(16))) bf16 sdy[64 * kPixRow];            // [co][p]
+    __shared__ __attribute__((aligned(16))) bf16 sx[3 * CI * 3 * kPixRow];     // [shift][ci][row][p]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, r16 = lane & 15;
-    const int hw = a.h * a.w;
-    const int npix = a.nb * hw;
-    const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
-    const int nblk_ci = a.cin16 / 16;
+    const int nwt = (a.w + 31) >> 5;
+    const long ntiles = (long)a.nb * a.h * nwt;
     const bool has_m = wave * 16 < a.coutp;
-    f32x4 acc[4];
+    f32x4 acc[9][CIB];
 #pragma unroll
-    for (int n = 0; n < 4; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
-    if (threadIdx.x < 64) sbias[threadIdx.x] = 0.f;
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int c = 0; c < CIB; ++c) acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
     float bsum = 0.f;
 
-    const int pbeg = chunk * a.chunk;
-    const int pend = min(pbeg + a.chunk, npix);
-    for (int p0 = pbeg; p0 < pend; p0 += 32) {
-        // stage dy[p0 .. p0+31][0 .. coutp) as sdy[co][p]
-        for (int e = threadIdx.x; e < 32 * a.coutp; e += 256) {
-            const int p = e / a.coutp, co = e - p * a.coutp;
-            const int pix = p0 + p;
-            bf16 v = (bf16)0.0f;
-            if (pix < pend && co < a.dy_stride) v = a.dy[(long)pix * a.dy_stride + co];
-            sdy[co * kPixPad + p] = v;
-        }
-        // stage x at the tap-shifted pixel as sx[ci][p]
-        for (int e = threadIdx.x; e < 32 * a.cin16; e += 256) {
-            const int p = e / a.cin16, ci = e - p * a.cin16;
-            const int pix = p0 + p;
-            bf16 v = (bf16)0.0f;
-            if (pix < pend && ci < a.cin) {
-                const int n = pix / hw, rem = pix - n * hw;
-                const int hh = rem / a.w + dyo, ww = rem % a.w + dxo;
-                if (hh >= 0 && hh < a.h && ww >= 0 && ww < a.w)
-                    v = a.x[((long)(n * a.h + hh) * a.w + ww) * a.cin + ci];
-            }
-            sx[ci * kPixPad + p] = v;
-        }
-        __syncthreads();
-        if (has_m) {
-            const bf16x8 A = *reinterpret_cast<const bf16x8 *>(sdy + (wave * 16 + r16) * kPixPad + g * 8);
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int n = (int)(tile / ((long)a.h * nwt));
+        const int rem = (int)(tile - (long)n * a.h * nwt);
+        const int h = rem / nwt, w0 = (rem - h * nwt) * 32;
+        // dy: 8 channels of one pixel per thread (pixel index fastest)
+        for (int e = tid; e < 32 * (a.coutp >> 3); e += 256) {
+            const int p = e & 31, c8 = (e >> 5) * 8;
+            bf16x8 v = zero8();
+            if (w0 + p < a.w && c8 < a.dy_stride)
+                v = ld8(a.dy + ((long)(n * a.h + h) * a.w + w0 + p) * a.dy_stride + c8);
 #pragma unroll
-            for (int n = 0; n < 4; ++n) {
-                if (n < nblk_ci) {
-                    const bf16x8 B = *reinterpret_cast<const bf16x8 *>(sx + (n * 16 + r16) * kPixPad + g * 8);
-                    acc[n] = mfma(A, B, acc[n]);
+            for (int k = 0; k < 8; ++k) sdy[(c8 + k) * kPixRow + p] = v[k];
+        }
+        // x: sx[s][ci][r][p] = x[row h-1+r][column w0-1+p+s][ci]
+        for (int e = tid; e < 3 * 34 * (CI >> 3); e += 256) {
+            const int col = e % 34, t2 = e / 34;
+            const int r = t2 % 3, c8 = (t2 / 3) * 8;
+            const int hh = h - 1 + r, ww = w0 - 1 + col;
+            bf16x8 v = zero8();
+            if (hh >= 0 && hh < a.h && ww >= 0 && ww < a.w && c8 < a.cin)
+                v = ld8(a.x + ((long)(n * a.h + hh) * a.w + ww) * a.cin + c8);
+#pragma unroll
+            for (int sh = 0; sh < 3; ++sh) {
+                const int p = col - sh;
+                if (p >= 0 && p < 32) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) sx[((sh * CI + c8 + k) * 3 + r) * kPixRow + p] = v[k];
                 }
             }
         }
-        if (tap == 4 && threadIdx.x < a.coutp) {
-            for (int p = 0; p < 32; ++p) bsum += (float)sdy[threadIdx.x * kPixPad + p];
+        __syncthreads();
+        if (has_m) {
+            const bf16x8 A =
+                *reinterpret_cast<const bf16x8 *>(sdy + (wave * 16 + r16) * kPixRow + g * 8);
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) {
+                const int r = tap / 3, sh = tap % 3;
+#pragma unroll
+                for (int cb = 0; cb < CIB; ++cb) {
+                    const bf16x8 B = *reinterpret_cast<const bf16x8 *>(
+                        sx + ((sh * CI + cb * 16 + r16) * 3 + r) * kPixRow + g * 8);
+                    acc[tap][cb] = mfma(A, B, acc[tap][cb]);
+                }
+            }
         }
+        if (tid < a.coutp)
+            for (int p = 0; p < 32; ++p) bsum += (float)sdy[tid * kPixRow + p];
         __syncthreads();
     }
     if (has_m) {
-        float *part = a.part + (long)chunk * a.coutp * 9 * a.cin16;
+        float *part = a.part + (long)blockIdx.x * a.coutp * 9 * CI;
 #pragma unroll
-        for (int n = 0; n < 4; ++n) {
-            if (n >= nblk_ci) continue;
+        for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int co = wave * 16 + 4 * g + e;
-                const int ci = n * 16 + r16;
-                part[((long)co * 9 + tap) * a.cin16 + ci] = acc[n][e];
-            }
-        }
+            for (int cb = 0; cb < CIB; ++cb)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int co = wave * 16 + 4 * g + e;
+                    const int ci = cb * 16 + r16;
+                    part[((long)co * 9 + tap) * CI + ci] = acc[tap][cb][e];
+                }
     }
-    if (tap == 4 && threadIdx.x < a.coutp) a.partb[(long)chunk * a.coutp + threadIdx.x] = bsum;
+    if (tid < a.coutp) a.partb[(long)blockIdx.x * a.coutp + tid] = bsum;
 }
 
 __global__ void wgrad_reduce_kernel(const float *part, const float *partb, int nchunk, int cout,
@@ -434,9 +446,14 @@ hipError_t conv_fwd(const ConvFwd &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t conv_wgrad(const ConvWgrad &a, int nchunk, hipStream_t s)
+hipError_t conv_wgrad(const ConvWgrad &a, int nblocks, hipStream_t s)
 {
-    hipLaunchKernelGGL(conv_wgrad_kernel, dim3(nchunk, 9), dim3(256), 0, s, a);
+    switch (a.cin16) {
+    case 16: hipLaunchKernelGGL(conv_wgrad_kernel<1>, dim3(nblocks), dim3(256), 0, s, a); break;
+    case 32: hipLaunchKernelGGL(conv_wgrad_kernel<2>, dim3(nblocks), dim3(256), 0, s, a); break;
+    case 64: hipLaunchKernelGGL(conv_wgrad_kernel<4>, dim3(nblocks), dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

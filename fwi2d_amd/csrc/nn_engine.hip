@@ -62,7 +62,7 @@ struct fwi_nn {
          *dz5 = nullptr, *dp4 = nullptr, *dz4 = nullptr, *dz3 = nullptr, *dp2 = nullptr,
          *dz2 = nullptr, *dz1 = nullptr;
     float *part = nullptr, *partb = nullptr;
-    int chunk = 4096, maxchunks = 1;
+    int maxchunks = 512;         // blocks of the weight-gradient kernel (partial sums)
     float *loss_part = nullptr, *loss = nullptr;
     int nloss = 512;
     std::vector<void *> allocs;
@@ -155,8 +155,8 @@ int conv_backward_data(fwi_nn *n, const Layer &l, int nb, const bf16 *dy, const 
 
 int conv_backward_weights(fwi_nn *n, const Layer &l, int nb, const bf16 *x, const bf16 *dy)
 {
-    const long npix = (long)nb * l.h * l.w;
-    const int nchunk = (int)((npix + n->chunk - 1) / n->chunk);
+    const long ntiles = (long)nb * l.h * ((l.w + 31) / 32);
+    const int nchunk = (int)std::min<long>(n->maxchunks, ntiles);
     fwi::nn::ConvWgrad a{};
     a.x = x;
     a.cin = l.cin;
@@ -169,7 +169,6 @@ int conv_backward_weights(fwi_nn *n, const Layer &l, int nb, const bf16 *x, cons
     a.nb = nb;
     a.h = l.h;
     a.w = l.w;
-    a.chunk = n->chunk;
     NN_TRY(fwi::nn::conv_wgrad(a, nchunk, n->stream));
     NN_TRY(fwi::nn::wgrad_reduce(n->part, n->partb, nchunk, l.cout, l.cin, a.cin16, l.coutp,
                                  n->grads + l.woff, n->grads + l.boff, n->stream));
@@ -354,7 +353,6 @@ int fwi_nn_create(int32_t batch, int32_t height, int32_t width, uint64_t seed, f
     A(&n->ds5, P / 4 * 64); A(&n->dz5, P / 16 * 64); A(&n->dp4, P / 16 * 64);
     A(&n->dz4, P / 4 * 64); A(&n->dz3, P / 4 * 64); A(&n->dp2, P / 4 * 32);
     A(&n->dz2, P * 32); A(&n->dz1, P * 32);
-    n->maxchunks = (int)((P + n->chunk - 1) / n->chunk);
     A(&n->part, (long)n->maxchunks * 64 * 9 * 64);
     A(&n->partb, (long)n->maxchunks * 64);
     A(&n->loss_part, n->nloss);

@@ -35,14 +35,14 @@ hipError_t conv_fwd(const ConvFwd &a, hipStream_t s);
 // pixels of dy[p][co] x[p + off(tap)][ci]; partb[chunk][co] = sum dy[p][co].
 struct ConvWgrad {
     const bf16 *x;
-    int cin, cin16;       // cin16 = cin rounded up to 16
+    int cin, cin16;       // cin: channel stride of x (multiple of 8); cin16: 16, 32 or 64
     const bf16 *dy;
-    int coutp, dy_stride;
-    float *part, *partb;
+    int coutp, dy_stride; // coutp: 16, 32 or 64; dy_stride: channel stride of dy
+    float *part, *partb;  // per-block partial sums [block][coutp][9][cin16], [block][coutp]
     int nb, h, w;
-    int chunk;            // pixels per block (multiple of 32)
 };
-hipError_t conv_wgrad(const ConvWgrad &a, int nchunk, hipStream_t s);
+// nblocks blocks share the 32-pixel row tiles; each writes its partial sums.
+hipError_t conv_wgrad(const ConvWgrad &a, int nblocks, hipStream_t s);
 // dw[co][tap][ci] (padded master layout cin) = sum_chunk part; db[co] likewise.
 hipError_t wgrad_reduce(const float *part, const float *partb, int nchunk, int cout, int cin,
                         int cin16, int coutp, float *dw, float *db, hipStream_t s);
