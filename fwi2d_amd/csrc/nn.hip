@@ -226,25 +226,38 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(ConvWgrad a)
     if (tid < a.coutp) a.partb[(long)blockIdx.x * a.coutp + tid] = bsum;
 }
 
-__global__ void wgrad_reduce_kernel(const float *part, const float *partb, int nchunk, int cout,
-                                    int cin, int cin16, int coutp, float *dw, float *db)
+// dw / db = sum over the blocks' partial sums.  A block covers 32
+// consecutive outputs (weights, then biases) with 8 threads each, which sum
+// every 8th partial; the 8 sums are added in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float *part, const float *partb,
+                                                           int nchunk, int cout, int cin,
+                                                           int cin16, int coutp, float *dw,
+                                                           float *db)
 {
+    __shared__ float red[8][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const long n = (long)cout * 9 * cin;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n + cout;
-         idx += (long)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        if (idx < n) {
-            const int ci = (int)(idx % cin);
-            const int tap = (int)((idx / cin) % 9);
-            const int co = (int)(idx / (9L * cin));
-            const long src = ((long)co * 9 + tap) * cin16 + ci;
-            for (int k = 0; k < nchunk; ++k) s += part[(long)k * coutp * 9 * cin16 + src];
-            dw[idx] = s;
-        } else {
-            const int co = (int)(idx - n);
-            for (int k = 0; k < nchunk; ++k) s += partb[(long)k * coutp + co];
-            db[co] = s;
-        }
+    const long idx = (long)blockIdx.x * 32 + tx;
+    float s = 0.f;
+    if (idx < n) {
+        const int ci = (int)(idx % cin);
+        const int tap = (int)((idx / cin) % 9);
+        const int co = (int)(idx / (9L * cin));
+        const long src = ((long)co * 9 + tap) * cin16 + ci;
+        const long stride = (long)coutp * 9 * cin16;
+        for (int k = ty; k < nchunk; k += 8) s += part[(long)k * stride + src];
+    } else if (idx < n + cout) {
+        const int co = (int)(idx - n);
+        for (int k = ty; k < nchunk; k += 8) s += partb[(long)k * coutp + co];
+    }
+    red[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && idx < n + cout) {
+        float t = red[0][tx];
+#pragma unroll
+        for (int j = 1; j < 8; ++j) t += red[j][tx];
+        if (idx < n) dw[idx] = t;
+        else db[idx - n] = t;
     }
 }
 
@@ -461,8 +474,8 @@ hipError_t wgrad_reduce(const float *part, const float *partb, int nchunk, int c
                         int cin16, int coutp, float *dw, float *db, hipStream_t s)
 {
     const long n = (long)cout * 9 * cin + cout;
-    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, part,
-                       partb, nchunk, cout, cin, cin16, coutp, dw, db);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, s,
+                       part, partb, nchunk, cout, cin, cin16, coutp, dw, db);
     return hipGetLastError();
 }
 
