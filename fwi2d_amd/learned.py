@@ -90,6 +90,12 @@ TRAIN_SMALL = FwiSetup(nz=128, nx=384, dh=10.0, nt=1875, dt=8e-4, nsrc=16, nrec=
                        peaks=(4.0, 8.0, 12.0), band_iters=(3, 3, 2), zmask=60.0)
 COMPARE_SMALL = dataclasses.replace(TRAIN_SMALL, nsrc=32, nrec=384, seed=5,
                                     band_iters=(7, 7, 6))
+# The comparison configuration at full size: 401 x 1601 at 10 m, 200-cell
+# random boundary, 256 shots, 1601 receivers, nt 5001 at 0.6 ms, bands
+# 4 / 8 / 12 Hz (20 iterations: 7 / 7 / 6).
+COMPARE_FULL = FwiSetup(nz=401, nx=1601, dh=10.0, nt=5001, dt=6e-4, nsrc=256, nrec=1601,
+                        zsrc=20.0, zrec=10.0, rbc_nb=200, abc_nb=100, varVp_rel=0.4, seed=5,
+                        peaks=(4.0, 8.0, 12.0), band_iters=(7, 7, 6), zmask=60.0)
 
 
 def velocity(m):
@@ -105,13 +111,17 @@ class Fwi:
     """One survey on one engine: observed data per band, the gradient pass,
     the misfit of a model and the plain line search."""
 
-    def __init__(self, setup, v_true, v_init):
+    def __init__(self, setup, v_true, v_init, autotune=True):
         s = self.setup = setup
         self.eng = A.Engine(s.nz, s.nx, s.dh, s.nt, s.dt, abc_nb=s.abc_nb,
                             abc_factor=s.abc_factor, rbc_nb=s.rbc_nb, rbc_varVp=0.0,
                             rbc_varVp_rel=s.varVp_rel, rbc_seed=s.seed, max_batch=s.nsrc)
         self.eng.set_geometry(*s.acquisition())
         self.eng.set_model(v_true)
+        if autotune:
+            w, tlag = A.ricker_wavelet(s.nt, s.dt, 3.0 * s.peaks[-1])
+            self.eng.set_wavelet(w, tlag)
+            self.eng.autotune(s.nsrc, reps=4)
         self.bands = []
         for peak in s.peaks:
             w, tlag = A.ricker_wavelet(s.nt, s.dt, 3.0 * peak)
@@ -177,12 +187,14 @@ def normalise_inputs(raw, ill, m):
 
 
 class Direction:
-    """The network as a search-direction generator for [nx, nz] grids."""
+    """The network as a search-direction generator for [nx, nz] grids.  The
+    network runs on the grid padded (edge values repeated) to multiples of 4
+    (its two 2x2 pools) and its output is cropped back."""
 
     def __init__(self, nx, nz, params=None, seed=0, batch=1):
-        if nx % 4 or nz % 4:
-            raise ValueError("grid sides must be multiples of 4 (two 2x2 pools)")
-        self.net = A.Net(batch, nx, nz, seed)
+        self.nx, self.nz = nx, nz
+        self.px, self.pz = (-nx) % 4, (-nz) % 4
+        self.net = A.Net(batch, nx + self.px, nz + self.pz, seed)
         if params is not None:
             self.net.set_params(params)
 
@@ -190,7 +202,11 @@ class Direction:
         self.net.close()
 
     def __call__(self, raw, ill, m, g):
-        d = self.net.forward(normalise_inputs(raw, ill, m)[None])[0] - g
+        x = normalise_inputs(raw, ill, m)
+        if self.px or self.pz:
+            x = np.pad(x, ((0, 0), (0, self.px), (0, self.pz)), mode="edge")
+        y = self.net.forward(np.ascontiguousarray(x)[None])[0][:self.nx, :self.nz]
+        d = y - g
         s = float(np.max(np.abs(d)))
         return d / s if s > 0 else d
 
@@ -359,6 +375,11 @@ def main(argv=None):
     p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--compare-seed", type=int, default=10005)
     p.add_argument("--params", default="", help="trained parameters (skips generation and training)")
+    p.add_argument("--size", choices=("small", "full"), default="small",
+                   help="comparison at the reduced size or at the full config-5 size")
+    p.add_argument("--runs", default="plain,control,learned",
+                   help="comma-separated runs of the comparison (plain, control, learned)")
+    p.add_argument("--bands", default="", help="iterations per band of the comparison runs")
     p.add_argument("--short-bands", default="",
                    help="iterations per band of extra, shorter control and learned runs")
     a = p.parse_args(argv)
@@ -380,28 +401,35 @@ def main(argv=None):
         log({"holdout_cosine_learned": cos[0], "holdout_cosine_plain": cos[1]})
     params.tofile(os.path.join(a.out, "preconditioner_params.bin"))
 
-    s = COMPARE_SMALL
+    s = COMPARE_SMALL if a.size == "small" else COMPARE_FULL
+    if a.bands:
+        s = dataclasses.replace(s, band_iters=tuple(int(k) for k in a.bands.split(",")))
     v, v0 = C.config5_models(a.compare_seed, s.nz, s.nx)
+    t0 = time.perf_counter()
     fwi = Fwi(s, v, v0)
+    log({"setup_s": time.perf_counter() - t0, "grid": [s.nz, s.nx], "shots": s.nsrc,
+         "band_iters": list(s.band_iters)})
     d = Direction(s.nx, s.nz, params)
-    runs = {"plain": run_fwi(fwi, None, log=log)[0],
-            "control": run_fwi(fwi, plain_direction, log=log)[0],
-            "learned": run_fwi(fwi, d, log=log)[0]}
+    arms = {"plain": None, "control": plain_direction, "learned": d}
+    runs = {}
+    for name in a.runs.split(","):
+        runs[name] = run_fwi(fwi, arms[name], log=lambda e, n=name: log(dict(e, run=n)))[0]
     if a.short_bands:
         bi = tuple(int(k) for k in a.short_bands.split(","))
         runs["control_short"] = run_fwi(fwi, plain_direction, band_iters=bi, log=log)[0]
         runs["learned_short"] = run_fwi(fwi, d, band_iters=bi, log=log)[0]
     d.close()
     fwi.close()
-    target = runs["plain"][-1]["misfit"]
-    t_plain = runs["plain"][-1]["time"]
+    ref = runs.get("plain")
+    target = ref[-1]["misfit"] if ref else None
+    t_plain = ref[-1]["time"] if ref else None
     summary = {}
     for name, h in runs.items():
-        t = time_to_misfit(h, target, runs["plain"][-1]["band"])
+        t = time_to_misfit(h, target, ref[-1]["band"]) if ref else None
         summary[name] = {"final_misfit": h[-1]["misfit"], "time_s": h[-1]["time"],
                          "final_model_rms_error": h[-1]["model_rms_error"],
                          "time_to_plain_final_misfit_s": t,
-                         "time_ratio": (t / t_plain) if t else None,
+                         "time_ratio": (t / t_plain) if (t and t_plain) else None,
                          "trial_steps_taken": sum(e["taken"] == "learned" for e in h)}
     report = {"samples": int(len(x)), "generation_s": t_gen, "training_s": t_train,
               "holdout_cosine": {"learned": cos[0], "plain_direction": cos[1]},
