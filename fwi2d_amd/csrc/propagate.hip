@@ -10,28 +10,36 @@
 //  * the wave marches along x over xw columns and keeps the 9 columns the
 //    x-derivative needs in a register queue, so the current field is read
 //    from memory about once per step (plus an 8-column warm-up per chunk);
-//  * the +-4 z-neighbours come from the neighbouring lanes by cross-lane
-//    shuffles; only the edge lanes read the 4-row halo from memory;
+//  * the +-4 z-neighbours come from the neighbouring lanes by DPP whole-wave
+//    shifts; the 4 rows beyond each end of the wave come from one prefetched
+//    float4 "halo" load per column;
+//  * two time steps per launch (temporal blocking): u_{t+1} is computed with
+//    a 4-row / 4-column margin and kept in a second register queue, u_{t+2}
+//    on the inner rows; c, read for the first step, waits in an LDS ring for
+//    the second;
 //  * source injection, the Cerjan sponge, receiver sampling (forward) and
 //    data injection, imaging / gradient and illumination (reverse) are fused
-//    into the same pass, so one launch per time step does all the work;
+//    into the same pass; the receiver tables and data of a wave's columns
+//    are staged in LDS once per launch;
 //  * blocks are renumbered so that each XCD receives a contiguous run of
 //    tiles: neighbouring chunks share their halo columns in that XCD's L2.
 //
-// FP32 operation order: the Laplacian is summed as differences from the
-// centre cell and the update as (p + (p - old)) + c*lap, with c = dt^2 v^2/dh^2.
-// This is the reference's recurrence with less rounding error (about 7x less
-// than the reference's order against an FP64 evaluation of the same
-// recurrence, over 2,000+ steps).  Every multiply-add that feeds a stored
-// value is an explicit fmaf, so all launch shapes (one or two steps per
-// launch, any tiling) produce bit-identical fields, images and gradients.
+// FP32 operation order: the Laplacian's nearest ring is summed as
+// differences from the centre cell (the farther rings as sums minus 4u) and
+// the update as (p + (p - old)) + c*lap, with c = dt^2 v^2/dh^2.  This is the
+// reference's recurrence with less rounding error (about 4-5x less than the
+// reference's order against an FP64 evaluation of the same recurrence, over
+// 2,000+ steps).  Every multiply-add that feeds a stored value is an explicit
+// fmaf, so all launch shapes (one or two steps per launch, any tiling)
+// produce bit-identical fields, images and gradients.
 //
 // Source injection is folded into the write of the field it enters:
 // the field written at step t receives idh2*w[t+1], which is what the
 // reference's step t+1 adds before reading it (modeling.cu:316-317).  With
-// the sponge on, the reference damps the current field in place
-// (modeling.cu:337-343); here the stored field is undamped and the factor is
-// applied when it is read back as the previous field one step later:
+// the sponge on, the reference damps both the new and the current field in
+// place after each step (modeling.cu:337-343); here the stored field carries
+// the first factor (as the new field) and the second one is applied when it
+// is read back as the previous field one step later:
 //   u_{t+1} = d * ((u_t + (u_t - d*u_{t-1})) + c*lap u_t)  [+ idh2 w[t+1]].
 #include "kernels.h"
 
