@@ -222,13 +222,17 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
     const int pkind = fa ? 0 : 1;                   // profile slot: forward / reverse
     const fwi::Tiling tl = tiling_of(c, g, kind);
     const int per = c->nsteps[kind];
+    // the remainder of a three-step pass runs as a two- or one-step launch,
+    // whose waves own more rows each
+    fwi::Tiling tls[4] = {tl, tl, tl, tl};
+    for (int n = 1; n <= 3; ++n) tls[n].nzseg = fwi::zsegments(g.nzz, tl.vz, n);
     Quad s0 = sq, r0 = rq ? *rq : sq;
     int launches = 0;
     auto enqueue = [&](bool launch) -> hipError_t {
         Quad s = s0, r = r0;
         launches = 0;
         for (int t = 0; t < T;) {
-            const int n = (per == 2 && t + 2 <= T) ? 2 : 1;
+            const int n = (per == 3 && t + 3 <= T) ? 3 : (per >= 2 && t + 2 <= T) ? 2 : 1;
             int f1 = 0, f2 = 0, h1 = 0, h2 = 0;
             s.free_pair(&f1, &f2);
             r.free_pair(&h1, &h2);
@@ -237,9 +241,9 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
                 fwi::FwdStep a = *fa;
                 a.cur = s.b[s.cur];
                 a.old = s.b[s.old];
-                a.out1 = (n == 2) ? s.b[f1] : s.b[s.old];
-                a.out2 = s.b[f2];
-                if (launch) e = fwi::launch_fwd_step(a, tl, nslots, t, n, c->stream);
+                a.out1 = (n >= 2) ? s.b[f1] : s.b[s.old];    // u_{t+n-1} (n >= 2)
+                a.out2 = s.b[f2];                              // u_{t+n}
+                if (launch) e = fwi::launch_fwd_step(a, tls[n], nslots, t, n, c->stream);
             } else {
                 fwi::RevStep a = *ra;
                 a.scur = s.b[s.cur];
@@ -250,10 +254,10 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
                 a.rold = r.b[r.old];
                 a.rout1 = (n == 2) ? r.b[h1] : r.b[r.old];
                 a.rout2 = r.b[h2];
-                if (launch) e = fwi::launch_rev_step(a, tl, nslots, t, n, c->stream);
+                if (launch) e = fwi::launch_rev_step(a, tls[n], nslots, t, n, c->stream);
             }
             if (e != hipSuccess) return e;
-            if (n == 2) {
+            if (n >= 2) {
                 s.after_double(f1, f2);
                 r.after_double(h1, h2);
             } else {
@@ -931,7 +935,9 @@ int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_
     if (!c) return fail(FWI_ERR_ARG, "null context");
     if (kernel < -1 || kernel > 2) return fail(FWI_ERR_ARG, "kernel must be -1, 0, 1 or 2");
     if (vz > 0 && vz != 1 && vz != 2 && vz != 4) return fail(FWI_ERR_ARG, "vz must be 1, 2 or 4");
-    if (steps > 2) return fail(FWI_ERR_ARG, "steps per launch must be 1 or 2");
+    if (steps > 3) return fail(FWI_ERR_ARG, "steps per launch must be 1, 2 or 3");
+    if (steps == 3 && kernel != 0 && kernel != 2)
+        return fail(FWI_ERR_ARG, "three steps per launch exist for the forward kernels only");
     for (int k = 0; k < 3; ++k) {
         if (kernel != -1 && kernel != k) continue;
         if (steps > 0) c->nsteps[k] = steps;
@@ -981,7 +987,7 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
         const long cells = (long)nslots * gb.g.nzz * gb.g.nxx;
         const int ncols = gb.g.nxx - 8;
         int bst = c->nsteps[kind], bvz = c->vz[kind], bxw = c->xw[kind];
-        for (int st = 1; st <= 2; ++st)
+        for (int st = 1; st <= (kind == 1 ? 2 : 3); ++st)
             for (int a = 0; a < 3; ++a) {
                 // fixed widths, narrow ones for small batches (more waves in
                 // flight), and widths that make the launch about 1-4 full
@@ -1006,7 +1012,7 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
                     fwi::FwdStep fa = kind == 2 ? fwd_args(c, FWI_BOUNDARY_ABC, true)
                                                 : fwd_args(c, FWI_BOUNDARY_RBC, false);
                     fa.cur = c->d_f[1]; fa.old = c->d_f[0];
-                    fa.out1 = (st == 2) ? c->d_f[2] : c->d_f[0]; fa.out2 = c->d_f[3];
+                    fa.out1 = (st >= 2) ? c->d_f[2] : c->d_f[0]; fa.out2 = c->d_f[3];
                     fwi::RevStep ra = rev_args(c, 0);
                     ra.scur = c->d_f[1]; ra.sold = c->d_f[0];
                     ra.sout1 = (st == 2) ? c->d_f[2] : c->d_f[0]; ra.sout2 = c->d_f[3];

@@ -694,6 +694,201 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
     }
 }
 
+// Three time steps per launch.  The wave computes u_{t+1} on 64*VZ rows and
+// its x-chunk plus 8 columns each side, u_{t+2} on the inner 64*VZ - 8 rows
+// and the chunk plus 4 columns, u_{t+3} on the inner 64*VZ - 16 rows of the
+// chunk; the three fields of a column travel through three register queues.
+// It reads u_t, u_{t-1} and c once and writes only what the next launch
+// needs: u_{t+2} (out1) and u_{t+3} (out2) for the cells it owns.
+template <int VZ, bool ABC>
+__global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int t)
+{
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int ldz = g.ldz, nxx = g.nxx;
+    const int zs = id.zseg * (64 * VZ - 16) - 4;    // first computed row; owned from zs + 8
+    const int zb = zs + lane * VZ;
+    const int zl = zb < 0 ? 0 : zb;                  // load row: rows above 0 feed rim rows only
+    const bool active = zb < ldz;
+    const bool own = active && (zb >= zs + 8) && (zb < zs + 64 * VZ - 8);
+    const int x0 = 4 + id.xc * xw;
+    const int x1 = min(x0 + xw, nxx - 4);
+
+    const long soff = (long)id.slot * g.fstride;
+    const float *cur = a.cur + soff;
+    const float *old = a.old + soff;
+    float *out1 = a.out1 + soff;
+    float *out2 = a.out2 + soff;
+    const float *cc = a.c + (long)id.slot * a.cstride;
+    const int gshot = *a.shot0 + id.slot;
+    const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
+    const float w1 = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
+    const float w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
+    const float w3 = (t + 3 < a.nt) ? a.wav[t + 3] : 0.f;
+    const bool rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
+    const bool rec3 = (a.seis != nullptr) && (t + 2 >= a.tlag) && (t + 2 <= a.T - 2);
+    float *seis = (rec1 || rec2 || rec3) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
+    const int xa = x0 - 8;                           // window [x0 - 8, x1 + 8)
+    __shared__ FwdStage st;
+    const bool staged = stage_fwd(a, rec1 || rec2 || rec3, ABC, xa, x1 + 8 - xa, st, lane);
+
+    float dzr[VZ];
+    bool inz[VZ];
+#pragma unroll
+    for (int r = 0; r < VZ; ++r) dzr[r] = ABC ? pad_factor(a.b1d, zb + r, g.nzz, g.nb, inz[r]) : 1.f;
+
+    float q1[9][VZ];     // u_t,     columns j-4 .. j+4
+    float q2[9][VZ];     // u_{t+1}, columns j-8 .. j
+    float q3[9][VZ];     // u_{t+2}, columns j-12 .. j-4
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        ldcol<VZ>(q1[k], cur, x0 - 12 + k, nxx, ldz, zl);
+        vzero<VZ>(q2[k]);
+        vzero<VZ>(q3[k]);
+    }
+    float on[VZ], cn[VZ], pn[VZ];
+    ldcol<VZ>(on, old, x0 - 8, nxx, ldz, zl);
+    ldcol<VZ>(cn, cc, x0 - 8, nxx, ldz, zl);
+    vzero<VZ>(pn);
+    float4 h = ldhalo<VZ>(cur, x0 - 8, 4, nxx - 4, ldz, zs, lane);
+    // c of the last nine columns (each lane its own rows), for steps 2 and 3
+    __shared__ float cring[9][64 * VZ];
+    int slot = 0;                                    // slot of column j
+
+    for (int j = x0 - 8; j < x1 + 8; ++j) {
+        float o1[VZ], cj[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
+        vstore<VZ>(&cring[slot][lane * VZ], cj);
+        const int slot4 = (slot >= 4) ? slot - 4 : slot + 5;     // column j - 4
+        const int slot8 = (slot >= 8) ? slot - 8 : slot + 1;     // column j - 8
+        slot = (slot == 8) ? 0 : slot + 1;
+        if (j + 1 < x1 + 8) {
+            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zl);
+            ldcol<VZ>(on, old, j + 1, nxx, ldz, zl);
+            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zl);
+        }
+        // ---- step 1: u_{t+1} at column j (0 on the rim columns)
+        float n1[VZ];
+        vzero<VZ>(n1);
+        if (j >= 4 && j < nxx - 4) {
+            const bool inx = (j >= g.nb) && (j < nxx - g.nb);
+            float dxj = 1.f;
+            if constexpr (ABC) {
+                bool in;
+                dxj = staged ? st.dx[j - xa] : pad_factor(a.b1d, j, nxx, g.nb, in);
+            }
+            float ext[VZ + 8], dv[VZ];
+            zext<VZ>(ext, q1[4], h);
+            h = ldhalo<VZ>(cur, j + 1, 4, nxx - 4, ldz, zs, lane);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
+                dv[r] = d;
+                float val = leapfrog<ABC>(q1[4][r], o1[r], cj[r], lap8r<VZ, 0>(q1, ext, r), d);
+                if (i == siz && j == six) val += w1;
+                n1[r] = (i >= 4 && i < g.nzz - 4) ? val : o1[r];
+            }
+            if (rec1 && own && j >= x0 && j < x1) {
+                if (staged) sample_staged<VZ>(st, a.nt, n1, dv, j - xa, zb, seis, t + 1 - a.tlag);
+                else sample_receivers<VZ, ABC>(a, n1, j, zb, seis, t + 1 - a.tlag);
+            }
+        } else {
+            h = ldhalo<VZ>(cur, j + 1, 4, nxx - 4, ldz, zs, lane);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) q2[k][r] = q2[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) q2[8][r] = n1[r];
+
+        // ---- step 2: u_{t+2} at column jj = j - 4 (0 on the rim columns)
+        const int jj = j - 4;
+        float n2[VZ];
+        vzero<VZ>(n2);
+        if (jj >= x0 - 4 && jj >= 4 && jj < nxx - 4) {
+            const bool inx = (jj >= g.nb) && (jj < nxx - g.nb);
+            float dxj = 1.f;
+            if constexpr (ABC) {
+                bool in;
+                dxj = staged ? st.dx[jj - xa] : pad_factor(a.b1d, jj, nxx, g.nb, in);
+            }
+            float c2[VZ];
+            vload<VZ>(c2, &cring[slot4][lane * VZ]);
+            float ext[VZ + 8], dv[VZ];
+            zext_local<VZ>(ext, q2[4]);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const float o2 = q1[0][r];              // u_t at column jj
+                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
+                dv[r] = d;
+                float val = leapfrog<ABC>(q2[4][r], o2, c2[r], lap8r<VZ, 0>(q2, ext, r), d);
+                if (i == siz && jj == six) val += w2;
+                n2[r] = (i >= 4 && i < g.nzz - 4) ? val : o2;
+            }
+            if (own && jj >= x0 && jj < x1) {
+                vstore<VZ>(out1 + (long)jj * ldz + zb, n2);
+                if (rec2) {
+                    if (staged) sample_staged<VZ>(st, a.nt, n2, dv, jj - xa, zb, seis, t + 2 - a.tlag);
+                    else sample_receivers<VZ, ABC>(a, n2, jj, zb, seis, t + 2 - a.tlag);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) q3[k][r] = q3[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) q3[8][r] = n2[r];
+
+        // ---- step 3: u_{t+3} at column jjj = j - 8 (an owned column)
+        const int jjj = j - 8;
+        if (jjj >= x0) {
+            const bool inx = (jjj >= g.nb) && (jjj < nxx - g.nb);
+            float dxj = 1.f;
+            if constexpr (ABC) {
+                bool in;
+                dxj = staged ? st.dx[jjj - xa] : pad_factor(a.b1d, jjj, nxx, g.nb, in);
+            }
+            float c3[VZ];
+            vload<VZ>(c3, &cring[slot8][lane * VZ]);
+            float ext[VZ + 8], dv[VZ];
+            zext_local<VZ>(ext, q3[4]);
+            float n3[VZ];
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const float o3 = q2[0][r];              // u_{t+1} at column jjj
+                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
+                dv[r] = d;
+                float val = leapfrog<ABC>(q3[4][r], o3, c3[r], lap8r<VZ, 0>(q3, ext, r), d);
+                if (i == siz && jjj == six) val += w3;
+                n3[r] = (i >= 4 && i < g.nzz - 4) ? val : o3;
+            }
+            if (own) {
+                vstore<VZ>(out2 + (long)jjj * ldz + zb, n3);
+                if (rec3) {
+                    if (staged) sample_staged<VZ>(st, a.nt, n3, dv, jjj - xa, zb, seis, t + 3 - a.tlag);
+                    else sample_receivers<VZ, ABC>(a, n3, jjj, zb, seis, t + 3 - a.tlag);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) q1[k][r] = q1[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) q1[8][r] = pn[r];
+    }
+}
+
 // ---------------------------------------------------------------- reverse
 
 // One reverse step k: S_new = leapfrog(S_cur, S_old) over S_old, R likewise,
@@ -1191,7 +1386,14 @@ hipError_t fwd_dispatch(const FwdStep &a, const Tiling &tl, int nslots, int t, i
 {
     const int ntiles = tl.nzseg * tl.nxchunk * nslots;
     const int nblocks = ((ntiles + 7) >> 3) << 3;
-    if (steps == 2) {
+    if (steps == 3) {
+        if (a.abc)
+            hipLaunchKernelGGL((fwd3_kernel<VZ, true>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+        else
+            hipLaunchKernelGGL((fwd3_kernel<VZ, false>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    } else if (steps == 2) {
         if (a.abc)
             hipLaunchKernelGGL((fwd2_kernel<VZ, true>), dim3(nblocks), dim3(64), 0, s, a,
                                tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
@@ -1238,7 +1440,7 @@ hipError_t rev_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, i
 int zsegments(int nzz, int vz, int steps)
 {
     // waves tile the updated rows 4 .. nzz-5; the rim rows stay zero
-    const int rows = (steps == 2) ? 64 * vz - 8 : 64 * vz;
+    const int rows = 64 * vz - 8 * (steps - 1);
     const int n = (nzz - 8 + rows - 1) / rows;
     return n > 1 ? n : 1;
 }
@@ -1281,9 +1483,13 @@ template <int VZ>
 int resident_vz(int kind, int steps)
 {
     if (kind == 0)
-        return steps == 2 ? resident_of(fwd2_kernel<VZ, false>) : resident_of(fwd1_kernel<VZ, false>);
+        return steps == 3   ? resident_of(fwd3_kernel<VZ, false>)
+               : steps == 2 ? resident_of(fwd2_kernel<VZ, false>)
+                            : resident_of(fwd1_kernel<VZ, false>);
     if (kind == 2)
-        return steps == 2 ? resident_of(fwd2_kernel<VZ, true>) : resident_of(fwd1_kernel<VZ, true>);
+        return steps == 3   ? resident_of(fwd3_kernel<VZ, true>)
+               : steps == 2 ? resident_of(fwd2_kernel<VZ, true>)
+                            : resident_of(fwd1_kernel<VZ, true>);
     return steps == 2 ? resident_of(rev2_kernel<VZ, 0>) : resident_of(rev1_kernel<VZ, 0>);
 }
 }  // namespace

@@ -116,6 +116,21 @@ def test_forward_rbc_matches_oracle():
 
 SHAPES = [(1, 1, 8, 0), (1, 2, 17, 1), (1, 4, 64, 1), (1, 4, 1000, 0), (1, 1, 3, 1),
           (2, 1, 8, 0), (2, 2, 17, 1), (2, 4, 64, 1), (2, 4, 1000, 0), (2, 1, 3, 1), (2, 2, 5, 0)]
+# three steps per launch exist for the forward kernels; the window of a
+# 144-column chunk (160 columns) is the largest that is staged in LDS
+SHAPES3 = [(3, 1, 8, 0), (3, 2, 17, 1), (3, 4, 64, 1), (3, 4, 1000, 0), (3, 1, 3, 1),
+           (3, 2, 5, 0), (3, 2, 144, 1), (3, 1, 145, 0)]
+
+
+def set_shape(eng, st, vz, xw, g):
+    """One launch shape for every kernel; with three steps the reverse kernel
+    takes two."""
+    if st == 3:
+        eng.set_tuning(vz, xw, g, kernel=1, steps=2)
+        for k in (0, 2):
+            eng.set_tuning(vz, xw, g, kernel=k, steps=3)
+    else:
+        eng.set_tuning(vz, xw, g, steps=st)
 
 
 def test_launch_shapes_bit_identical():
@@ -123,10 +138,24 @@ def test_launch_shapes_bit_identical():
     s = overthrust_survey(abc_nb=51, abc_factor=0.004, max_batch=3)
     s.eng.set_tuning(steps=1, vz=4, xw=32, use_graph=0)
     base = s.eng.modeling(F.ABC)
-    for st, vz, xw, g in SHAPES:
-        s.eng.set_tuning(vz, xw, g, steps=st)
+    for st, vz, xw, g in SHAPES + SHAPES3:
+        set_shape(s.eng, st, vz, xw, g)
         out = s.eng.modeling(F.ABC)
         assert np.array_equal(out, base), (st, vz, xw, g)
+
+
+@pytest.mark.parametrize("nt", [998, 999, 1000])
+def test_three_step_remainders_bit_identical(nt):
+    # passes of T = nt + tlag steps with T mod 3 = 0, 1, 2 end in a 3-, 1- or
+    # 2-step launch; the sampled traces equal the one-step kernel's
+    s = overthrust_survey(nt=nt, nsrc=2, abc_nb=51, abc_factor=0.004, rbc_nb=41,
+                          varVp=700.0, seed=4, max_batch=2)
+    s.eng.set_tuning(2, 64, 0, steps=1)
+    base = [s.eng.modeling(F.ABC), s.eng.modeling(F.RBC)]
+    for st, vz, xw, g in [(3, 2, 64, 1), (3, 1, 16, 0), (3, 4, 96, 1)]:
+        set_shape(s.eng, st, vz, xw, g)
+        assert np.array_equal(s.eng.modeling(F.ABC), base[0]), (nt, vz, xw, g)
+        assert np.array_equal(s.eng.modeling(F.RBC), base[1]), (nt, vz, xw, g)
 
 
 def test_reverse_launch_shapes_bit_identical():
@@ -136,8 +165,8 @@ def test_reverse_launch_shapes_bit_identical():
     s.eng.set_observed(obs)
     s.eng.set_model((s.v * 0.98).astype(np.float32))
     results = []
-    for st, vz, xw, g in [(1, 2, 32, 0)] + SHAPES[5:]:
-        s.eng.set_tuning(vz, xw, g, steps=st)
+    for st, vz, xw, g in [(1, 2, 32, 0)] + SHAPES[5:] + SHAPES3[:3]:
+        set_shape(s.eng, st, vz, xw, g)
         s.eng.reset()
         s.eng.migration()
         img = s.eng.get_image()
@@ -208,15 +237,15 @@ def test_dense_receiver_column_matches_oracle():
         print(f"shot {k} seismogram relL2 {err:.3e}")
         assert err <= SEIS_TOL
         obs.append(ref)
-    for st in (1, 2):                       # windows wider than the table: global path
-        s.eng.set_tuning(2, 200, 0, steps=st)
+    for st in (1, 2, 3):                    # windows wider than the table: global path
+        set_shape(s.eng, st, 2, 200, 0)
         assert np.array_equal(s.eng.modeling(F.ABC), seis), st
     obs = np.stack(obs).astype(np.float32)
     s.eng.set_observed(obs)
     rref, rraw, rill = migration_oracle(s, obs)
     first = None
-    for st, vz, xw in [(2, 2, 64), (1, 1, 8), (2, 4, 128), (1, 2, 200)]:
-        s.eng.set_tuning(vz, xw, 0, steps=st)
+    for st, vz, xw in [(2, 2, 64), (1, 1, 8), (2, 4, 128), (1, 2, 200), (3, 2, 64), (3, 4, 200)]:
+        set_shape(s.eng, st, vz, xw, 0)
         s.eng.reset()
         s.eng.migration()
         img, raw, ill = s.eng.get_image()
@@ -301,3 +330,10 @@ def test_errors_are_reported():
     with pytest.raises(F.FwiError) as e:
         eng.set_model(np.full((201, 81), -1.0, np.float32))
     assert e.value.code == 5
+    for kernel in (-1, 1):            # three steps per launch: forward kernels only
+        with pytest.raises(F.FwiError):
+            eng.set_tuning(kernel=kernel, steps=3)
+    with pytest.raises(F.FwiError):
+        eng.set_tuning(kernel=0, steps=4)
+    eng.set_tuning(kernel=2, steps=3)
+    assert eng.get_tuning(2)[0] == 3
