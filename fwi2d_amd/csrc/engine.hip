@@ -208,8 +208,8 @@ fwi::RevStep rev_args(fwi_ctx *c, int mode)
     return a;
 }
 
-// Enqueues the launches of one pass over T steps -- two steps per launch
-// when the tuning asks for it, a single step for an odd remainder -- or
+// Enqueues the launches of one pass over T steps -- the tuned number of
+// steps per launch (1-3), shorter launches for the remainder -- or
 // replays the captured graph of the same sequence.  `sq` is the source (or
 // forward) wavefield, `rq` the receiver wavefield of a reverse pass; both
 // are advanced to their final state.
@@ -248,11 +248,11 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
                 fwi::RevStep a = *ra;
                 a.scur = s.b[s.cur];
                 a.sold = s.b[s.old];
-                a.sout1 = (n == 2) ? s.b[f1] : s.b[s.old];
+                a.sout1 = (n >= 2) ? s.b[f1] : s.b[s.old];
                 a.sout2 = s.b[f2];
                 a.rcur = r.b[r.cur];
                 a.rold = r.b[r.old];
-                a.rout1 = (n == 2) ? r.b[h1] : r.b[r.old];
+                a.rout1 = (n >= 2) ? r.b[h1] : r.b[r.old];
                 a.rout2 = r.b[h2];
                 if (launch) e = fwi::launch_rev_step(a, tls[n], nslots, t, n, c->stream);
             }
@@ -936,13 +936,19 @@ int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_
     if (kernel < -1 || kernel > 2) return fail(FWI_ERR_ARG, "kernel must be -1, 0, 1 or 2");
     if (vz > 0 && vz != 1 && vz != 2 && vz != 4) return fail(FWI_ERR_ARG, "vz must be 1, 2 or 4");
     if (steps > 3) return fail(FWI_ERR_ARG, "steps per launch must be 1, 2 or 3");
-    if (steps == 3 && kernel != 0 && kernel != 2)
-        return fail(FWI_ERR_ARG, "three steps per launch exist for the forward kernels only");
+    int ns[3], vs[3], ws[3];
     for (int k = 0; k < 3; ++k) {
-        if (kernel != -1 && kernel != k) continue;
-        if (steps > 0) c->nsteps[k] = steps;
-        if (vz > 0) c->vz[k] = vz;
-        if (xw > 0) c->xw[k] = xw;
+        const bool sel = kernel == -1 || kernel == k;
+        ns[k] = (sel && steps > 0) ? steps : c->nsteps[k];
+        vs[k] = (sel && vz > 0) ? vz : c->vz[k];
+        ws[k] = (sel && xw > 0) ? xw : c->xw[k];
+    }
+    if (ns[1] == 3 && vs[1] == 4)
+        return fail(FWI_ERR_ARG, "the three-step reverse kernel takes 1 or 2 rows per lane");
+    for (int k = 0; k < 3; ++k) {
+        c->nsteps[k] = ns[k];
+        c->vz[k] = vs[k];
+        c->xw[k] = ws[k];
     }
     if (use_graph >= 0) c->use_graph = use_graph ? 1 : 0;
     drop_graphs(c);
@@ -987,8 +993,9 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
         const long cells = (long)nslots * gb.g.nzz * gb.g.nxx;
         const int ncols = gb.g.nxx - 8;
         int bst = c->nsteps[kind], bvz = c->vz[kind], bxw = c->xw[kind];
-        for (int st = 1; st <= (kind == 1 ? 2 : 3); ++st)
+        for (int st = 1; st <= 3; ++st)
             for (int a = 0; a < 3; ++a) {
+                if (kind == 1 && st == 3 && vzs[a] == 4) continue;
                 // fixed widths, narrow ones for small batches (more waves in
                 // flight), and widths that make the launch about 1-4 full
                 // rounds of the waves the device holds at once
@@ -1015,9 +1022,9 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
                     fa.out1 = (st >= 2) ? c->d_f[2] : c->d_f[0]; fa.out2 = c->d_f[3];
                     fwi::RevStep ra = rev_args(c, 0);
                     ra.scur = c->d_f[1]; ra.sold = c->d_f[0];
-                    ra.sout1 = (st == 2) ? c->d_f[2] : c->d_f[0]; ra.sout2 = c->d_f[3];
+                    ra.sout1 = (st >= 2) ? c->d_f[2] : c->d_f[0]; ra.sout2 = c->d_f[3];
                     ra.rcur = c->d_f[5]; ra.rold = c->d_f[4];
-                    ra.rout1 = (st == 2) ? c->d_f[6] : c->d_f[4]; ra.rout2 = c->d_f[7];
+                    ra.rout1 = (st >= 2) ? c->d_f[6] : c->d_f[4]; ra.rout2 = c->d_f[7];
                     auto launch = [&]() {
                         return kind != 1
                                    ? fwi::launch_fwd_step(fa, tl, nslots, tstep, st, c->stream)

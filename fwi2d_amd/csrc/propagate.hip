@@ -82,6 +82,12 @@ __device__ __forceinline__ void vzero(float (&r)[VZ])
     for (int k = 0; k < VZ; ++k) r[k] = 0.f;
 }
 
+// Compile-time queue phase of an unrolled column loop.
+template <int N>
+struct Phase {
+    static constexpr int value = N;
+};
+
 // Whole-wave lane shifts (GFX9 DPP wave_shr:1 / wave_shl:1): lane i takes
 // lane i-1's (shr) or lane i+1's (shl) value; the lane with no source (0 or
 // 63) takes `edge`.  Plain VALU moves, no LDS traffic.
@@ -292,6 +298,30 @@ __device__ __forceinline__ void ldcol(float (&r)[VZ], const float *base, int col
     else vzero<VZ>(r);
 }
 
+// Loads for the unrolled three-step kernels, with no branch: the column is
+// clamped into the grid, so a column beyond the edge reads the edge column,
+// which is a zero rim column of every wavefield (sources and receivers sit
+// inside the rim).  Such values only feed rim columns.
+template <int VZ>
+__device__ __forceinline__ void ldcolc(float (&r)[VZ], const float *base, int col, int nxx,
+                                       int ldz, int zb)
+{
+    col = min(max(col, 0), nxx - 1);
+    vload<VZ>(r, base + (long)col * ldz + zb);
+}
+// The halo rows of column col clamped into [4, nxx-4), the columns whose
+// values it feeds.  The first row segment starts at z0 = -4, so its rows
+// z0-4 .. z0-1 lie in the column before: column 4 keeps that read inside
+// the slot's field (a column below 4 would not).
+template <int VZ>
+__device__ __forceinline__ float4 ldhaloc(const float *base, int col, int nxx, int ldz, int z0,
+                                          int lane)
+{
+    col = min(max(col, 4), nxx - 5);
+    return *reinterpret_cast<const float4 *>(base + (long)col * ldz +
+                                             (lane < 32 ? z0 - 4 : z0 + 64 * VZ));
+}
+
 // Receivers in column j whose row is in this lane: seis[rec][tsamp] = d * v[row].
 template <int VZ, bool ABC>
 __device__ __forceinline__ void sample_receivers(const FwdStep &a, const float (&v)[VZ], int j,
@@ -343,8 +373,7 @@ struct FwdStage {
 struct RevStage {
     int cp[kStageCols + 1];
     int iz[kStageRec];
-    float v1[kStageRec];        // data row of the first step (rows in the wave)
-    float v2[kStageRec];        // data row of the second step
+    float v[3][kStageRec];      // data rows of the launch's steps 1-3 (rows in the wave)
 };
 
 // Column offsets and rows of the window's receivers; returns false (for the
@@ -385,21 +414,23 @@ __device__ __forceinline__ bool stage_fwd(const FwdStep &a, bool rec, bool abc, 
 // Rows [zlo, zhi) are the wave's; data of other rows is never read.
 __device__ __forceinline__ bool stage_rev(const RevStep &a, const float *dat, int xa, int nc,
                                           int zlo, int zhi, bool on1, int trow1, bool on2,
-                                          int trow2, RevStage &st, int lane)
+                                          int trow2, bool on3, int trow3, RevStage &st, int lane)
 {
     int e0 = 0, ne = 0;
-    const bool ok = (on1 || on2) && stage_table(a.rec, xa, nc, a.g.nxx, st, lane, e0, ne);
+    const bool ok = (on1 || on2 || on3) && stage_table(a.rec, xa, nc, a.g.nxx, st, lane, e0, ne);
     if (ok) {
         for (int e = lane; e < ne; e += 64) {
             const int iz = st.iz[e];        // written by this lane
-            float v1 = 0.f, v2 = 0.f;
+            float v1 = 0.f, v2 = 0.f, v3 = 0.f;
             if (iz >= zlo && iz < zhi) {
                 const float *row = dat + (long)a.rec.id[e0 + e] * a.nt;
                 if (on1) v1 = row[trow1];
                 if (on2) v2 = row[trow2];
+                if (on3) v3 = row[trow3];
             }
-            st.v1[e] = v1;
-            st.v2[e] = v2;
+            st.v[0][e] = v1;
+            st.v[1][e] = v2;
+            st.v[2][e] = v3;
         }
     }
     __syncthreads();
@@ -423,14 +454,14 @@ __device__ __forceinline__ void sample_staged(const FwdStage &st, int nt, const 
 
 // Adds idh2 * data to the receivers of window column c in this lane, in
 // table order.
-template <int VZ, bool SECOND>
+template <int VZ, int STEP>
 __device__ __forceinline__ void inject_staged(const RevStage &st, float idh2, float (&v)[VZ], int c,
                                               int zb)
 {
     const int m0 = st.cp[c], m1 = st.cp[c + 1];
     for (int m = m0; m < m1; ++m) {
         const int rr = st.iz[m] - zb;
-        const float val = SECOND ? st.v2[m] : st.v1[m];
+        const float val = st.v[STEP][m];
 #pragma unroll
         for (int r = 0; r < VZ; ++r)
             if (rr == r) v[r] = fmaf(idh2, val, v[r]);
@@ -741,41 +772,40 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
 #pragma unroll
     for (int r = 0; r < VZ; ++r) dzr[r] = ABC ? pad_factor(a.b1d, zb + r, g.nzz, g.nb, inz[r]) : 1.f;
 
-    float q1[9][VZ];     // u_t,     columns j-4 .. j+4
-    float q2[9][VZ];     // u_{t+1}, columns j-8 .. j
-    float q3[9][VZ];     // u_{t+2}, columns j-12 .. j-4
+    float q1[9][VZ];     // u_t
+    float q2[9][VZ];     // u_{t+1}
+    float q3[9][VZ];     // u_{t+2}
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        ldcol<VZ>(q1[k], cur, x0 - 12 + k, nxx, ldz, zl);
-        vzero<VZ>(q2[k]);
-        vzero<VZ>(q3[k]);
+    for (int m = 0; m < 9; ++m) {
+        ldcolc<VZ>(q1[m], cur, x0 - 12 + m, nxx, ldz, zl);
+        vzero<VZ>(q2[m]);
+        vzero<VZ>(q3[m]);
     }
     float on[VZ], cn[VZ], pn[VZ];
-    ldcol<VZ>(on, old, x0 - 8, nxx, ldz, zl);
-    ldcol<VZ>(cn, cc, x0 - 8, nxx, ldz, zl);
-    vzero<VZ>(pn);
-    float4 h = ldhalo<VZ>(cur, x0 - 8, 4, nxx - 4, ldz, zs, lane);
-    // c of the last nine columns (each lane its own rows), for steps 2 and 3
-    __shared__ float cring[9][64 * VZ];
-    int slot = 0;                                    // slot of column j
+    ldcolc<VZ>(on, old, x0 - 8, nxx, ldz, zl);
+    ldcolc<VZ>(cn, cc, x0 - 8, nxx, ldz, zl);
+    float4 h = ldhaloc<VZ>(cur, x0 - 8, nxx, ldz, zs, lane);
+    __shared__ float cring[9][64 * VZ];              // c of the last nine columns
+    const int jend = x1 + 8;
+    int j = x0 - 8;
 
-    for (int j = x0 - 8; j < x1 + 8; ++j) {
+    // One column j; P is the phase of the unrolled loop: slot (m + P) % 9 of
+    // a queue holds its column m (q1: j-4+m; q2, cring: j-8+m; q3: j-12+m
+    // once the column's results are in).  Steps 1 and 2 run on every column:
+    // a column outside [4, nxx-4) takes its old value, a zero rim value.
+    auto column = [&](auto phase) {
+        constexpr int P = decltype(phase)::value;
+        constexpr int C0 = P % 9, C4 = (P + 4) % 9, C8 = (P + 8) % 9;
         float o1[VZ], cj[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
-        vstore<VZ>(&cring[slot][lane * VZ], cj);
-        const int slot4 = (slot >= 4) ? slot - 4 : slot + 5;     // column j - 4
-        const int slot8 = (slot >= 8) ? slot - 8 : slot + 1;     // column j - 8
-        slot = (slot == 8) ? 0 : slot + 1;
-        if (j + 1 < x1 + 8) {
-            ldcol<VZ>(pn, cur, j + 5, nxx, ldz, zl);
-            ldcol<VZ>(on, old, j + 1, nxx, ldz, zl);
-            ldcol<VZ>(cn, cc, j + 1, nxx, ldz, zl);
-        }
-        // ---- step 1: u_{t+1} at column j (0 on the rim columns)
-        float n1[VZ];
-        vzero<VZ>(n1);
-        if (j >= 4 && j < nxx - 4) {
+        vstore<VZ>(&cring[C8][lane * VZ], cj);
+        ldcolc<VZ>(pn, cur, j + 5, nxx, ldz, zl);
+        ldcolc<VZ>(on, old, j + 1, nxx, ldz, zl);
+        ldcolc<VZ>(cn, cc, j + 1, nxx, ldz, zl);
+        // ---- step 1: u_{t+1} at column j
+        {
+            const bool jin = (j >= 4) && (j < nxx - 4);
             const bool inx = (j >= g.nb) && (j < nxx - g.nb);
             float dxj = 1.f;
             if constexpr (ABC) {
@@ -783,71 +813,53 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
                 dxj = staged ? st.dx[j - xa] : pad_factor(a.b1d, j, nxx, g.nb, in);
             }
             float ext[VZ + 8], dv[VZ];
-            zext<VZ>(ext, q1[4], h);
-            h = ldhalo<VZ>(cur, j + 1, 4, nxx - 4, ldz, zs, lane);
+            zext<VZ>(ext, q1[C4], h);
+            h = ldhaloc<VZ>(cur, j + 1, nxx, ldz, zs, lane);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
                 dv[r] = d;
-                float val = leapfrog<ABC>(q1[4][r], o1[r], cj[r], lap8r<VZ, 0>(q1, ext, r), d);
+                float val = leapfrog<ABC>(q1[C4][r], o1[r], cj[r], lap8r<VZ, C0>(q1, ext, r), d);
                 if (i == siz && j == six) val += w1;
-                n1[r] = (i >= 4 && i < g.nzz - 4) ? val : o1[r];
+                q2[C8][r] = (jin && i >= 4 && i < g.nzz - 4) ? val : o1[r];
             }
             if (rec1 && own && j >= x0 && j < x1) {
-                if (staged) sample_staged<VZ>(st, a.nt, n1, dv, j - xa, zb, seis, t + 1 - a.tlag);
-                else sample_receivers<VZ, ABC>(a, n1, j, zb, seis, t + 1 - a.tlag);
+                if (staged) sample_staged<VZ>(st, a.nt, q2[C8], dv, j - xa, zb, seis, t + 1 - a.tlag);
+                else sample_receivers<VZ, ABC>(a, q2[C8], j, zb, seis, t + 1 - a.tlag);
             }
-        } else {
-            h = ldhalo<VZ>(cur, j + 1, 4, nxx - 4, ldz, zs, lane);
         }
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) q2[k][r] = q2[k + 1][r];
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) q2[8][r] = n1[r];
-
-        // ---- step 2: u_{t+2} at column jj = j - 4 (0 on the rim columns)
-        const int jj = j - 4;
-        float n2[VZ];
-        vzero<VZ>(n2);
-        if (jj >= x0 - 4 && jj >= 4 && jj < nxx - 4) {
+        // ---- step 2: u_{t+2} at column jj = j - 4 (columns before x0 - 4 are not used)
+        {
+            const int jj = j - 4;
+            const bool jin = (jj >= 4) && (jj < nxx - 4);
             const bool inx = (jj >= g.nb) && (jj < nxx - g.nb);
             float dxj = 1.f;
             if constexpr (ABC) {
                 bool in;
-                dxj = staged ? st.dx[jj - xa] : pad_factor(a.b1d, jj, nxx, g.nb, in);
+                dxj = staged ? st.dx[max(jj - xa, 0)] : pad_factor(a.b1d, jj, nxx, g.nb, in);
             }
-            float c2[VZ];
-            vload<VZ>(c2, &cring[slot4][lane * VZ]);
             float ext[VZ + 8], dv[VZ];
-            zext_local<VZ>(ext, q2[4]);
+            zext_local<VZ>(ext, q2[C4]);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
-                const float o2 = q1[0][r];              // u_t at column jj
+                const float o2 = q1[C0][r];             // u_t at column jj
+                const float c2 = cring[C4][lane * VZ + r];
                 const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
                 dv[r] = d;
-                float val = leapfrog<ABC>(q2[4][r], o2, c2[r], lap8r<VZ, 0>(q2, ext, r), d);
+                float val = leapfrog<ABC>(q2[C4][r], o2, c2, lap8r<VZ, C0>(q2, ext, r), d);
                 if (i == siz && jj == six) val += w2;
-                n2[r] = (i >= 4 && i < g.nzz - 4) ? val : o2;
+                q3[C8][r] = (jin && i >= 4 && i < g.nzz - 4) ? val : o2;
             }
             if (own && jj >= x0 && jj < x1) {
-                vstore<VZ>(out1 + (long)jj * ldz + zb, n2);
+                vstore<VZ>(out1 + (long)jj * ldz + zb, q3[C8]);
                 if (rec2) {
-                    if (staged) sample_staged<VZ>(st, a.nt, n2, dv, jj - xa, zb, seis, t + 2 - a.tlag);
-                    else sample_receivers<VZ, ABC>(a, n2, jj, zb, seis, t + 2 - a.tlag);
+                    if (staged) sample_staged<VZ>(st, a.nt, q3[C8], dv, jj - xa, zb, seis, t + 2 - a.tlag);
+                    else sample_receivers<VZ, ABC>(a, q3[C8], jj, zb, seis, t + 2 - a.tlag);
                 }
             }
         }
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) q3[k][r] = q3[k + 1][r];
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) q3[8][r] = n2[r];
-
         // ---- step 3: u_{t+3} at column jjj = j - 8 (an owned column)
         const int jjj = j - 8;
         if (jjj >= x0) {
@@ -858,17 +870,17 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
                 dxj = staged ? st.dx[jjj - xa] : pad_factor(a.b1d, jjj, nxx, g.nb, in);
             }
             float c3[VZ];
-            vload<VZ>(c3, &cring[slot8][lane * VZ]);
+            vload<VZ>(c3, &cring[C0][lane * VZ]);
             float ext[VZ + 8], dv[VZ];
-            zext_local<VZ>(ext, q3[4]);
+            zext_local<VZ>(ext, q3[C4]);
             float n3[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
-                const float o3 = q2[0][r];              // u_{t+1} at column jjj
+                const float o3 = q2[C0][r];             // u_{t+1} at column jjj
                 const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
                 dv[r] = d;
-                float val = leapfrog<ABC>(q3[4][r], o3, c3[r], lap8r<VZ, 0>(q3, ext, r), d);
+                float val = leapfrog<ABC>(q3[C4][r], o3, c3[r], lap8r<VZ, C0>(q3, ext, r), d);
                 if (i == siz && jjj == six) val += w3;
                 n3[r] = (i >= 4 && i < g.nzz - 4) ? val : o3;
             }
@@ -880,12 +892,20 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
                 }
             }
         }
+        // column j + 5 of u_t replaces column j - 4
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-#pragma unroll
-            for (int r = 0; r < VZ; ++r) q1[k][r] = q1[k + 1][r];
-#pragma unroll
-        for (int r = 0; r < VZ; ++r) q1[8][r] = pn[r];
+        for (int r = 0; r < VZ; ++r) q1[C0][r] = pn[r];
+    };
+    for (;;) {
+        column(Phase<0>{}); if (++j >= jend) break;
+        column(Phase<1>{}); if (++j >= jend) break;
+        column(Phase<2>{}); if (++j >= jend) break;
+        column(Phase<3>{}); if (++j >= jend) break;
+        column(Phase<4>{}); if (++j >= jend) break;
+        column(Phase<5>{}); if (++j >= jend) break;
+        column(Phase<6>{}); if (++j >= jend) break;
+        column(Phase<7>{}); if (++j >= jend) break;
+        column(Phase<8>{}); if (++j >= jend) break;
     }
 }
 
@@ -924,7 +944,7 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
     const int zw = 4 + id.zseg * 64 * VZ;
     __shared__ RevStage st;
     const bool staged = stage_rev(a, dat, j0, j1 - j0, zw, zw + 64 * VZ, inj_on, a.nt - 2 - k,
-                                  false, 0, st, lane);
+                                  false, 0, false, 0, st, lane);
 
     float qs[9][VZ], qr[9][VZ];
 #pragma unroll
@@ -980,7 +1000,7 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
             }
         }
         if (inj_on) {
-            if (staged) inject_staged<VZ, false>(st, a.idh2, nr, j - j0, zb);
+            if (staged) inject_staged<VZ, 0>(st, a.idh2, nr, j - j0, zb);
             else inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
         }
         if (active) {
@@ -1036,7 +1056,7 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
     const int xa = x0 - 4;                           // window [x0 - 4, x1 + 4)
     __shared__ RevStage st;
     const bool staged = stage_rev(a, dat, xa, x1 + 4 - xa, zs, zs + 64 * VZ, inj1, a.nt - 2 - k,
-                                  inj2, a.nt - 3 - k, st, lane);
+                                  inj2, a.nt - 3 - k, false, 0, st, lane);
 
     float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur: columns j-4 .. j+4
     float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R: columns j-8 .. j
@@ -1098,7 +1118,7 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj1) {
-                if (staged) inject_staged<VZ, false>(st, a.idh2, nr, j - xa, zb);
+                if (staged) inject_staged<VZ, 0>(st, a.idh2, nr, j - xa, zb);
                 else inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
             }
             if (own && j >= x0 && j < x1) {
@@ -1152,7 +1172,7 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj2) {
-                if (staged) inject_staged<VZ, true>(st, a.idh2, nr2, jj - xa, zb);
+                if (staged) inject_staged<VZ, 1>(st, a.idh2, nr2, jj - xa, zb);
                 else inject_receivers<VZ>(a, nr2, jj, zb, dat, a.nt - 3 - k);
             }
             if (own) {
@@ -1169,6 +1189,218 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
             for (int r = 0; r < VZ; ++r) { qs1[m][r] = qs1[m + 1][r]; qr1[m][r] = qr1[m + 1][r]; }
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { qs1[8][r] = psn[r]; qr1[8][r] = prn[r]; }
+    }
+}
+
+// Three reverse steps per launch, tiled like fwd3_kernel, for both
+// wavefields.  Image and illumination of a column are read when step 1
+// reaches it, take the three steps' rounded products in step order in a
+// 9-column queue of partial sums, and are written once when step 3 leaves it.
+template <int VZ, int MODE>
+__global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int k)
+{
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int ldz = g.ldz, nxx = g.nxx;
+    const int zs = id.zseg * (64 * VZ - 16) - 4;    // first computed row; owned from zs + 8
+    const int zb = zs + lane * VZ;
+    const int zl = zb < 0 ? 0 : zb;                  // load row: rows above 0 feed rim rows only
+    const bool active = zb < ldz;
+    const bool own = active && (zb >= zs + 8) && (zb < zs + 64 * VZ - 8);
+    const int x0 = 4 + id.xc * xw;
+    const int x1 = min(x0 + xw, nxx - 4);
+
+    const long soff = (long)id.slot * g.fstride;
+    const float *scur = a.scur + soff;
+    const float *sold = a.sold + soff;
+    float *sout1 = a.sout1 + soff;
+    float *sout2 = a.sout2 + soff;
+    const float *rcur = a.rcur + soff;
+    const float *rold = a.rold + soff;
+    float *rout1 = a.rout1 + soff;
+    float *rout2 = a.rout2 + soff;
+    const float *cc = a.c + (long)id.slot * a.cstride;
+    float *acc = a.acc + soff;
+    float *ill = a.ill + soff;
+    const bool inj1 = (k + 1 < a.nt);
+    const bool inj2 = (k + 2 < a.nt);
+    const bool inj3 = (k + 3 < a.nt);
+    const int drow = a.data_by_shot ? (*a.shot0 + id.slot) : id.slot;
+    const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
+    const int xa = x0 - 8;                           // window [x0 - 8, x1 + 8)
+    __shared__ RevStage st;
+    const bool staged = stage_rev(a, dat, xa, x1 + 8 - xa, zs, zs + 64 * VZ, inj1, a.nt - 2 - k,
+                                  inj2, a.nt - 3 - k, inj3, a.nt - 4 - k, st, lane);
+
+    float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur
+    float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R
+    float qs3[9][VZ], qr3[9][VZ];    // step-2 S, R
+    float pa[9][VZ], pi[9][VZ];      // image / illumination partial sums
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        ldcolc<VZ>(qs1[m], scur, x0 - 12 + m, nxx, ldz, zl);
+        ldcolc<VZ>(qr1[m], rcur, x0 - 12 + m, nxx, ldz, zl);
+        vzero<VZ>(qs2[m]); vzero<VZ>(qr2[m]);
+        vzero<VZ>(qs3[m]); vzero<VZ>(qr3[m]);
+        vzero<VZ>(pa[m]); vzero<VZ>(pi[m]);
+    }
+    float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ], an[VZ], in[VZ];
+    ldcolc<VZ>(osn, sold, x0 - 8, nxx, ldz, zl);
+    ldcolc<VZ>(orn, rold, x0 - 8, nxx, ldz, zl);
+    ldcolc<VZ>(cn, cc, x0 - 8, nxx, ldz, zl);
+    vzero<VZ>(an); vzero<VZ>(in);
+    float4 hs = ldhaloc<VZ>(scur, x0 - 8, nxx, ldz, zs, lane);
+    float4 hr = ldhaloc<VZ>(rcur, x0 - 8, nxx, ldz, zs, lane);
+    __shared__ float cring[9][64 * VZ];              // c of the last nine columns
+    const int jend = x1 + 8;                         // window [x0 - 8, x1 + 8)
+    int j = x0 - 8;
+
+    // One column j.  P is the phase of the unrolled loop: slot (m + P) % 9 of
+    // a queue holds its column m (q1: j-4+m; q2, pa, pi: j-8+m; q3: j-12+m
+    // once the column's results are in; cring: j-8+m), so the queues advance
+    // without register moves.  Steps 1 and 2 run on every column: a column
+    // outside [4, nxx-4) takes its old value, which is a zero rim value.
+    auto column = [&](auto phase) {
+        constexpr int P = decltype(phase)::value;
+        constexpr int C0 = P % 9, C4 = (P + 4) % 9, C8 = (P + 8) % 9;
+        float os[VZ], orr[VZ], cj[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) {
+            os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r];
+            pa[C8][r] = an[r]; pi[C8][r] = in[r];
+        }
+        vstore<VZ>(&cring[C8][lane * VZ], cj);
+        ldcolc<VZ>(psn, scur, j + 5, nxx, ldz, zl);
+        ldcolc<VZ>(prn, rcur, j + 5, nxx, ldz, zl);
+        ldcolc<VZ>(osn, sold, j + 1, nxx, ldz, zl);
+        ldcolc<VZ>(orn, rold, j + 1, nxx, ldz, zl);
+        ldcolc<VZ>(cn, cc, j + 1, nxx, ldz, zl);
+        if (j + 1 >= x0 && j + 1 < x1) {             // image / illumination of owned columns
+            vload<VZ>(an, acc + (long)(j + 1) * ldz + zl);
+            vload<VZ>(in, ill + (long)(j + 1) * ldz + zl);
+        }
+        // ---- step 1 at column j
+        const bool jin1 = (j >= 4) && (j < nxx - 4);
+        {
+            float es[VZ + 8], er[VZ + 8];
+            zext<VZ>(es, qs1[C4], hs);
+            zext<VZ>(er, qr1[C4], hr);
+            hs = ldhaloc<VZ>(scur, j + 1, nxx, ldz, zs, lane);
+            hr = ldhaloc<VZ>(rcur, j + 1, nxx, ldz, zs, lane);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = jin1 && (i >= 4) && (i < g.nzz - 4);
+                const float ps = qs1[C4][r], pr = qr1[C4][r];
+                const float lapS = lap8r<VZ, C0>(qs1, es, r);
+                const float sn = fmaf(cj[r], lapS, ps + (ps - os[r]));
+                const float rn = fmaf(cj[r], lap8r<VZ, C0>(qr1, er, r), pr + (pr - orr[r]));
+                qs2[C8][r] = inside ? sn : os[r];
+                qr2[C8][r] = inside ? rn : orr[r];
+                if (inside) {
+                    pi[C8][r] = add_nc(pi[C8][r], mul_nc(sn, sn));
+                    pa[C8][r] = add_nc(pa[C8][r], (MODE == 0) ? mul_nc(sn, pr)
+                                                              : mul_nc(pr, mul_nc(cj[r], lapS)));
+                }
+            }
+            if (inj1 && jin1) {
+                if (staged) inject_staged<VZ, 0>(st, a.idh2, qr2[C8], j - xa, zb);
+                else inject_receivers<VZ>(a, qr2[C8], j, zb, dat, a.nt - 2 - k);
+            }
+        }
+        // ---- step 2 at column jj = j - 4 (columns before x0 - 4 are not used)
+        const int jj = j - 4;
+        const bool jin2 = (jj >= 4) && (jj < nxx - 4);
+        {
+            float es[VZ + 8], er[VZ + 8];
+            zext_local<VZ>(es, qs2[C4]);
+            zext_local<VZ>(er, qr2[C4]);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = jin2 && (i >= 4) && (i < g.nzz - 4);
+                const float c2 = cring[C4][lane * VZ + r];
+                const float os2 = qs1[C0][r];           // S_cur at column jj
+                const float or2 = qr1[C0][r];           // R_cur at column jj
+                const float ps = qs2[C4][r], pr = qr2[C4][r];
+                const float lapS = lap8r<VZ, C0>(qs2, es, r);
+                const float sn = fmaf(c2, lapS, ps + (ps - os2));
+                const float rn = fmaf(c2, lap8r<VZ, C0>(qr2, er, r), pr + (pr - or2));
+                qs3[C8][r] = inside ? sn : os2;
+                qr3[C8][r] = inside ? rn : or2;
+                if (inside) {
+                    pi[C4][r] = add_nc(pi[C4][r], mul_nc(sn, sn));
+                    pa[C4][r] = add_nc(pa[C4][r], (MODE == 0) ? mul_nc(sn, pr)
+                                                              : mul_nc(pr, mul_nc(c2, lapS)));
+                }
+            }
+            if (inj2 && jin2 && jj >= x0 - 4) {
+                if (staged) inject_staged<VZ, 1>(st, a.idh2, qr3[C8], jj - xa, zb);
+                else inject_receivers<VZ>(a, qr3[C8], jj, zb, dat, a.nt - 3 - k);
+            }
+            if (own && jj >= x0 && jj < x1) {
+                vstore<VZ>(sout1 + (long)jj * ldz + zb, qs3[C8]);
+                vstore<VZ>(rout1 + (long)jj * ldz + zb, qr3[C8]);
+            }
+        }
+        // ---- step 3 at column jjj = j - 8 (an owned column)
+        const int jjj = j - 8;
+        if (jjj >= x0) {
+            float c3[VZ];
+            vload<VZ>(c3, &cring[C0][lane * VZ]);
+            float es[VZ + 8], er[VZ + 8];
+            zext_local<VZ>(es, qs3[C4]);
+            zext_local<VZ>(er, qr3[C4]);
+            float ns3[VZ], nr3[VZ], fa[VZ], fi[VZ];
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = (i >= 4) && (i < g.nzz - 4);
+                const float os3 = qs2[C0][r];           // step-1 S at column jjj
+                const float or3 = qr2[C0][r];           // step-1 R at column jjj
+                const float ps = qs3[C4][r], pr = qr3[C4][r];
+                const float lapS = lap8r<VZ, C0>(qs3, es, r);
+                const float sn = fmaf(c3[r], lapS, ps + (ps - os3));
+                const float rn = fmaf(c3[r], lap8r<VZ, C0>(qr3, er, r), pr + (pr - or3));
+                ns3[r] = inside ? sn : os3;
+                nr3[r] = inside ? rn : or3;
+                fi[r] = pi[C0][r];
+                fa[r] = pa[C0][r];
+                if (inside) {
+                    fi[r] = add_nc(fi[r], mul_nc(sn, sn));
+                    fa[r] = add_nc(fa[r], (MODE == 0) ? mul_nc(sn, pr)
+                                                      : mul_nc(pr, mul_nc(c3[r], lapS)));
+                }
+            }
+            if (inj3) {
+                if (staged) inject_staged<VZ, 2>(st, a.idh2, nr3, jjj - xa, zb);
+                else inject_receivers<VZ>(a, nr3, jjj, zb, dat, a.nt - 4 - k);
+            }
+            if (own) {
+                const long off = (long)jjj * ldz + zb;
+                vstore<VZ>(sout2 + off, ns3);
+                vstore<VZ>(rout2 + off, nr3);
+                vstore<VZ>(acc + off, fa);
+                vstore<VZ>(ill + off, fi);
+            }
+        }
+        // column j + 5 of S_cur, R_cur replaces column j - 4
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { qs1[C0][r] = psn[r]; qr1[C0][r] = prn[r]; }
+    };
+    for (;;) {
+        column(Phase<0>{}); if (++j >= jend) break;
+        column(Phase<1>{}); if (++j >= jend) break;
+        column(Phase<2>{}); if (++j >= jend) break;
+        column(Phase<3>{}); if (++j >= jend) break;
+        column(Phase<4>{}); if (++j >= jend) break;
+        column(Phase<5>{}); if (++j >= jend) break;
+        column(Phase<6>{}); if (++j >= jend) break;
+        column(Phase<7>{}); if (++j >= jend) break;
+        column(Phase<8>{}); if (++j >= jend) break;
     }
 }
 
@@ -1417,7 +1649,16 @@ hipError_t rev_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, i
 {
     const int ntiles = tl.nzseg * tl.nxchunk * nslots;
     const int nblocks = ((ntiles + 7) >> 3) << 3;
-    if (steps == 2) {
+    if (steps == 3) {
+        // six register queues per lane: four rows per lane do not fit
+        if constexpr (VZ == 4) return hipErrorInvalidValue;
+        else if (a.mode == 0)
+            hipLaunchKernelGGL((rev3_kernel<VZ, 0>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+        else
+            hipLaunchKernelGGL((rev3_kernel<VZ, 1>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+    } else if (steps == 2) {
         if (a.mode == 0)
             hipLaunchKernelGGL((rev2_kernel<VZ, 0>), dim3(nblocks), dim3(64), 0, s, a,
                                tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
@@ -1490,6 +1731,8 @@ int resident_vz(int kind, int steps)
         return steps == 3   ? resident_of(fwd3_kernel<VZ, true>)
                : steps == 2 ? resident_of(fwd2_kernel<VZ, true>)
                             : resident_of(fwd1_kernel<VZ, true>);
+    if constexpr (VZ != 4)
+        if (steps == 3) return resident_of(rev3_kernel<VZ, 0>);
     return steps == 2 ? resident_of(rev2_kernel<VZ, 0>) : resident_of(rev1_kernel<VZ, 0>);
 }
 }  // namespace

@@ -116,16 +116,16 @@ def test_forward_rbc_matches_oracle():
 
 SHAPES = [(1, 1, 8, 0), (1, 2, 17, 1), (1, 4, 64, 1), (1, 4, 1000, 0), (1, 1, 3, 1),
           (2, 1, 8, 0), (2, 2, 17, 1), (2, 4, 64, 1), (2, 4, 1000, 0), (2, 1, 3, 1), (2, 2, 5, 0)]
-# three steps per launch exist for the forward kernels; the window of a
-# 144-column chunk (160 columns) is the largest that is staged in LDS
+# three steps per launch: the window of a 144-column chunk (160 columns) is
+# the largest that is staged in LDS
 SHAPES3 = [(3, 1, 8, 0), (3, 2, 17, 1), (3, 4, 64, 1), (3, 4, 1000, 0), (3, 1, 3, 1),
            (3, 2, 5, 0), (3, 2, 144, 1), (3, 1, 145, 0)]
 
 
 def set_shape(eng, st, vz, xw, g):
-    """One launch shape for every kernel; with three steps the reverse kernel
-    takes two."""
-    if st == 3:
+    """One launch shape for every kernel; the three-step reverse kernel has no
+    four-rows-per-lane form and then takes two steps."""
+    if st == 3 and vz == 4:
         eng.set_tuning(vz, xw, g, kernel=1, steps=2)
         for k in (0, 2):
             eng.set_tuning(vz, xw, g, kernel=k, steps=3)
@@ -147,15 +147,31 @@ def test_launch_shapes_bit_identical():
 @pytest.mark.parametrize("nt", [998, 999, 1000])
 def test_three_step_remainders_bit_identical(nt):
     # passes of T = nt + tlag steps with T mod 3 = 0, 1, 2 end in a 3-, 1- or
-    # 2-step launch; the sampled traces equal the one-step kernel's
+    # 2-step launch; traces, image, illumination, misfit and gradient equal
+    # the one-step kernels'
     s = overthrust_survey(nt=nt, nsrc=2, abc_nb=51, abc_factor=0.004, rbc_nb=41,
                           varVp=700.0, seed=4, max_batch=2)
     s.eng.set_tuning(2, 64, 0, steps=1)
-    base = [s.eng.modeling(F.ABC), s.eng.modeling(F.RBC)]
+
+    def run():
+        obs = s.eng.modeling(F.ABC)
+        out = [obs, s.eng.modeling(F.RBC)]
+        s.eng.set_observed(obs)
+        s.eng.set_model((s.v * 0.98).astype(np.float32))
+        s.eng.reset()
+        s.eng.migration()
+        out += list(s.eng.get_image()[1:])
+        s.eng.reset()
+        out.append(np.float64(s.eng.gradient(iteration=1)))
+        out += list(s.eng.get_gradient(100.0)[1:])
+        s.eng.set_model(s.v)
+        return out
+
+    base = run()
     for st, vz, xw, g in [(3, 2, 64, 1), (3, 1, 16, 0), (3, 4, 96, 1)]:
         set_shape(s.eng, st, vz, xw, g)
-        assert np.array_equal(s.eng.modeling(F.ABC), base[0]), (nt, vz, xw, g)
-        assert np.array_equal(s.eng.modeling(F.RBC), base[1]), (nt, vz, xw, g)
+        for k, (a, b) in enumerate(zip(run(), base)):
+            assert np.array_equal(a, b), (nt, vz, xw, g, k)
 
 
 def test_reverse_launch_shapes_bit_identical():
@@ -165,7 +181,7 @@ def test_reverse_launch_shapes_bit_identical():
     s.eng.set_observed(obs)
     s.eng.set_model((s.v * 0.98).astype(np.float32))
     results = []
-    for st, vz, xw, g in [(1, 2, 32, 0)] + SHAPES[5:] + SHAPES3[:3]:
+    for st, vz, xw, g in [(1, 2, 32, 0)] + SHAPES[5:] + SHAPES3:
         set_shape(s.eng, st, vz, xw, g)
         s.eng.reset()
         s.eng.migration()
@@ -330,10 +346,12 @@ def test_errors_are_reported():
     with pytest.raises(F.FwiError) as e:
         eng.set_model(np.full((201, 81), -1.0, np.float32))
     assert e.value.code == 5
-    for kernel in (-1, 1):            # three steps per launch: forward kernels only
+    eng.set_tuning(1, 64, kernel=-1, steps=3)
+    for kernel in (-1, 1):            # no three-step reverse kernel with 4 rows per lane
         with pytest.raises(F.FwiError):
-            eng.set_tuning(kernel=kernel, steps=3)
+            eng.set_tuning(4, kernel=kernel)
+    assert eng.get_tuning(1)[:2] == (3, 1)      # a rejected call changes nothing
+    eng.set_tuning(4, kernel=2)
+    assert eng.get_tuning(2)[:2] == (3, 4)
     with pytest.raises(F.FwiError):
         eng.set_tuning(kernel=0, steps=4)
-    eng.set_tuning(kernel=2, steps=3)
-    assert eng.get_tuning(2)[0] == 3
