@@ -32,6 +32,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_FWD = 16.0      # read u_t, u_{t-1}, v; write u_{t+1}  (FP32)
 BYTES_REV = 22.0      # per cell-update of the reverse pass (44 B per cell-step)
+# non-packed FP32 VALU issue: one wave64 instruction per cycle per CU
+# (4 SIMD16 units), 256 CUs at the 2.4 GHz peak engine clock
+VALU_PEAK = 256 * 2.4e9       # wave-instructions / s
 PMC_NT = 300                  # time samples of the counter-collection run
 PMC_COPY_BYTES = 256 << 20    # calibration copy (reads and writes this many bytes)
 PMC_TIMEOUT = 240             # seconds per counter pass
@@ -100,18 +103,36 @@ def cpu_baseline(s, vmig, xs, zs, xr, zr, w, tlag, obs, threads, nshots):
                       f"{el:.1f} s of CPU time"}
 
 
-def roof(ms, n, cells, bpc, copy_gbs):
+def compulsory_bpc(kind, steps):
+    """Bytes per cell one launch must move however it is tiled: reverse
+    (kind 1) reads S, S_old, R, R_old, c, image, illum and writes S, R (two
+    fields each when steps >= 2), image, illum; forward reads u, u_old, c and
+    writes u (two fields when steps >= 2)."""
+    if kind == 1:
+        return 52.0 if steps >= 2 else 44.0
+    return 20.0 if steps >= 2 else 16.0
+
+
+def roof(ms, n, cells, bpc, copy_gbs, kind=0, steps=1):
     """Roofline record of one kind of step launch from the engine's event
-    profile: ms over n launches that advanced `cells` cell-updates."""
+    profile: ms over n launches that advanced `cells` cell-updates.  `achieved`
+    counts the algorithmic bytes of unblocked steps (bpc per cell-update); a
+    launch that advances several steps moves fewer bytes (its compulsory
+    bytes), so `frac` can pass 1."""
     avg_s = ms / n * 1e-3
     per_launch = cells / n
     gbs = per_launch * bpc / avg_s / 1e9
+    cells_launch = per_launch / steps / (2.0 if kind == 1 else 1.0)
+    comp = cells_launch * compulsory_bpc(kind, steps)
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
             "algorithmic_bytes_per_launch": per_launch * bpc,
             "kernel_avg_us": round(avg_s * 1e6, 2),
             "gcell_per_s": round(per_launch / avg_s / 1e9, 2),
-            "frac_of_measured_copy": round(gbs / copy_gbs, 4)}
+            "frac_of_measured_copy": round(gbs / copy_gbs, 4),
+            "steps_per_launch": steps,
+            "compulsory_bytes_per_launch": comp,
+            "compulsory_frac": round(comp / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def other_workload(a):
@@ -183,16 +204,21 @@ def other_workload(a):
     fwd = eng.profile_read(0)
     rev = eng.profile_read(1)
     eng.profile_enable(False)
+    tune = {name: dict(zip(("steps_per_launch", "vz", "xw"), eng.get_tuning(k)[:3]))
+            for k, name in ((0, "forward"), (1, "reverse"), (2, "forward_sponge"))}
     eng.close()
     out = {"metric": "Gcell-updates/s", "value": round(cells / el / 1e9, 2),
            "unit": "Gcell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": round(el / a.steps * 1e3, 2), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": a.workload, "shots": s.nsrc, "nzz": nzz, "nxx": nxx,
-                      "steps_per_pass": T, "measured_copy_GBs": round(copy_gbs, 1)},
-           "roofline_forward": roof(*fwd, BYTES_FWD, copy_gbs)}
+                      "steps_per_pass": T, "tuning": tune,
+                      "measured_copy_GBs": round(copy_gbs, 1)},
+           # forward launches of both boundaries share one profile slot
+           "roofline_forward": roof(*fwd, BYTES_FWD, copy_gbs, 0,
+                                    tune["forward"]["steps_per_launch"])}
     if rev[1]:
-        out["roofline"] = roof(*rev, BYTES_REV, copy_gbs)
+        out["roofline"] = roof(*rev, BYTES_REV, copy_gbs, 1, tune["reverse"]["steps_per_launch"])
     print(json.dumps(out), flush=True)
 
 
@@ -253,7 +279,16 @@ def measure_traffic(nshots, tuning):
              "--fwd-tuning", f"{stf},{vzf},{xwf}", "--rev-tuning", f"{str_},{vzr},{xwr}"]
     want = {"forward": kernel_key(0, stf, vzf), "reverse": kernel_key(1, str_, vzr)}
     res = {"forward": {}, "reverse": {}, "calibration": {}}
+    valu = {}
     with tempfile.TemporaryDirectory(prefix="fwi_pmc_") as wd:
+        try:       # VALU instructions issued per dispatch (the compute side)
+            vals = counter_pass(prof, "SQ_INSTS_VALU", child, wd)
+            for kind, pre in want.items():
+                vs = [v for k, v_ in vals.items() if pre in k for v in v_]
+                if vs:
+                    valu[kind] = float(np.median(vs))
+        except RuntimeError:
+            valu = {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             vals = counter_pass(prof, counter, child, wd)
             copy = [v for k, vs in vals.items() if "copy_kernel" in k for v in vs]
@@ -272,7 +307,7 @@ def measure_traffic(nshots, tuning):
             f"counter's reading for a {PMC_COPY_BYTES >> 20} MiB copy (factors "
             f"{res['calibration']})")
     return {"bytes": out, "read": {k: res[k]["FETCH_SIZE"] for k in out},
-            "write": {k: res[k]["WRITE_SIZE"] for k in out}}, note
+            "write": {k: res[k]["WRITE_SIZE"] for k in out}, "valu": valu}, note
 
 
 def pmc_child(a):
@@ -380,13 +415,13 @@ def main():
     cells_step = nshots * nzz * nxx * T * 3.0
     value = cells_step * a.steps / el / 1e9
 
-    rev_roof = roof(rev_ms, rev_n, rev_cells, BYTES_REV, copy_gbs)
-    fwd_roof = roof(fwd_ms, fwd_n, fwd_cells, BYTES_FWD, copy_gbs)
-    rev_roof["kernel"] = "rev_step_kernel (fused reverse + imaging)"
-    fwd_roof["kernel"] = "fwd_step_kernel (random boundary)"
     stf, vzf, xwf, g = eng.get_tuning(0)
     str_, vzr, xwr, _ = eng.get_tuning(1)
     sts, vzs, xws, _ = eng.get_tuning(2)
+    rev_roof = roof(rev_ms, rev_n, rev_cells, BYTES_REV, copy_gbs, 1, str_)
+    fwd_roof = roof(fwd_ms, fwd_n, fwd_cells, BYTES_FWD, copy_gbs, 0, stf)
+    rev_roof["kernel"] = kernel_key(1, str_, vzr) + " (fused reverse + imaging)"
+    fwd_roof["kernel"] = kernel_key(0, stf, vzf) + " (random-boundary forward)"
     out = {
         "metric": "Gcell-updates/s", "value": round(value, 2), "unit": "Gcell-updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -417,6 +452,12 @@ def main():
                 out[key]["traffic_write"] = round(tr["write"][kind], 0)
                 out[key]["traffic_over_algorithmic"] = round(
                     tr["bytes"][kind] / out[key]["algorithmic_bytes_per_launch"], 4)
+                if kind in tr["valu"]:
+                    n_valu = tr["valu"][kind]
+                    rate = n_valu / (out[key]["kernel_avg_us"] * 1e-6)
+                    out[key]["valu"] = {"insts_per_launch": n_valu, "achieved": round(rate, 0),
+                                        "peak": VALU_PEAK, "unit": "wave-instructions/s",
+                                        "frac": round(rate / VALU_PEAK, 4)}
         out["roofline"]["traffic_method"] = note
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)

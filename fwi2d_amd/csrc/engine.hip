@@ -986,7 +986,8 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
     double best[3] = {1e30, 1e30, 1e30};
     for (int kind = 0; kind < 3; ++kind) {
         // kind 2, the sponge forward, runs on its own grid with the receivers
-        // sampled (step index past tlag); the others at step T (no injection)
+        // sampled (step index past tlag); kind 0 at step T (no source); the
+        // reverse kernel at step 0, receiver data injected
         if (kind == 2 && !c->grid[0].on) continue;
         const GridBuf &gb = c->grid[kind == 2 ? 0 : 1];
         const int tstep = (kind == 2 && c->tlag + 3 <= T) ? c->tlag + 1 : T;
@@ -1020,7 +1021,11 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
                                                 : fwd_args(c, FWI_BOUNDARY_RBC, false);
                     fa.cur = c->d_f[1]; fa.old = c->d_f[0];
                     fa.out1 = (st >= 2) ? c->d_f[2] : c->d_f[0]; fa.out2 = c->d_f[3];
+                    // receiver data injected at every step, as in a pass:
+                    // the slot-indexed seismogram buffer covers every slot
                     fwi::RevStep ra = rev_args(c, 0);
+                    ra.data = c->d_seis;
+                    ra.data_by_shot = 0;
                     ra.scur = c->d_f[1]; ra.sold = c->d_f[0];
                     ra.sout1 = (st >= 2) ? c->d_f[2] : c->d_f[0]; ra.sout2 = c->d_f[3];
                     ra.rcur = c->d_f[5]; ra.rold = c->d_f[4];
@@ -1028,7 +1033,7 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
                     auto launch = [&]() {
                         return kind != 1
                                    ? fwi::launch_fwd_step(fa, tl, nslots, tstep, st, c->stream)
-                                   : fwi::launch_rev_step(ra, tl, nslots, T, st, c->stream);
+                                   : fwi::launch_rev_step(ra, tl, nslots, 0, st, c->stream);
                     };
                     for (int w = 0; w < 2; ++w) HIP_TRY(launch());
                     HIP_TRY(hipEventRecord(c->ev0, c->stream));

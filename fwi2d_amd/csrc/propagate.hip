@@ -153,21 +153,6 @@ __device__ __forceinline__ void zext(float (&ext)[VZ + 8], const float (&own)[VZ
     }
 }
 
-// Cerjan factor of cell (i,j): 1 inside, b1d on the edges, b1d[a]+b1d[b]-1
-// in the corners (modeling.cu:355-402).  i is clamped into the grid.
-__device__ __forceinline__ float damper(const float *b1d, int i, int j, const Grid &g)
-{
-    i = min(max(i, 0), g.nzz - 1);
-    const int nb = g.nb;
-    const bool inz = (i >= nb) && (i < g.nzz - nb);
-    const bool inx = (j >= nb) && (j < g.nxx - nb);
-    if (inz && inx) return 1.f;
-    const int a = (i < nb) ? i : (g.nzz - 1 - i);
-    const int b = (j < nb) ? j : (g.nxx - 1 - j);
-    if (inx) return b1d[a];
-    if (inz) return b1d[b];
-    return (b1d[a] + b1d[b]) - 1.f;
-}
 
 // Tile of a 1-D launch after the XCD renumbering: each of the 8 XCDs gets a
 // contiguous run of per = ceil(ntiles/8) logical tiles.  Returns -1 for the
@@ -216,20 +201,42 @@ __device__ __forceinline__ float add_nc(float a, float b)
 
 // Sponge factors split by axis: dz(i) = b1d of the row's edge distance in
 // the z pads (1 elsewhere), dx(j) likewise; the reference's lookup
-// (modeling.cu:355-402) is then d = dz (x inside), dx (z inside),
-// (dz + dx) - 1 (corners), 1 (interior).
+// (modeling.cu:355-402) is d = 1 (interior), dz (x inside), dx (z inside),
+// dz + dx - 1 (corners).  All four are d = dx + (dz - 1): dz - 1 is exact
+// for dz >= 1/2 (Sterbenz) and 1 + (dz - 1) then rounds back to dz, so the
+// edges and the interior come out exactly and the corners are the sum.  One
+// add per cell and no per-lane or per-column flags.
 __device__ __forceinline__ float pad_factor(const float *b1d, int k, int n, int nb, bool &in)
 {
     k = min(max(k, 0), n - 1);
     in = (k >= nb) && (k < n - nb);
     return in ? 1.f : b1d[(k < nb) ? k : (n - 1 - k)];
 }
-__device__ __forceinline__ float sponge(float dz, bool inz, float dx, bool inx)
+__device__ __forceinline__ float sponge(float dx, float dzm1)
 {
-    if (inz && inx) return 1.f;
-    if (inx) return dz;
-    if (inz) return dx;
-    return (dz + dx) - 1.f;
+    return dx + dzm1;
+}
+// dz - 1 of a lane's VZ rows.
+template <int VZ>
+__device__ __forceinline__ void row_factors(const float *b1d, int zb, const Grid &g,
+                                            float (&dzm1)[VZ])
+{
+#pragma unroll
+    for (int r = 0; r < VZ; ++r) {
+        bool in;
+        dzm1[r] = pad_factor(b1d, zb + r, g.nzz, g.nb, in) - 1.f;
+    }
+}
+
+// Cerjan factor of cell (i,j): 1 inside, b1d on the edges, b1d[a]+b1d[b]-1
+// in the corners (modeling.cu:355-402), in the form of sponge() below.
+// i is clamped into the grid.
+__device__ __forceinline__ float damper(const float *b1d, int i, int j, const Grid &g)
+{
+    bool in;
+    const float dz = pad_factor(b1d, i, g.nzz, g.nb, in);
+    const float dx = pad_factor(b1d, j, g.nxx, g.nb, in);
+    return sponge(dx, dz - 1.f);
 }
 
 // One leapfrog update with explicit FMAs:
@@ -240,6 +247,20 @@ __device__ __forceinline__ float leapfrog(float p, float o, float c, float lap, 
 {
     if constexpr (ABC) return d * fmaf(c, lap, p + fmaf(-d, o, p));
     else return fmaf(c, lap, p + (p - o));
+}
+
+// Adds the source sample w to the source cell (row siz of column six) when
+// column j holds it; the cell is inside the rim, so adding after the rim
+// select gives the same value as adding before it.
+template <int VZ>
+__device__ __forceinline__ void add_source(float (&v)[VZ], int j, int six, int zb, int siz,
+                                           float w)
+{
+    if (j == six) {
+#pragma unroll
+        for (int r = 0; r < VZ; ++r)
+            if (zb + r == siz) v[r] += w;
+    }
 }
 
 // z-extension of a field that exists only in registers (the inner field of a
@@ -302,12 +323,14 @@ __device__ __forceinline__ void ldcol(float (&r)[VZ], const float *base, int col
 // clamped into the grid, so a column beyond the edge reads the edge column,
 // which is a zero rim column of every wavefield (sources and receivers sit
 // inside the rim).  Such values only feed rim columns.
+// The address is a wave-uniform column pointer plus the lane's unsigned row
+// offset, so the access takes its base in SGPRs and one offset VGPR.
 template <int VZ>
 __device__ __forceinline__ void ldcolc(float (&r)[VZ], const float *base, int col, int nxx,
-                                       int ldz, int zb)
+                                       int ldz, unsigned zl)
 {
     col = min(max(col, 0), nxx - 1);
-    vload<VZ>(r, base + (long)col * ldz + zb);
+    vload<VZ>(r, (base + (long)col * ldz) + zl);
 }
 // The halo rows of column col clamped into [4, nxx-4), the columns whose
 // values it feeds.  The first row segment starts at z0 = -4, so its rows
@@ -318,8 +341,8 @@ __device__ __forceinline__ float4 ldhaloc(const float *base, int col, int nxx, i
                                           int lane)
 {
     col = min(max(col, 4), nxx - 5);
-    return *reinterpret_cast<const float4 *>(base + (long)col * ldz +
-                                             (lane < 32 ? z0 - 4 : z0 + 64 * VZ));
+    const unsigned off = lane < 32 ? 0u : unsigned(64 * VZ + 4);
+    return *reinterpret_cast<const float4 *>((base + (long)col * ldz + (z0 - 4)) + off);
 }
 
 // Receivers in column j whose row is in this lane: seis[rec][tsamp] = d * v[row].
@@ -349,9 +372,9 @@ __device__ __forceinline__ void inject_receivers(const RevStep &a, float (&v)[VZ
     const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
     for (int m = k0; m < k1; ++m) {
         const int rr = a.rec.iz[m] - zb;
+        const float val = dat[(long)a.rec.id[m] * a.nt + trow];
 #pragma unroll
-        for (int r = 0; r < VZ; ++r)
-            if (rr == r) v[r] = fmaf(a.idh2, dat[(long)a.rec.id[m] * a.nt + trow], v[r]);
+        for (int r = 0; r < VZ; ++r) v[r] = fmaf(a.idh2, rr == r ? val : 0.f, v[r]);
     }
 }
 
@@ -437,23 +460,27 @@ __device__ __forceinline__ bool stage_rev(const RevStep &a, const float *dat, in
     return ok;
 }
 
-// seis[rec][tsamp] = d * v for the receivers of window column c in this lane.
-template <int VZ>
+// seis[rec][tsamp] = d * v for the receivers of window column c in this
+// lane, d the sponge factor of the cell (dx of the column, dz - 1 of the
+// row) or 1.
+template <int VZ, bool ABC>
 __device__ __forceinline__ void sample_staged(const FwdStage &st, int nt, const float (&v)[VZ],
-                                              const float (&d)[VZ], int c, int zb, float *seis,
-                                              int tsamp)
+                                              float dx, const float (&zm1)[VZ], int c, int zb,
+                                              float *seis, int tsamp)
 {
     const int m0 = st.cp[c], m1 = st.cp[c + 1];
     for (int m = m0; m < m1; ++m) {
         const int rr = st.iz[m] - zb;
 #pragma unroll
         for (int r = 0; r < VZ; ++r)
-            if (rr == r) seis[(long)st.id[m] * nt + tsamp] = v[r] * d[r];
+            if (rr == r)
+                seis[(long)st.id[m] * nt + tsamp] = v[r] * (ABC ? sponge(dx, zm1[r]) : 1.f);
     }
 }
 
 // Adds idh2 * data to the receivers of window column c in this lane, in
-// table order.
+// table order.  Branch-free: every entry takes one FMA per row, with data 0
+// for the rows that are not its own (an exact + 0).
 template <int VZ, int STEP>
 __device__ __forceinline__ void inject_staged(const RevStage &st, float idh2, float (&v)[VZ], int c,
                                               int zb)
@@ -463,8 +490,7 @@ __device__ __forceinline__ void inject_staged(const RevStage &st, float idh2, fl
         const int rr = st.iz[m] - zb;
         const float val = st.v[STEP][m];
 #pragma unroll
-        for (int r = 0; r < VZ; ++r)
-            if (rr == r) v[r] = fmaf(idh2, val, v[r]);
+        for (int r = 0; r < VZ; ++r) v[r] = fmaf(idh2, rr == r ? val : 0.f, v[r]);
     }
 }
 
@@ -513,10 +539,8 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
     __shared__ FwdStage st;
     const bool staged = stage_fwd(a, rec_on, ABC, j0, j1 - j0, st, lane);
 
-    float dzr[VZ];
-    bool inz[VZ];
-#pragma unroll
-    for (int r = 0; r < VZ; ++r) dzr[r] = ABC ? pad_factor(a.b1d, zb + r, g.nzz, g.nb, inz[r]) : 1.f;
+    float zm1[VZ];                                   // dz - 1 of the lane's rows
+    if constexpr (ABC) row_factors<VZ>(a.b1d, zb, g, zm1);
 
     float q[9][VZ];
 #pragma unroll
@@ -529,7 +553,6 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
     float4 h = ldhalo<VZ>(cur, j0, 4, nxx - 4, ldz, z0, lane);
 
     for (int j = j0; j < j1; ++j) {
-        const bool inx = (j >= g.nb) && (j < nxx - g.nb);
         float dxj = 1.f;
         if constexpr (ABC) {
             bool in;
@@ -546,20 +569,19 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
         float ext[VZ + 8];
         zext<VZ>(ext, q[4], h);
         h = ldhalo<VZ>(cur, j + 1, 4, nxx - 4, ldz, z0, lane);
-        float nv[VZ], dv[VZ];
+        float nv[VZ];
 #pragma unroll
         for (int r = 0; r < VZ; ++r) {
             const int i = zb + r;
-            const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
-            dv[r] = d;
+            const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
             float val = leapfrog<ABC>(q[4][r], o[r], c[r], lap8r<VZ, 0>(q, ext, r), d);
-            if (i == siz && j == six) val += wsrc;
             nv[r] = (i >= 4 && i < g.nzz - 4) ? val : o[r];
         }
+        add_source<VZ>(nv, j, six, zb, siz, wsrc);
         if (active) {
             vstore<VZ>(out + (long)j * ldz + zb, nv);
             if (rec_on) {
-                if (staged) sample_staged<VZ>(st, a.nt, nv, dv, j - j0, zb, seis, t + 1 - a.tlag);
+                if (staged) sample_staged<VZ, ABC>(st, a.nt, nv, dxj, zm1, j - j0, zb, seis, t + 1 - a.tlag);
                 else sample_receivers<VZ, ABC>(a, nv, j, zb, seis, t + 1 - a.tlag);
             }
         }
@@ -612,10 +634,8 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
     __shared__ FwdStage st;
     const bool staged = stage_fwd(a, rec1 || rec2, ABC, xa, x1 + 4 - xa, st, lane);
 
-    float dzr[VZ];
-    bool inz[VZ];
-#pragma unroll
-    for (int r = 0; r < VZ; ++r) dzr[r] = ABC ? pad_factor(a.b1d, zb + r, g.nzz, g.nb, inz[r]) : 1.f;
+    float zm1[VZ];                                   // dz - 1 of the lane's rows
+    if constexpr (ABC) row_factors<VZ>(a.b1d, zb, g, zm1);
 
     float q1[9][VZ];     // u_t, columns j-4 .. j+4
     float q2[9][VZ];     // u_{t+1}, columns j-8 .. j
@@ -649,28 +669,26 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
         float n1[VZ];
         vzero<VZ>(n1);
         if (j >= 4 && j < nxx - 4) {
-            const bool inx = (j >= g.nb) && (j < nxx - g.nb);
             float dxj = 1.f;
             if constexpr (ABC) {
                 bool in;
                 dxj = staged ? st.dx[j - xa] : pad_factor(a.b1d, j, nxx, g.nb, in);
             }
-            float ext[VZ + 8], dv[VZ];
+            float ext[VZ + 8];
             zext<VZ>(ext, q1[4], h);
             h = ldhalo<VZ>(cur, j + 1, 4, nxx - 4, ldz, zs, lane);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
-                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
-                dv[r] = d;
+                const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q1[4][r], o1[r], cj[r], lap8r<VZ, 0>(q1, ext, r), d);
-                if (i == siz && j == six) val += w1;
                 n1[r] = (i >= 4 && i < g.nzz - 4) ? val : o1[r];
             }
+            add_source<VZ>(n1, j, six, zb, siz, w1);
             if (own && j >= x0 && j < x1) {
                 vstore<VZ>(out1 + (long)j * ldz + zb, n1);
                 if (rec1) {
-                    if (staged) sample_staged<VZ>(st, a.nt, n1, dv, j - xa, zb, seis, t + 1 - a.tlag);
+                    if (staged) sample_staged<VZ, ABC>(st, a.nt, n1, dxj, zm1, j - xa, zb, seis, t + 1 - a.tlag);
                     else sample_receivers<VZ, ABC>(a, n1, j, zb, seis, t + 1 - a.tlag);
                 }
             }
@@ -687,7 +705,6 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
         // ---- step 2: u_{t+2} at column jj = j - 4 (an owned column)
         const int jj = j - 4;
         if (jj >= x0) {
-            const bool inx = (jj >= g.nb) && (jj < nxx - g.nb);
             float dxj = 1.f;
             if constexpr (ABC) {
                 bool in;
@@ -695,23 +712,22 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
             }
             float c2[VZ];                               // c at column jj (parked 4 columns ago)
             vload<VZ>(c2, &cring[slot][lane * VZ]);
-            float ext[VZ + 8], dv[VZ];
+            float ext[VZ + 8];
             zext_local<VZ>(ext, q2[4]);
             float n2[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float o2 = q1[0][r];              // u_t at column jj
-                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
-                dv[r] = d;
+                const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q2[4][r], o2, c2[r], lap8r<VZ, 0>(q2, ext, r), d);
-                if (i == siz && jj == six) val += w2;
                 n2[r] = (i >= 4 && i < g.nzz - 4) ? val : o2;
             }
+            add_source<VZ>(n2, jj, six, zb, siz, w2);
             if (own) {
                 vstore<VZ>(out2 + (long)jj * ldz + zb, n2);
                 if (rec2) {
-                    if (staged) sample_staged<VZ>(st, a.nt, n2, dv, jj - xa, zb, seis, t + 2 - a.tlag);
+                    if (staged) sample_staged<VZ, ABC>(st, a.nt, n2, dxj, zm1, jj - xa, zb, seis, t + 2 - a.tlag);
                     else sample_receivers<VZ, ABC>(a, n2, jj, zb, seis, t + 2 - a.tlag);
                 }
             }
@@ -743,6 +759,7 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
     const int zs = id.zseg * (64 * VZ - 16) - 4;    // first computed row; owned from zs + 8
     const int zb = zs + lane * VZ;
     const int zl = zb < 0 ? 0 : zb;                  // load row: rows above 0 feed rim rows only
+    const unsigned uzl = unsigned(zl);
     const bool active = zb < ldz;
     const bool own = active && (zb >= zs + 8) && (zb < zs + 64 * VZ - 8);
     const int x0 = 4 + id.xc * xw;
@@ -767,10 +784,8 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
     __shared__ FwdStage st;
     const bool staged = stage_fwd(a, rec1 || rec2 || rec3, ABC, xa, x1 + 8 - xa, st, lane);
 
-    float dzr[VZ];
-    bool inz[VZ];
-#pragma unroll
-    for (int r = 0; r < VZ; ++r) dzr[r] = ABC ? pad_factor(a.b1d, zb + r, g.nzz, g.nb, inz[r]) : 1.f;
+    float zm1[VZ];                                   // dz - 1 of the lane's rows
+    if constexpr (ABC) row_factors<VZ>(a.b1d, zb, g, zm1);
 
     float q1[9][VZ];     // u_t
     float q2[9][VZ];     // u_{t+1}
@@ -806,26 +821,24 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
         // ---- step 1: u_{t+1} at column j
         {
             const bool jin = (j >= 4) && (j < nxx - 4);
-            const bool inx = (j >= g.nb) && (j < nxx - g.nb);
             float dxj = 1.f;
             if constexpr (ABC) {
                 bool in;
                 dxj = staged ? st.dx[j - xa] : pad_factor(a.b1d, j, nxx, g.nb, in);
             }
-            float ext[VZ + 8], dv[VZ];
+            float ext[VZ + 8];
             zext<VZ>(ext, q1[C4], h);
             h = ldhaloc<VZ>(cur, j + 1, nxx, ldz, zs, lane);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
-                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
-                dv[r] = d;
+                const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q1[C4][r], o1[r], cj[r], lap8r<VZ, C0>(q1, ext, r), d);
-                if (i == siz && j == six) val += w1;
                 q2[C8][r] = (jin && i >= 4 && i < g.nzz - 4) ? val : o1[r];
             }
+            add_source<VZ>(q2[C8], j, six, zb, siz, w1);
             if (rec1 && own && j >= x0 && j < x1) {
-                if (staged) sample_staged<VZ>(st, a.nt, q2[C8], dv, j - xa, zb, seis, t + 1 - a.tlag);
+                if (staged) sample_staged<VZ, ABC>(st, a.nt, q2[C8], dxj, zm1, j - xa, zb, seis, t + 1 - a.tlag);
                 else sample_receivers<VZ, ABC>(a, q2[C8], j, zb, seis, t + 1 - a.tlag);
             }
         }
@@ -833,29 +846,28 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
         {
             const int jj = j - 4;
             const bool jin = (jj >= 4) && (jj < nxx - 4);
-            const bool inx = (jj >= g.nb) && (jj < nxx - g.nb);
             float dxj = 1.f;
             if constexpr (ABC) {
                 bool in;
                 dxj = staged ? st.dx[max(jj - xa, 0)] : pad_factor(a.b1d, jj, nxx, g.nb, in);
             }
-            float ext[VZ + 8], dv[VZ];
+            float ext[VZ + 8], c2v[VZ];
             zext_local<VZ>(ext, q2[C4]);
+            vload<VZ>(c2v, &cring[C4][lane * VZ]);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float o2 = q1[C0][r];             // u_t at column jj
-                const float c2 = cring[C4][lane * VZ + r];
-                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
-                dv[r] = d;
+                const float c2 = c2v[r];
+                const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q2[C4][r], o2, c2, lap8r<VZ, C0>(q2, ext, r), d);
-                if (i == siz && jj == six) val += w2;
                 q3[C8][r] = (jin && i >= 4 && i < g.nzz - 4) ? val : o2;
             }
+            add_source<VZ>(q3[C8], jj, six, zb, siz, w2);
             if (own && jj >= x0 && jj < x1) {
-                vstore<VZ>(out1 + (long)jj * ldz + zb, q3[C8]);
+                vstore<VZ>((out1 + (long)jj * ldz) + uzl, q3[C8]);
                 if (rec2) {
-                    if (staged) sample_staged<VZ>(st, a.nt, q3[C8], dv, jj - xa, zb, seis, t + 2 - a.tlag);
+                    if (staged) sample_staged<VZ, ABC>(st, a.nt, q3[C8], dxj, zm1, jj - xa, zb, seis, t + 2 - a.tlag);
                     else sample_receivers<VZ, ABC>(a, q3[C8], jj, zb, seis, t + 2 - a.tlag);
                 }
             }
@@ -863,7 +875,6 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
         // ---- step 3: u_{t+3} at column jjj = j - 8 (an owned column)
         const int jjj = j - 8;
         if (jjj >= x0) {
-            const bool inx = (jjj >= g.nb) && (jjj < nxx - g.nb);
             float dxj = 1.f;
             if constexpr (ABC) {
                 bool in;
@@ -871,23 +882,22 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
             }
             float c3[VZ];
             vload<VZ>(c3, &cring[C0][lane * VZ]);
-            float ext[VZ + 8], dv[VZ];
+            float ext[VZ + 8];
             zext_local<VZ>(ext, q3[C4]);
             float n3[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float o3 = q2[C0][r];             // u_{t+1} at column jjj
-                const float d = ABC ? sponge(dzr[r], inz[r], dxj, inx) : 1.f;
-                dv[r] = d;
+                const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q3[C4][r], o3, c3[r], lap8r<VZ, C0>(q3, ext, r), d);
-                if (i == siz && jjj == six) val += w3;
                 n3[r] = (i >= 4 && i < g.nzz - 4) ? val : o3;
             }
+            add_source<VZ>(n3, jjj, six, zb, siz, w3);
             if (own) {
-                vstore<VZ>(out2 + (long)jjj * ldz + zb, n3);
+                vstore<VZ>((out2 + (long)jjj * ldz) + uzl, n3);
                 if (rec3) {
-                    if (staged) sample_staged<VZ>(st, a.nt, n3, dv, jjj - xa, zb, seis, t + 3 - a.tlag);
+                    if (staged) sample_staged<VZ, ABC>(st, a.nt, n3, dxj, zm1, jjj - xa, zb, seis, t + 3 - a.tlag);
                     else sample_receivers<VZ, ABC>(a, n3, jjj, zb, seis, t + 3 - a.tlag);
                 }
             }
@@ -1208,6 +1218,7 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
     const int zs = id.zseg * (64 * VZ - 16) - 4;    // first computed row; owned from zs + 8
     const int zb = zs + lane * VZ;
     const int zl = zb < 0 ? 0 : zb;                  // load row: rows above 0 feed rim rows only
+    const unsigned uzl = unsigned(zl);
     const bool active = zb < ldz;
     const bool own = active && (zb >= zs + 8) && (zb < zs + 64 * VZ - 8);
     const int x0 = 4 + id.xc * xw;
@@ -1279,8 +1290,8 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
         ldcolc<VZ>(orn, rold, j + 1, nxx, ldz, zl);
         ldcolc<VZ>(cn, cc, j + 1, nxx, ldz, zl);
         if (j + 1 >= x0 && j + 1 < x1) {             // image / illumination of owned columns
-            vload<VZ>(an, acc + (long)(j + 1) * ldz + zl);
-            vload<VZ>(in, ill + (long)(j + 1) * ldz + zl);
+            vload<VZ>(an, (acc + (long)(j + 1) * ldz) + uzl);
+            vload<VZ>(in, (ill + (long)(j + 1) * ldz) + uzl);
         }
         // ---- step 1 at column j
         const bool jin1 = (j >= 4) && (j < nxx - 4);
@@ -1315,14 +1326,15 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
         const int jj = j - 4;
         const bool jin2 = (jj >= 4) && (jj < nxx - 4);
         {
-            float es[VZ + 8], er[VZ + 8];
+            float es[VZ + 8], er[VZ + 8], c2v[VZ];
             zext_local<VZ>(es, qs2[C4]);
             zext_local<VZ>(er, qr2[C4]);
+            vload<VZ>(c2v, &cring[C4][lane * VZ]);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const bool inside = jin2 && (i >= 4) && (i < g.nzz - 4);
-                const float c2 = cring[C4][lane * VZ + r];
+                const float c2 = c2v[r];
                 const float os2 = qs1[C0][r];           // S_cur at column jj
                 const float or2 = qr1[C0][r];           // R_cur at column jj
                 const float ps = qs2[C4][r], pr = qr2[C4][r];
@@ -1342,8 +1354,8 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 else inject_receivers<VZ>(a, qr3[C8], jj, zb, dat, a.nt - 3 - k);
             }
             if (own && jj >= x0 && jj < x1) {
-                vstore<VZ>(sout1 + (long)jj * ldz + zb, qs3[C8]);
-                vstore<VZ>(rout1 + (long)jj * ldz + zb, qr3[C8]);
+                vstore<VZ>((sout1 + (long)jj * ldz) + uzl, qs3[C8]);
+                vstore<VZ>((rout1 + (long)jj * ldz) + uzl, qr3[C8]);
             }
         }
         // ---- step 3 at column jjj = j - 8 (an owned column)
@@ -1380,11 +1392,11 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 else inject_receivers<VZ>(a, nr3, jjj, zb, dat, a.nt - 4 - k);
             }
             if (own) {
-                const long off = (long)jjj * ldz + zb;
-                vstore<VZ>(sout2 + off, ns3);
-                vstore<VZ>(rout2 + off, nr3);
-                vstore<VZ>(acc + off, fa);
-                vstore<VZ>(ill + off, fi);
+                const long off = (long)jjj * ldz;
+                vstore<VZ>((sout2 + off) + uzl, ns3);
+                vstore<VZ>((rout2 + off) + uzl, nr3);
+                vstore<VZ>((acc + off) + uzl, fa);
+                vstore<VZ>((ill + off) + uzl, fi);
             }
         }
         // column j + 5 of S_cur, R_cur replaces column j - 4
