@@ -58,8 +58,9 @@ def parse():
     p.add_argument("--no-traffic", action="store_true",
                    help="skip the rocprofv3 counter passes (roofline.traffic = null)")
     p.add_argument("--workload", default="rtm64",
-                   choices=("rtm64", "forward1", "fwi128", "grad256"),
-                   help="rtm64 is the headline; the others show the rate's dependence on shape")
+                   choices=("rtm64", "forward1", "fwi128", "grad256", "net"),
+                   help="rtm64 is the headline; forward1, fwi128 and grad256 show the rate's "
+                        "dependence on shape; net times the learned preconditioner's Adam step")
     p.add_argument("--pmc-child", action="store_true",
                    help="internal: the short run profiled by the counter passes")
     return p.parse_args()
@@ -222,6 +223,36 @@ def other_workload(a):
     print(json.dumps(out), flush=True)
 
 
+def net_workload(a):
+    """The learned preconditioner's network on its training shape: batch 16
+    of 3 x 384 x 128 inputs (seeded random), `steps` Adam steps (forward,
+    backward, weight update) after `warmup`, then as many forward passes."""
+    from fwi2d_amd import abi as A
+    B, H, W = 16, 384, 128
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((B, 3, H, W)).astype(np.float32)
+    t = (0.1 * rng.standard_normal((B, H, W))).astype(np.float32)
+    net = A.Net(B, H, W, 0)
+    for _ in range(a.warmup):
+        net.train_step(x, t, 1e-3)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        net.train_step(x, t, 1e-3)
+    el = (time.perf_counter() - t0) / a.steps
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        net.forward(x)
+    fw = (time.perf_counter() - t0) / a.steps
+    print(json.dumps({"metric": "preconditioner Adam steps/s", "value": round(1.0 / el, 2),
+                      "unit": "steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                      "ms_per_step": round(el * 1e3, 3), "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+                      "data": "synthetic (seeded random inputs and targets)",
+                      "config": {"workload": "net", "batch": B, "channels_in": 3, "height": H,
+                                 "width": W, "forward_ms": round(fw * 1e3, 3),
+                                 "note": "host copies of the batch included"}}), flush=True)
+
+
 def kernel_key(kind, steps, vz):
     """Name fragment of the random-boundary forward (kind 0) or imaging
     reverse (kind 1) kernel instance."""
@@ -351,6 +382,8 @@ def main():
         sys.exit(2)
     if a.pmc_child:
         return pmc_child(a)
+    if a.workload == "net":
+        return net_workload(a)
     if a.workload != "rtm64":
         return other_workload(a)
     import fwi2d_amd as F
