@@ -64,6 +64,7 @@ struct GridBuf {
     float *vpad = nullptr;       // padded base model (slot layout)
     int *src_iz = nullptr, *src_ix = nullptr;
     int *col_ptr = nullptr, *rec_iz = nullptr, *rec_id = nullptr;
+    std::vector<int> h_col_ptr;  // host copy of col_ptr (staging decisions)
 };
 
 // Four buffers of one wavefield: `cur` holds the latest step, `old` the one
@@ -141,7 +142,27 @@ fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g, int kind)
     t.xw = c->xw[kind];
     t.nzseg = fwi::zsegments(g.nzz, t.vz, c->nsteps[kind]);
     t.nxchunk = (g.nxx - 8 + t.xw - 1) / t.xw;
+    t.staged = true;
     return t;
+}
+
+// Whether every wave of a launch of `steps` steps with tiling t can stage its
+// window in LDS: at most kStageCols columns and, when the launch uses the
+// receiver tables, at most kStageRec receivers (the windows and counts the
+// kernels' stage_table computes).
+bool staging_fits(const GridBuf &gb, const fwi::Tiling &t, int steps, bool tables)
+{
+    const int h = fwi::window_halo(steps);
+    if (t.xw + 2 * h > fwi::kStageCols) return false;
+    if (!tables || gb.h_col_ptr.empty()) return true;
+    const int nxx = gb.g.nxx;
+    for (int xc = 0; xc < t.nxchunk; ++xc) {
+        const int x0 = 4 + xc * t.xw, x1 = std::min(x0 + t.xw, nxx - 4);
+        const int ca = std::min(std::max(x0 - h, 0), nxx);
+        const int cb = std::min(std::max(x1 + h, 0), nxx);
+        if (gb.h_col_ptr[cb] - gb.h_col_ptr[ca] > fwi::kStageRec) return false;
+    }
+    return true;
 }
 
 void drop_graphs(fwi_ctx *c)
@@ -225,7 +246,12 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
     // the remainder of a three-step pass runs as a two- or one-step launch,
     // whose waves own more rows each
     fwi::Tiling tls[4] = {tl, tl, tl, tl};
-    for (int n = 1; n <= 3; ++n) tls[n].nzseg = fwi::zsegments(g.nzz, tl.vz, n);
+    const GridBuf &gbuf = c->grid[kind == 2 ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC];
+    const bool tables = fa ? (fa->seis != nullptr) : true;
+    for (int n = 1; n <= 3; ++n) {
+        tls[n].nzseg = fwi::zsegments(g.nzz, tl.vz, n);
+        tls[n].staged = staging_fits(gbuf, tls[n], n, tables);
+    }
     Quad s0 = sq, r0 = rq ? *rq : sq;
     int launches = 0;
     auto enqueue = [&](bool launch) -> hipError_t {
@@ -611,6 +637,7 @@ int fwi_set_geometry(fwi_ctx *c, int32_t nsrc, const float *xsrc, const float *z
         HIP_TRY(hipMemcpy(gb.src_iz, siz.data(), nsrc * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(gb.src_ix, six.data(), nsrc * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(gb.col_ptr, col_ptr.data(), col_ptr.size() * 4, hipMemcpyHostToDevice));
+        gb.h_col_ptr = col_ptr;
         HIP_TRY(hipMemcpy(gb.rec_iz, s_iz.data(), nrec * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(gb.rec_id, s_id.data(), nrec * 4, hipMemcpyHostToDevice));
     }
@@ -1016,7 +1043,8 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
                     c->nsteps[kind] = st;
                     c->vz[kind] = vzs[a];
                     c->xw[kind] = xwv;
-                    const fwi::Tiling tl = tiling_of(c, gb.g, kind);
+                    fwi::Tiling tl = tiling_of(c, gb.g, kind);
+                    tl.staged = staging_fits(gb, tl, st, kind != 0);
                     fwi::FwdStep fa = kind == 2 ? fwd_args(c, FWI_BOUNDARY_ABC, true)
                                                 : fwd_args(c, FWI_BOUNDARY_RBC, false);
                     fa.cur = c->d_f[1]; fa.old = c->d_f[0];

@@ -26,7 +26,15 @@ struct Tiling {
     int vz;            // rows per lane: 1, 2 or 4
     int xw;            // columns per wave
     int nzseg, nxchunk;
+    bool staged;       // every wave's window fits the LDS tables below
 };
+
+// Per-wave LDS copies of the receiver table (and sponge x factors) cover a
+// window of at most kStageCols columns holding at most kStageRec receivers;
+// a launch of s steps has the window [x0 - halo, x1 + halo), halo = 4(s-1).
+constexpr int kStageCols = 160;
+constexpr int kStageRec = 256;
+inline int window_halo(int steps) { return 4 * (steps - 1); }
 
 // Receivers of one grid, sorted by column (CSR over columns).
 struct RecTable {

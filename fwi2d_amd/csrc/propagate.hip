@@ -364,6 +364,28 @@ __device__ __forceinline__ void sample_receivers(const FwdStep &a, const float (
     }
 }
 
+// sample3_staged from the global receiver table.
+template <int VZ, bool ABC>
+__device__ __forceinline__ void sample3_receivers(const FwdStep &a, const float (&v1)[VZ],
+                                                  const float (&v2)[VZ], const float (&v3)[VZ],
+                                                  int j, int zb, float *seis, int tsamp, bool r1,
+                                                  bool r2, bool r3)
+{
+    const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
+    for (int k = k0; k < k1; ++k) {
+        const int rr = a.rec.iz[k] - zb;
+#pragma unroll
+        for (int r = 0; r < VZ; ++r)
+            if (rr == r) {
+                const float d = ABC ? damper(a.b1d, zb + r, j, a.g) : 1.f;
+                const long base = (long)a.rec.id[k] * a.nt + tsamp;
+                if (r1) seis[base] = v1[r] * d;
+                if (r2) seis[base + 1] = v2[r] * d;
+                if (r3) seis[base + 2] = v3[r] * d;
+            }
+    }
+}
+
 // Adds idh2 * data[rec][trow] to the receivers of column j in this lane.
 template <int VZ>
 __device__ __forceinline__ void inject_receivers(const RevStep &a, float (&v)[VZ], int j, int zb,
@@ -382,10 +404,11 @@ __device__ __forceinline__ void inject_receivers(const RevStep &a, float (&v)[VZ
 // factors) over the wave's column window [xa, xa + nc).  It is filled once
 // at kernel start so that the column march never waits on a global load whose
 // result it needs at once: such a wait also drains the loads prefetched for
-// the next column.  Windows with more than kStageRec receivers use the
-// global-memory path instead.
-constexpr int kStageCols = 160;
-constexpr int kStageRec = 256;
+// the next column.  Each step kernel is built twice: staged (ST), launched
+// when every window of the launch fits the tables (the host checks it from
+// the receiver columns), and with the global-memory path only.  One kernel
+// holding both paths would join them inside the unrolled column loop, where
+// the compiler then drains all outstanding loads at each join.
 
 struct FwdStage {
     int cp[kStageCols + 1];     // col_ptr[xa + c] - col_ptr[xa]
@@ -478,6 +501,30 @@ __device__ __forceinline__ void sample_staged(const FwdStage &st, int nt, const 
     }
 }
 
+// The three samples (steps 1-3 of a launch) of the receivers of window
+// column c in this lane, written when the last step leaves the column:
+// seis[rec][tsamp + k] = d * v_k for the steps k whose flag is set.
+template <int VZ, bool ABC>
+__device__ __forceinline__ void sample3_staged(const FwdStage &st, int nt, const float (&v1)[VZ],
+                                               const float (&v2)[VZ], const float (&v3)[VZ],
+                                               float dx, const float (&zm1)[VZ], int c, int zb,
+                                               float *seis, int tsamp, bool r1, bool r2, bool r3)
+{
+    const int m0 = st.cp[c], m1 = st.cp[c + 1];
+    for (int m = m0; m < m1; ++m) {
+        const int rr = st.iz[m] - zb;
+#pragma unroll
+        for (int r = 0; r < VZ; ++r)
+            if (rr == r) {
+                const float d = ABC ? sponge(dx, zm1[r]) : 1.f;
+                const long base = (long)st.id[m] * nt + tsamp;
+                if (r1) seis[base] = v1[r] * d;
+                if (r2) seis[base + 1] = v2[r] * d;
+                if (r3) seis[base + 2] = v3[r] * d;
+            }
+    }
+}
+
 // Adds idh2 * data to the receivers of window column c in this lane, in
 // table order.  Branch-free: every entry takes one FMA per row, with data 0
 // for the rows that are not its own (an exact + 0).
@@ -512,7 +559,7 @@ __device__ __forceinline__ bool tile_of(int nzseg, int nxchunk, int nslots, Tile
 
 // One time step: u_{t+1} = leapfrog(u_t, u_{t-1}) written over u_{t-1}
 // (the odd remainder of a pass, and a tuning alternative).
-template <int VZ, bool ABC>
+template <int VZ, bool ABC, bool ST>
 __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int t)
 {
@@ -537,7 +584,8 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
     const bool rec_on = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
     float *seis = rec_on ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
     __shared__ FwdStage st;
-    const bool staged = stage_fwd(a, rec_on, ABC, j0, j1 - j0, st, lane);
+    constexpr bool staged = ST;     // tables fit the LDS (the host checks)
+    if constexpr (ST) (void)stage_fwd(a, rec_on, ABC, j0, j1 - j0, st, lane);
 
     float zm1[VZ];                                   // dz - 1 of the lane's rows
     if constexpr (ABC) row_factors<VZ>(a.b1d, zb, g, zm1);
@@ -581,7 +629,7 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
         if (active) {
             vstore<VZ>(out + (long)j * ldz + zb, nv);
             if (rec_on) {
-                if (staged) sample_staged<VZ, ABC>(st, a.nt, nv, dxj, zm1, j - j0, zb, seis, t + 1 - a.tlag);
+                if constexpr (staged) sample_staged<VZ, ABC>(st, a.nt, nv, dxj, zm1, j - j0, zb, seis, t + 1 - a.tlag);
                 else sample_receivers<VZ, ABC>(a, nv, j, zb, seis, t + 1 - a.tlag);
             }
         }
@@ -600,7 +648,7 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
 // It reads u_t, u_{t-1} and c once for both steps and writes u_{t+1} (out1)
 // and u_{t+2} (out2) for the cells it owns; the four buffers are distinct,
 // so no wave reads a value another wave of the same launch writes.
-template <int VZ, bool ABC>
+template <int VZ, bool ABC, bool ST>
 __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int t)
 {
@@ -632,7 +680,8 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
     float *seis = (rec1 || rec2) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
     const int xa = x0 - 4;                           // window [x0 - 4, x1 + 4)
     __shared__ FwdStage st;
-    const bool staged = stage_fwd(a, rec1 || rec2, ABC, xa, x1 + 4 - xa, st, lane);
+    constexpr bool staged = ST;     // tables fit the LDS (the host checks)
+    if constexpr (ST) (void)stage_fwd(a, rec1 || rec2, ABC, xa, x1 + 4 - xa, st, lane);
 
     float zm1[VZ];                                   // dz - 1 of the lane's rows
     if constexpr (ABC) row_factors<VZ>(a.b1d, zb, g, zm1);
@@ -688,7 +737,7 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
             if (own && j >= x0 && j < x1) {
                 vstore<VZ>(out1 + (long)j * ldz + zb, n1);
                 if (rec1) {
-                    if (staged) sample_staged<VZ, ABC>(st, a.nt, n1, dxj, zm1, j - xa, zb, seis, t + 1 - a.tlag);
+                    if constexpr (staged) sample_staged<VZ, ABC>(st, a.nt, n1, dxj, zm1, j - xa, zb, seis, t + 1 - a.tlag);
                     else sample_receivers<VZ, ABC>(a, n1, j, zb, seis, t + 1 - a.tlag);
                 }
             }
@@ -727,7 +776,7 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
             if (own) {
                 vstore<VZ>(out2 + (long)jj * ldz + zb, n2);
                 if (rec2) {
-                    if (staged) sample_staged<VZ, ABC>(st, a.nt, n2, dxj, zm1, jj - xa, zb, seis, t + 2 - a.tlag);
+                    if constexpr (staged) sample_staged<VZ, ABC>(st, a.nt, n2, dxj, zm1, jj - xa, zb, seis, t + 2 - a.tlag);
                     else sample_receivers<VZ, ABC>(a, n2, jj, zb, seis, t + 2 - a.tlag);
                 }
             }
@@ -747,8 +796,8 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
 // chunk; the three fields of a column travel through three register queues.
 // It reads u_t, u_{t-1} and c once and writes only what the next launch
 // needs: u_{t+2} (out1) and u_{t+3} (out2) for the cells it owns.
-template <int VZ, bool ABC>
-__global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
+template <int VZ, bool ABC, bool ST>
+__global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int t)
 {
     TileId id;
@@ -782,7 +831,8 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
     float *seis = (rec1 || rec2 || rec3) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
     const int xa = x0 - 8;                           // window [x0 - 8, x1 + 8)
     __shared__ FwdStage st;
-    const bool staged = stage_fwd(a, rec1 || rec2 || rec3, ABC, xa, x1 + 8 - xa, st, lane);
+    constexpr bool staged = ST;     // tables fit the LDS (the host checks)
+    if constexpr (ST) (void)stage_fwd(a, rec1 || rec2 || rec3, ABC, xa, x1 + 8 - xa, st, lane);
 
     float zm1[VZ];                                   // dz - 1 of the lane's rows
     if constexpr (ABC) row_factors<VZ>(a.b1d, zb, g, zm1);
@@ -837,10 +887,6 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
                 q2[C8][r] = (jin && i >= 4 && i < g.nzz - 4) ? val : o1[r];
             }
             add_source<VZ>(q2[C8], j, six, zb, siz, w1);
-            if (rec1 && own && j >= x0 && j < x1) {
-                if (staged) sample_staged<VZ, ABC>(st, a.nt, q2[C8], dxj, zm1, j - xa, zb, seis, t + 1 - a.tlag);
-                else sample_receivers<VZ, ABC>(a, q2[C8], j, zb, seis, t + 1 - a.tlag);
-            }
         }
         // ---- step 2: u_{t+2} at column jj = j - 4 (columns before x0 - 4 are not used)
         {
@@ -864,13 +910,7 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
                 q3[C8][r] = (jin && i >= 4 && i < g.nzz - 4) ? val : o2;
             }
             add_source<VZ>(q3[C8], jj, six, zb, siz, w2);
-            if (own && jj >= x0 && jj < x1) {
-                vstore<VZ>((out1 + (long)jj * ldz) + uzl, q3[C8]);
-                if (rec2) {
-                    if (staged) sample_staged<VZ, ABC>(st, a.nt, q3[C8], dxj, zm1, jj - xa, zb, seis, t + 2 - a.tlag);
-                    else sample_receivers<VZ, ABC>(a, q3[C8], jj, zb, seis, t + 2 - a.tlag);
-                }
-            }
+            if (own && jj >= x0 && jj < x1) vstore<VZ>((out1 + (long)jj * ldz) + uzl, q3[C8]);
         }
         // ---- step 3: u_{t+3} at column jjj = j - 8 (an owned column)
         const int jjj = j - 8;
@@ -896,9 +936,14 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
             add_source<VZ>(n3, jjj, six, zb, siz, w3);
             if (own) {
                 vstore<VZ>((out2 + (long)jjj * ldz) + uzl, n3);
-                if (rec3) {
-                    if (staged) sample_staged<VZ, ABC>(st, a.nt, n3, dxj, zm1, jjj - xa, zb, seis, t + 3 - a.tlag);
-                    else sample_receivers<VZ, ABC>(a, n3, jjj, zb, seis, t + 3 - a.tlag);
+                // the column's three samples: u_{t+1} (q2), u_{t+2} (q3), u_{t+3}
+                if (rec1 || rec2 || rec3) {
+                    if constexpr (staged)
+                        sample3_staged<VZ, ABC>(st, a.nt, q2[C0], q3[C4], n3, dxj, zm1, jjj - xa, zb,
+                                                seis, t + 1 - a.tlag, rec1, rec2, rec3);
+                    else
+                        sample3_receivers<VZ, ABC>(a, q2[C0], q3[C4], n3, jjj, zb, seis,
+                                                   t + 1 - a.tlag, rec1, rec2, rec3);
                 }
             }
         }
@@ -924,7 +969,7 @@ __global__ __launch_bounds__(64) void fwd3_kernel(FwdStep a, int nzseg, int nxch
 // One reverse step k: S_new = leapfrog(S_cur, S_old) over S_old, R likewise,
 // data injected into R_{k+1}, illum += S_new^2, image += S_new R_k or
 // gradient += R_k c lap(S_cur).
-template <int VZ, int MODE>
+template <int VZ, int MODE, bool ST>
 __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int k)
 {
@@ -953,8 +998,9 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
     const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
     const int zw = 4 + id.zseg * 64 * VZ;
     __shared__ RevStage st;
-    const bool staged = stage_rev(a, dat, j0, j1 - j0, zw, zw + 64 * VZ, inj_on, a.nt - 2 - k,
-                                  false, 0, false, 0, st, lane);
+    constexpr bool staged = ST;     // tables fit the LDS (the host checks)
+    if constexpr (ST) (void)stage_rev(a, dat, j0, j1 - j0, zw, zw + 64 * VZ, inj_on, a.nt - 2 - k,
+                                        false, 0, false, 0, st, lane);
 
     float qs[9][VZ], qr[9][VZ];
 #pragma unroll
@@ -1010,7 +1056,7 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
             }
         }
         if (inj_on) {
-            if (staged) inject_staged<VZ, 0>(st, a.idh2, nr, j - j0, zb);
+            if constexpr (staged) inject_staged<VZ, 0>(st, a.idh2, nr, j - j0, zb);
             else inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
         }
         if (active) {
@@ -1031,7 +1077,7 @@ __global__ __launch_bounds__(64) void rev1_kernel(RevStep a, int nzseg, int nxch
 // The step-1 image / illumination contributions (rounded products) wait in a
 // 5-column queue; image and illumination are read and written once per two
 // steps and summed in the same order as two single steps.
-template <int VZ, int MODE>
+template <int VZ, int MODE, bool ST>
 __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int k)
 {
@@ -1065,8 +1111,9 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
     const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
     const int xa = x0 - 4;                           // window [x0 - 4, x1 + 4)
     __shared__ RevStage st;
-    const bool staged = stage_rev(a, dat, xa, x1 + 4 - xa, zs, zs + 64 * VZ, inj1, a.nt - 2 - k,
-                                  inj2, a.nt - 3 - k, false, 0, st, lane);
+    constexpr bool staged = ST;     // tables fit the LDS (the host checks)
+    if constexpr (ST) (void)stage_rev(a, dat, xa, x1 + 4 - xa, zs, zs + 64 * VZ, inj1, a.nt - 2 - k,
+                                        inj2, a.nt - 3 - k, false, 0, st, lane);
 
     float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur: columns j-4 .. j+4
     float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R: columns j-8 .. j
@@ -1128,7 +1175,7 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj1) {
-                if (staged) inject_staged<VZ, 0>(st, a.idh2, nr, j - xa, zb);
+                if constexpr (staged) inject_staged<VZ, 0>(st, a.idh2, nr, j - xa, zb);
                 else inject_receivers<VZ>(a, nr, j, zb, dat, a.nt - 2 - k);
             }
             if (own && j >= x0 && j < x1) {
@@ -1182,7 +1229,7 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj2) {
-                if (staged) inject_staged<VZ, 1>(st, a.idh2, nr2, jj - xa, zb);
+                if constexpr (staged) inject_staged<VZ, 1>(st, a.idh2, nr2, jj - xa, zb);
                 else inject_receivers<VZ>(a, nr2, jj, zb, dat, a.nt - 3 - k);
             }
             if (own) {
@@ -1206,7 +1253,7 @@ __global__ __launch_bounds__(64) void rev2_kernel(RevStep a, int nzseg, int nxch
 // wavefields.  Image and illumination of a column are read when step 1
 // reaches it, take the three steps' rounded products in step order in a
 // 9-column queue of partial sums, and are written once when step 3 leaves it.
-template <int VZ, int MODE>
+template <int VZ, int MODE, bool ST>
 __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int k)
 {
@@ -1243,8 +1290,9 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
     const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
     const int xa = x0 - 8;                           // window [x0 - 8, x1 + 8)
     __shared__ RevStage st;
-    const bool staged = stage_rev(a, dat, xa, x1 + 8 - xa, zs, zs + 64 * VZ, inj1, a.nt - 2 - k,
-                                  inj2, a.nt - 3 - k, inj3, a.nt - 4 - k, st, lane);
+    constexpr bool staged = ST;     // tables fit the LDS (the host checks)
+    if constexpr (ST) (void)stage_rev(a, dat, xa, x1 + 8 - xa, zs, zs + 64 * VZ, inj1, a.nt - 2 - k,
+                                        inj2, a.nt - 3 - k, inj3, a.nt - 4 - k, st, lane);
 
     float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur
     float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R
@@ -1318,7 +1366,7 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj1 && jin1) {
-                if (staged) inject_staged<VZ, 0>(st, a.idh2, qr2[C8], j - xa, zb);
+                if constexpr (staged) inject_staged<VZ, 0>(st, a.idh2, qr2[C8], j - xa, zb);
                 else inject_receivers<VZ>(a, qr2[C8], j, zb, dat, a.nt - 2 - k);
             }
         }
@@ -1350,7 +1398,7 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj2 && jin2 && jj >= x0 - 4) {
-                if (staged) inject_staged<VZ, 1>(st, a.idh2, qr3[C8], jj - xa, zb);
+                if constexpr (staged) inject_staged<VZ, 1>(st, a.idh2, qr3[C8], jj - xa, zb);
                 else inject_receivers<VZ>(a, qr3[C8], jj, zb, dat, a.nt - 3 - k);
             }
             if (own && jj >= x0 && jj < x1) {
@@ -1388,7 +1436,7 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj3) {
-                if (staged) inject_staged<VZ, 2>(st, a.idh2, nr3, jjj - xa, zb);
+                if constexpr (staged) inject_staged<VZ, 2>(st, a.idh2, nr3, jjj - xa, zb);
                 else inject_receivers<VZ>(a, nr3, jjj, zb, dat, a.nt - 4 - k);
             }
             if (own) {
@@ -1624,33 +1672,53 @@ inline int grid1d(long n, int block, int cap = 8192)
     return (int)b;
 }
 
+template <int VZ, bool ABC, bool ST>
+void fwd_launch(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps, int nblocks,
+                hipStream_t s)
+{
+    if (steps == 3)
+        hipLaunchKernelGGL((fwd3_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    else if (steps == 2)
+        hipLaunchKernelGGL((fwd2_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    else
+        hipLaunchKernelGGL((fwd1_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+}
+
 template <int VZ>
 hipError_t fwd_dispatch(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
                         hipStream_t s)
 {
     const int ntiles = tl.nzseg * tl.nxchunk * nslots;
     const int nblocks = ((ntiles + 7) >> 3) << 3;
-    if (steps == 3) {
-        if (a.abc)
-            hipLaunchKernelGGL((fwd3_kernel<VZ, true>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
-        else
-            hipLaunchKernelGGL((fwd3_kernel<VZ, false>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
-    } else if (steps == 2) {
-        if (a.abc)
-            hipLaunchKernelGGL((fwd2_kernel<VZ, true>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
-        else
-            hipLaunchKernelGGL((fwd2_kernel<VZ, false>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    if (a.abc) {
+        if (tl.staged) fwd_launch<VZ, true, true>(a, tl, nslots, t, steps, nblocks, s);
+        else fwd_launch<VZ, true, false>(a, tl, nslots, t, steps, nblocks, s);
     } else {
-        if (a.abc)
-            hipLaunchKernelGGL((fwd1_kernel<VZ, true>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+        if (tl.staged) fwd_launch<VZ, false, true>(a, tl, nslots, t, steps, nblocks, s);
+        else fwd_launch<VZ, false, false>(a, tl, nslots, t, steps, nblocks, s);
+    }
+    return hipGetLastError();
+}
+
+template <int VZ, int MODE, bool ST>
+hipError_t rev_launch(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
+                      int nblocks, hipStream_t s)
+{
+    if (steps == 3) {
+        // six register queues per lane: four rows per lane do not fit
+        if constexpr (VZ == 4) return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((fwd1_kernel<VZ, false>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+            hipLaunchKernelGGL((rev3_kernel<VZ, MODE, ST>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+    } else if (steps == 2) {
+        hipLaunchKernelGGL((rev2_kernel<VZ, MODE, ST>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+    } else {
+        hipLaunchKernelGGL((rev1_kernel<VZ, MODE, ST>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
     }
     return hipGetLastError();
 }
@@ -1661,31 +1729,11 @@ hipError_t rev_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, i
 {
     const int ntiles = tl.nzseg * tl.nxchunk * nslots;
     const int nblocks = ((ntiles + 7) >> 3) << 3;
-    if (steps == 3) {
-        // six register queues per lane: four rows per lane do not fit
-        if constexpr (VZ == 4) return hipErrorInvalidValue;
-        else if (a.mode == 0)
-            hipLaunchKernelGGL((rev3_kernel<VZ, 0>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
-        else
-            hipLaunchKernelGGL((rev3_kernel<VZ, 1>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
-    } else if (steps == 2) {
-        if (a.mode == 0)
-            hipLaunchKernelGGL((rev2_kernel<VZ, 0>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
-        else
-            hipLaunchKernelGGL((rev2_kernel<VZ, 1>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
-    } else {
-        if (a.mode == 0)
-            hipLaunchKernelGGL((rev1_kernel<VZ, 0>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
-        else
-            hipLaunchKernelGGL((rev1_kernel<VZ, 1>), dim3(nblocks), dim3(64), 0, s, a,
-                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
-    }
-    return hipGetLastError();
+    if (a.mode == 0)
+        return tl.staged ? rev_launch<VZ, 0, true>(a, tl, nslots, k, steps, nblocks, s)
+                         : rev_launch<VZ, 0, false>(a, tl, nslots, k, steps, nblocks, s);
+    return tl.staged ? rev_launch<VZ, 1, true>(a, tl, nslots, k, steps, nblocks, s)
+                     : rev_launch<VZ, 1, false>(a, tl, nslots, k, steps, nblocks, s);
 }
 
 }  // namespace
@@ -1736,16 +1784,17 @@ template <int VZ>
 int resident_vz(int kind, int steps)
 {
     if (kind == 0)
-        return steps == 3   ? resident_of(fwd3_kernel<VZ, false>)
-               : steps == 2 ? resident_of(fwd2_kernel<VZ, false>)
-                            : resident_of(fwd1_kernel<VZ, false>);
+        return steps == 3   ? resident_of(fwd3_kernel<VZ, false, true>)
+               : steps == 2 ? resident_of(fwd2_kernel<VZ, false, true>)
+                            : resident_of(fwd1_kernel<VZ, false, true>);
     if (kind == 2)
-        return steps == 3   ? resident_of(fwd3_kernel<VZ, true>)
-               : steps == 2 ? resident_of(fwd2_kernel<VZ, true>)
-                            : resident_of(fwd1_kernel<VZ, true>);
+        return steps == 3   ? resident_of(fwd3_kernel<VZ, true, true>)
+               : steps == 2 ? resident_of(fwd2_kernel<VZ, true, true>)
+                            : resident_of(fwd1_kernel<VZ, true, true>);
     if constexpr (VZ != 4)
-        if (steps == 3) return resident_of(rev3_kernel<VZ, 0>);
-    return steps == 2 ? resident_of(rev2_kernel<VZ, 0>) : resident_of(rev1_kernel<VZ, 0>);
+        if (steps == 3) return resident_of(rev3_kernel<VZ, 0, true>);
+    return steps == 2 ? resident_of(rev2_kernel<VZ, 0, true>)
+                      : resident_of(rev1_kernel<VZ, 0, true>);
 }
 }  // namespace
 
