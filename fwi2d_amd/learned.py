@@ -90,6 +90,12 @@ TRAIN_SMALL = FwiSetup(nz=128, nx=384, dh=10.0, nt=1875, dt=8e-4, nsrc=16, nrec=
                        peaks=(4.0, 8.0, 12.0), band_iters=(3, 3, 2), zmask=60.0)
 COMPARE_SMALL = dataclasses.replace(TRAIN_SMALL, nsrc=32, nrec=384, seed=5,
                                     band_iters=(7, 7, 6))
+# Training at half the full size (200 x 800 at 10 m, 64 shots): the model
+# families scale their structures with the grid, so these gradients have
+# features closer in cells to the full-size comparison's.
+TRAIN_MID = FwiSetup(nz=200, nx=800, dh=10.0, nt=3126, dt=8e-4, nsrc=64, nrec=400,
+                     zsrc=20.0, zrec=10.0, rbc_nb=100, abc_nb=80, varVp_rel=0.4, seed=4,
+                     peaks=(4.0, 8.0, 12.0), band_iters=(3, 3, 2), zmask=60.0)
 # The comparison configuration at full size: 401 x 1601 at 10 m, 200-cell
 # random boundary, 256 shots, 1601 receivers, nt 5001 at 0.6 ms, bands
 # 4 / 8 / 12 Hz (20 iterations: 7 / 7 / 6).
@@ -371,6 +377,8 @@ def main(argv=None):
     p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     p.add_argument("--out", default="learned_run")
     p.add_argument("--train-models", type=int, default=48)
+    p.add_argument("--train-size", choices=("small", "mid"), default="small",
+                   help="training-sample survey: 128 x 384 (16 shots) or 200 x 800 (64 shots)")
     p.add_argument("--epochs", type=int, default=40)
     p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--compare-seed", type=int, default=10005)
@@ -378,7 +386,8 @@ def main(argv=None):
     p.add_argument("--size", choices=("small", "full"), default="small",
                    help="comparison at the reduced size or at the full config-5 size")
     p.add_argument("--runs", default="plain,control,learned",
-                   help="comma-separated runs of the comparison (plain, control, learned)")
+                   help="comma-separated runs of the comparison (plain, control, learned); "
+                        "empty: train only")
     p.add_argument("--bands", default="", help="iterations per band of the comparison runs")
     p.add_argument("--short-bands", default="",
                    help="iterations per band of extra, shorter control and learned runs")
@@ -392,7 +401,8 @@ def main(argv=None):
         cos = (None, None)
     else:
         t0 = time.perf_counter()
-        x, y, g, scales = make_samples(TRAIN_SMALL, range(1, 1 + a.train_models), log)
+        ts = TRAIN_SMALL if a.train_size == "small" else TRAIN_MID
+        x, y, g, scales = make_samples(ts, range(1, 1 + a.train_models), log)
         t_gen = time.perf_counter() - t0
         t0 = time.perf_counter()
         params, th, val = train(x, y, epochs=a.epochs, lr=a.lr, log=log)
@@ -400,6 +410,10 @@ def main(argv=None):
         cos = holdout_cosines(params, x, y, g, val)
         log({"holdout_cosine_learned": cos[0], "holdout_cosine_plain": cos[1]})
     params.tofile(os.path.join(a.out, "preconditioner_params.bin"))
+    if not a.runs:                    # training only
+        log({"samples": int(len(x)), "generation_s": t_gen, "training_s": t_train,
+             "holdout_cosine": {"learned": cos[0], "plain_direction": cos[1]}})
+        return
 
     s = COMPARE_SMALL if a.size == "small" else COMPARE_FULL
     if a.bands:

@@ -1,11 +1,21 @@
-"""Writes tests/golden/preconditioner_48models.bin: the learned preconditioner
-trained by this project's own kernels on plain-FWI samples.
+"""Writes the learned-preconditioner fixtures: networks trained by this
+project's own kernels on plain-FWI samples.
 
-    python tests/golden/make_preconditioner.py      (needs the GPU, ~4 minutes)
+    python tests/golden/make_preconditioner.py small   (GPU, ~4 minutes)
+    python tests/golden/make_preconditioner.py mid     (GPU, ~8 minutes)
 
-48 seeded training models of the config-4 family at 128 x 384 (10 m),
-8 plain FWI iterations each over bands 4/8/12 Hz (384 samples), 40 epochs of
-Adam at lr 1e-3, batch 16, no mirroring (fwi2d_amd.learned.TRAIN_SMALL).
+small -> preconditioner_48models.bin: 48 seeded training models of the
+config-4 family at 128 x 384 (10 m, 16 shots), 8 plain FWI iterations each
+over bands 4/8/12 Hz (384 samples), 40 epochs of Adam at lr 1e-3, batch 16,
+no mirroring (fwi2d_amd.learned.TRAIN_SMALL).  The small-size comparisons and
+the GPU tests use it.
+
+mid -> preconditioner_mid48.bin: the same 48 seeds at 200 x 800 (10 m,
+64 shots; fwi2d_amd.learned.TRAIN_MID), 384 samples, 40 epochs, mirrored
+samples (what `python -m fwi2d_amd.learned --train-size mid --runs ""`
+writes).  The full-size (401 x 1601) comparison uses it: the model families
+scale their structures with the grid, so mid-size gradients are closer in
+cells to the full-size ones.
 """
 import os
 import sys
@@ -16,13 +26,14 @@ sys.path.insert(0, ROOT)
 from fwi2d_amd import learned as L  # noqa: E402
 
 
-def main():
-    x, y, g, _ = L.make_samples(L.TRAIN_SMALL, range(1, 49), log=print)
-    params, hist, val = L.train(x, y, epochs=40, lr=1e-3, flip=False, log=print)
+def main(size):
+    setup, flip, name = {"small": (L.TRAIN_SMALL, False, "preconditioner_48models.bin"),
+                         "mid": (L.TRAIN_MID, True, "preconditioner_mid48.bin")}[size]
+    x, y, g, _ = L.make_samples(setup, range(1, 49), log=print)
+    params, hist, val = L.train(x, y, epochs=40, lr=1e-3, flip=flip, log=print)
     print("held-out cosine (learned, plain):", L.holdout_cosines(params, x, y, g, val))
-    params.tofile(os.path.join(os.path.dirname(os.path.abspath(__file__)),
-                               "preconditioner_48models.bin"))
+    params.tofile(os.path.join(os.path.dirname(os.path.abspath(__file__)), name))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else "small")
