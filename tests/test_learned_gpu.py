@@ -4,9 +4,10 @@
   learned run equals the control run step for step;
 * an ascent direction (+g) never passes the acceptance rule, so the run
   equals plain FWI step for step ("never worse than the plain step");
-* the trained network (tests/golden/preconditioner_48models.bin, made by
-  tests/golden/make_preconditioner.py) gains over plain FWI on a held-out
-  model of the comparison family (misfit and model error).
+* the trained networks (tests/golden/preconditioner_48models.bin and
+  preconditioner_mid48.bin, made by tests/golden/make_preconditioner.py)
+  gain over plain FWI on a held-out model of the comparison family at their
+  training size (misfit and model error).
 """
 import dataclasses
 import os
@@ -78,4 +79,28 @@ def test_trained_network_gains_over_plain_fwi():
     f = {k: h[-1]["misfit"] for k, h in runs.items()}
     e = {k: h[-1]["model_rms_error"] for k, h in runs.items()}
     assert f["learned"] < 0.8 * f["plain"]
+    assert e["learned"] < e["plain"]
+
+
+def test_mid_trained_network_gains_over_plain_fwi():
+    """The network trained at 200 x 800 (tests/golden/preconditioner_mid48.bin,
+    the one the full-size comparison uses) on a held-out model of the
+    comparison family at that size (64 shots, bands 4/8/12 Hz, 2 iterations
+    each): the learned run ends with a lower misfit and a lower model error
+    than plain FWI."""
+    params = np.fromfile(os.path.join(GOLDEN, "preconditioner_mid48.bin"), np.float32)
+    s = dataclasses.replace(L.TRAIN_MID, seed=5, band_iters=(2, 2, 2))
+    v, v0 = C.config5_models(30004, s.nz, s.nx)
+    fwi = L.Fwi(s, v, v0)
+    d = L.Direction(s.nx, s.nz, params)
+    runs = {"plain": L.run_fwi(fwi, None)[0], "learned": L.run_fwi(fwi, d)[0]}
+    d.close()
+    fwi.close()
+    for k, h in runs.items():
+        print(f"{k:8s} final misfit {h[-1]['misfit']:.4e}  model rms error "
+              f"{h[-1]['model_rms_error']:.1f} m/s  time {h[-1]['time']:.2f} s  "
+              f"trial steps {sum(e['taken'] == 'learned' for e in h)}")
+    f = {k: h[-1]["misfit"] for k, h in runs.items()}
+    e = {k: h[-1]["model_rms_error"] for k, h in runs.items()}
+    assert f["learned"] < f["plain"]
     assert e["learned"] < e["plain"]
