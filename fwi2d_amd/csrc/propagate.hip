@@ -13,14 +13,19 @@
 //  * the +-4 z-neighbours come from the neighbouring lanes by DPP whole-wave
 //    shifts; the 4 rows beyond each end of the wave come from one prefetched
 //    float4 "halo" load per column;
-//  * two time steps per launch (temporal blocking): u_{t+1} is computed with
-//    a 4-row / 4-column margin and kept in a second register queue, u_{t+2}
-//    on the inner rows; c, read for the first step, waits in an LDS ring for
-//    the second;
+//  * one, two or three time steps per launch (temporal blocking): each step
+//    after the first is computed on 8 fewer rows and columns than the one
+//    before, from a register queue of the previous step's values; c, read
+//    for the first step, waits in an LDS ring for the later ones; a launch
+//    writes only the two fields the next one needs;
+//  * the three-step kernels unroll their column loop over the nine phases
+//    of the queues, so the queues advance by renaming, not by moves, and
+//    their loads are branch-free (columns clamped into the grid);
 //  * source injection, the Cerjan sponge, receiver sampling (forward) and
 //    data injection, imaging / gradient and illumination (reverse) are fused
 //    into the same pass; the receiver tables and data of a wave's columns
-//    are staged in LDS once per launch;
+//    are staged in LDS once per launch (a second build of each kernel reads
+//    them from global memory, for launches whose windows do not fit);
 //  * blocks are renumbered so that each XCD receives a contiguous run of
 //    tiles: neighbouring chunks share their halo columns in that XCD's L2.
 //
@@ -30,7 +35,7 @@
 // reference's recurrence with less rounding error (about 4-5x less than the
 // reference's order against an FP64 evaluation of the same recurrence, over
 // 2,000+ steps).  Every multiply-add that feeds a stored value is an explicit
-// fmaf, so all launch shapes (one or two steps per launch, any tiling)
+// fmaf, so all launch shapes (one to three steps per launch, any tiling)
 // produce bit-identical fields, images and gradients.
 //
 // Source injection is folded into the write of the field it enters:
