@@ -61,6 +61,7 @@ int or_wavelet(int nt, double dt_in, double fmax_in, double *w)
     /* tlag in single precision, as modeling.cu:36-39 computes it */
     const float t0f = 2.0f * sqrtf((float)M_PI) / (float)fmax;
     const int tlag = (int)(t0f / (float)dt) + 1;
+    if (nt < 1) return tlag;
     double *s = (double *)malloc((size_t)nt * sizeof(double));
     double *re = (double *)malloc((size_t)(nt / 2 + 1) * sizeof(double));
     double *im = (double *)malloc((size_t)(nt / 2 + 1) * sizeof(double));
