@@ -101,7 +101,23 @@ def cpu_baseline(s, vmig, xs, zs, xr, zr, w, tlag, obs, threads, nshots):
             "kind": "port",
             "sample": f"{nshots} of {s.nsrc} shots, full RTM each (RB forward + reverse "
                       f"imaging), {nzz}x{nxx} cells x {T} steps, FP32 OpenMP, "
-                      f"{el:.1f} s of CPU time"}
+                      f"{el:.1f} s of CPU time",
+            "host": host_cpu()}
+
+
+def host_cpu():
+    """CPU model and logical CPU count of this host (Linux), for the record."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs"
+
 
 
 def compulsory_bpc(kind, steps):
