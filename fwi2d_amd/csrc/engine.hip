@@ -68,8 +68,8 @@ struct GridBuf {
 };
 
 // Four buffers of one wavefield: `cur` holds the latest step, `old` the one
-// before.  A single step writes over old; a double step writes the two free
-// buffers in turn.
+// before.  A single step writes over old; a launch of two or three steps
+// writes its last two steps to the two free buffers.
 struct Quad {
     float *b[4];
     int cur, old;

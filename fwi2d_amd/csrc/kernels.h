@@ -19,9 +19,9 @@ struct Grid {
     long fstride;      // floats per field per shot slot
 };
 
-// Wave tiling of one launch: each 64-lane wave owns 64*vz rows of one x-chunk
-// of xw columns of one shot slot (in a two-step launch 64*vz rows are
-// computed and the inner 64*vz - 8 are written).
+// Wave tiling of one launch: each 64-lane wave computes 64*vz rows of one
+// x-chunk of xw columns of one shot slot and writes the inner
+// 64*vz - 8*(steps-1) of them (steps per launch: 1, 2 or 3).
 struct Tiling {
     int vz;            // rows per lane: 1, 2 or 4
     int xw;            // columns per wave
@@ -65,8 +65,8 @@ struct FwdStep {
 };
 
 // One reverse launch at reverse step k.  S: cur = u_{T-1-k}, old = u_{T-k};
-// R: cur = R_k, old = R_{k-1}.  Single step writes over the olds; two steps
-// write steps k+1 / k+2 to the out buffers.
+// R: cur = R_k, old = R_{k-1}.  Single step writes over the olds; a launch
+// of s = 2 or 3 steps writes steps k+s-1 / k+s to the out buffers.
 struct RevStep {
     Grid g;
     const float *scur, *sold;
@@ -90,7 +90,7 @@ hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t
                            hipStream_t s);
 hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
                            hipStream_t s);
-// z segments of a launch shape (two-step launches write 64*vz - 8 rows per wave)
+// z segments of a launch shape (a launch of s steps writes 64*vz - 8(s-1) rows per wave)
 int zsegments(int nzz, int vz, int steps);
 // Waves of the random-boundary forward (kind 0), imaging reverse (kind 1) or
 // sponge forward (kind 2) step kernel that the current device holds at once
