@@ -45,8 +45,9 @@ struct RecTable {
 };
 
 // One forward launch at step t.  Single step: reads cur = u_t and
-// old = u_{t-1}, writes u_{t+1} over old (out1 == old).  Two steps: reads
-// cur, old; writes u_{t+1} to out1 and u_{t+2} to out2 (four distinct buffers).
+// old = u_{t-1}, writes u_{t+1} over old (out1 == old).  s = 2 or 3 steps:
+// reads cur, old; writes u_{t+s-1} to out1 and u_{t+s} to out2 (four
+// distinct buffers).
 struct FwdStep {
     Grid g;
     const float *cur;
