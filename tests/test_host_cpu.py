@@ -115,3 +115,20 @@ def test_mains_fail_cleanly_before_gpu_work(exe, tmp_path):
     assert r.returncode == 1 and "missing" in r.stderr
     r = run(path, str(tmp_path / "absent.txt"))
     assert r.returncode == 1 and "could not be opened" in r.stderr
+
+
+def test_viewer_parser_agrees_with_the_executables(driver, tmp_path):
+    """fwi2d_amd.viewer reads parameters.txt with the executables' rules."""
+    from fwi2d_amd import viewer
+    p = tmp_path / "parameters.txt"
+    p.write_text(PARAMS)
+    for key in ("x_samples", "x_samples_extra", "z_samples", "model_spacing", "model_file",
+                "quoted", "listed", "tight", "indented", "commented", "empty", "nope",
+                "x_sample"):
+        r = run(driver, "param", key, str(p))
+        if r.returncode == 0:
+            assert "[" + viewer.read_parameter(str(p), key) + "]" == r.stdout.strip(), key
+        else:
+            with pytest.raises(KeyError):
+                viewer.read_parameter(str(p), key)
+            assert viewer.read_parameter(str(p), key, "") == "", key
