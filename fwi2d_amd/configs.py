@@ -6,6 +6,11 @@ run/generate_input_data.py:22-46 -- direct-wave mute) and adds the seeded
 model families the benchmark configurations name (flat / dipping layers,
 Gaussian lenses, salt-like polygons).  Every model is [nx, nz] float32 with z
 fastest, the reference's file layout.
+
+The command line (`python -m fwi2d_amd.configs setup | init | mute`, see
+`main`) writes the files the three executables read: a named
+configuration's models, geometry and parameter files, a smoothed starting
+model of any model file, and the muted migration inputs.
 """
 import dataclasses
 import os
@@ -250,3 +255,172 @@ def ensure_dirs(root):
     for p in folders.values():
         os.makedirs(p, exist_ok=True)
     return folders
+
+
+# ------------------------------------------------------------ command line
+#
+#   python -m fwi2d_amd.configs setup NAME DIR [--seed S] [--depth-mask M] [--max-iteration N]
+#   python -m fwi2d_amd.configs init parameters.txt [--sigma 3] [--sources X0,X1,N,Z]
+#                                                   [--receivers X0,X1,N,Z]
+#   python -m fwi2d_amd.configs mute parameters.txt [--velocity 3600] [--delay 0.18] [--gstd 30]
+
+def named_models(name, seed=None):
+    """(survey, true model, smoothed starting model) of a named configuration."""
+    if name == CONFIG1.name:
+        v = config1_model(1 if seed is None else seed)
+        return CONFIG1, v, smooth_slowness(v, 10.0)
+    makers = {CONFIG2.name: (CONFIG2, config2_models, 2), CONFIG3.name: (CONFIG3, config3_models, 3),
+              CONFIG5.name: (CONFIG5, config5_models, 5)}
+    if name not in makers:
+        raise SystemExit(f"unknown configuration '{name}' "
+                         f"(one of {', '.join([CONFIG1.name] + list(makers))})")
+    s, make, default = makers[name]
+    true, init = make(default if seed is None else seed)
+    return s, true, init
+
+
+def seismogram_prefix(nt, nrec, dt):
+    """File prefix of one shot's seismogram as the modeling executable writes it:
+    seismogram_nt{nt}_nr{nrec}_{int(1e6f * dt)}us_shot_ (dt as a float32)."""
+    return f"seismogram_nt{nt}_nr{nrec}_{int(np.float32(1e6) * np.float32(dt))}us_shot_"
+
+
+def setup(name, root, seed=None, depth_mask=0.0, max_iteration=5):
+    """Writes everything the three executables read for a named configuration:
+    the true and starting models (models/<name>_{true,init}_{nz}x{nx}_{dh}m.bin;
+    the starting model keeps the rows above depth_mask from the true one, as
+    run/generate_setup.py:47-51 does), SPS/RPS and parameters_{modeling,
+    migration,inversion}.txt.  Migration reads the muted input_RTM_ files that
+    the `mute` command writes; inversion reads the raw seismograms.  Returns
+    the three parameter file paths."""
+    s, true, init = named_models(name, seed)
+    folders = ensure_dirs(root)
+    keep = int(depth_mask / s.dh)
+    if keep > 0:
+        init[:, :keep] = true[:, :keep]
+    tag = f"{s.nz}x{s.nx}_{int(s.dh)}m.bin"
+    true_file = os.path.join(folders["models"], f"{name}_true_{tag}")
+    init_file = true_file.replace("_true_", "_init_")
+    true.astype(np.float32).tofile(true_file)
+    init.astype(np.float32).tofile(init_file)
+    sps, rps = os.path.join(root, "SPS.txt"), os.path.join(root, "RPS.txt")
+    write_geometry(sps, rps, *s.acquisition())
+    prefix = seismogram_prefix(s.nt, s.nrec, s.dt)
+    out = []
+    for step, model in (("modeling", true_file), ("migration", init_file),
+                        ("inversion", init_file)):
+        p = os.path.join(root, f"parameters_{step}.txt")
+        write_parameters(p, s, model, sps, rps, folders, max_iteration=max_iteration,
+                         depth_mask=depth_mask, mig_prefix="input_RTM_" + prefix,
+                         inv_prefix=prefix)
+        out.append(p)
+    return out
+
+
+def _line(text):
+    x0, x1, n, z = text.split(",")
+    n = int(n)
+    xs = np.linspace(float(x0), float(x1), n)
+    return xs, np.full(n, float(z))
+
+
+def initial_model(parameters, sigma=3.0, sources=None, receivers=None):
+    """run/generate_setup.py:23-53 on the files a parameters.txt names: writes
+    1 / gaussian(1 / model, sigma cells) with the rows above depth_mask copied
+    from the model, to model_file with "true" replaced by "init"; when
+    `sources` / `receivers` ("x0,x1,n,z") are given, first writes SPS / RPS
+    with n evenly spaced points from x0 to x1 at depth z.  Returns the
+    starting model's path."""
+    from .viewer import read_parameter, stoi
+    model_file = read_parameter(parameters, "model_file")
+    if "true" not in os.path.basename(model_file):
+        raise SystemExit(f"model_file '{model_file}' has no 'true' in its name: the "
+                         "starting model would overwrite it")
+    nx = stoi(read_parameter(parameters, "x_samples"))
+    nz = stoi(read_parameter(parameters, "z_samples"))
+    dh = float(np.float32(read_parameter(parameters, "model_spacing")))
+    zmask = float(np.float32(read_parameter(parameters, "depth_mask", "0")))
+    if sources:
+        xs, zs = _line(sources)
+        np.savetxt(read_parameter(parameters, "SPS"), np.stack([xs, zs], 1), fmt="%.2f",
+                   delimiter=",")
+    if receivers:
+        xr, zr = _line(receivers)
+        np.savetxt(read_parameter(parameters, "RPS"), np.stack([xr, zr], 1), fmt="%.2f",
+                   delimiter=",")
+    v = np.fromfile(model_file, np.float32, count=nx * nz)
+    if v.size != nx * nz:
+        raise SystemExit(f"{model_file}: {v.size} floats, expected {nx} x {nz}")
+    v = v.reshape(nx, nz)
+    init = smooth_slowness(v, sigma, keep_rows=int(zmask / dh))
+    out = os.path.join(os.path.dirname(model_file),
+                       os.path.basename(model_file).replace("true", "init"))
+    init.tofile(out)
+    return out
+
+
+def mute(parameters, velocity=3600.0, delay=0.18, gstd=30.0):
+    """run/generate_input_data.py:22-46 on the modeling executable's output:
+    each shot's seismogram in mod_output_folder is divided by its max |.|,
+    the samples before (distance / velocity + delay) / dt get the Gaussian
+    taper, and the result is written beside it as input_RTM_<file>.  Returns
+    the written paths."""
+    from .viewer import read_parameter, stoi
+    nt = stoi(read_parameter(parameters, "time_samples"))
+    dt = float(read_parameter(parameters, "time_spacing"))
+    folder = read_parameter(parameters, "mod_output_folder", "")
+    sps = np.loadtxt(read_parameter(parameters, "SPS"), np.float32, delimiter=",", ndmin=2)
+    rps = np.loadtxt(read_parameter(parameters, "RPS"), np.float32, delimiter=",", ndmin=2)
+    nr = len(rps)
+    prefix = seismogram_prefix(nt, nr, dt)
+    out = []
+    for k in range(len(sps)):
+        src = f"{folder}{prefix}{k + 1}.bin"
+        data = np.fromfile(src, np.float32)
+        if data.size != nr * nt:
+            raise SystemExit(f"{src}: {data.size} floats, expected {nr} x {nt}")
+        d = mute_direct_wave(data.reshape(nr, nt), sps[k, 0], sps[k, 1], rps[:, 0], rps[:, 1],
+                             dt, float(velocity), float(delay), float(gstd))
+        dst = f"{folder}input_RTM_{prefix}{k + 1}.bin"
+        d.astype(np.float32).tofile(dst)
+        out.append(dst)
+    return out
+
+
+def main(argv=None):
+    import argparse
+    p = argparse.ArgumentParser(prog="python -m fwi2d_amd.configs",
+                                description="input files of the modeling, migration and "
+                                            "inversion executables")
+    sub = p.add_subparsers(dest="cmd", required=True)
+    ps = sub.add_parser("setup", help="models, geometry and parameter files of a named "
+                                      "configuration")
+    ps.add_argument("name", help=", ".join(s.name for s in (CONFIG1, CONFIG2, CONFIG3, CONFIG5)))
+    ps.add_argument("dir")
+    ps.add_argument("--seed", type=int, default=None)
+    ps.add_argument("--depth-mask", type=float, default=0.0)
+    ps.add_argument("--max-iteration", type=int, default=5)
+    pi = sub.add_parser("init", help="smoothed-slowness starting model of model_file")
+    pi.add_argument("parameters")
+    pi.add_argument("--sigma", type=float, default=3.0, help="Gaussian width in cells")
+    pi.add_argument("--sources", default=None, help="x0,x1,n,z: rewrite SPS")
+    pi.add_argument("--receivers", default=None, help="x0,x1,n,z: rewrite RPS")
+    pm = sub.add_parser("mute", help="direct-wave-muted migration inputs (input_RTM_ files)")
+    pm.add_argument("parameters")
+    pm.add_argument("--velocity", type=float, default=3600.0)
+    pm.add_argument("--delay", type=float, default=0.18)
+    pm.add_argument("--gstd", type=float, default=30.0)
+    a = p.parse_args(argv)
+    if a.cmd == "setup":
+        written = setup(a.name, a.dir, a.seed, a.depth_mask, a.max_iteration)
+    elif a.cmd == "init":
+        written = [initial_model(a.parameters, a.sigma, a.sources, a.receivers)]
+    else:
+        written = mute(a.parameters, a.velocity, a.delay, a.gstd)
+    for w in written:
+        print(w)
+    return written
+
+
+if __name__ == "__main__":
+    main()

@@ -23,6 +23,9 @@ int main(int argc, char** argv) {
             fwi::Geometry g; g.set_parameters(argv[2]);
             std::cout << g.nsrc << " " << g.nrec << " " << g.xsrc.back() << " "
                       << g.zrec.front() << std::endl;
+        } else if (std::string(argv[1]) == "us") {    // survey.cpp seismogram file names
+            const float dt = fwi::param_float("time_spacing", argv[2]);
+            std::cout << std::to_string(int(1e6f * dt)) << std::endl;
         } else if (std::string(argv[1]) == "bin") {
             std::vector<float> a(std::stol(argv[3]));
             fwi::import_binary_float(argv[2], a.data(), (long)a.size());
@@ -132,3 +135,14 @@ def test_viewer_parser_agrees_with_the_executables(driver, tmp_path):
             with pytest.raises(KeyError):
                 viewer.read_parameter(str(p), key)
             assert viewer.read_parameter(str(p), key, "") == "", key
+
+
+@pytest.mark.parametrize("dt", ["1e-3", "8e-4", "6e-4", "5e-4", "3e-4", "0.00025", "7e-5"])
+def test_prep_file_names_agree_with_the_executables(driver, tmp_path, dt):
+    """fwi2d_amd.configs names seismogram files with the executables' float32
+    rounding of 1e6 * dt."""
+    from fwi2d_amd import configs as C
+    p = tmp_path / "p.txt"
+    p.write_text(f"time_spacing = {dt}\n")
+    us = run(driver, "us", str(p)).stdout.strip()
+    assert C.seismogram_prefix(2001, 51, float(dt)) == f"seismogram_nt2001_nr51_{us}us_shot_"

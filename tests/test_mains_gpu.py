@@ -149,3 +149,52 @@ def test_inversion_resume_equals_straight_run(tmp_path):
     ra = open(os.path.join(fa["residuo"], "convergence_3_iterations.txt")).read()
     rb = open(os.path.join(fb["residuo"], "convergence_3_iterations.txt")).read()
     assert ra == rb
+
+
+def test_prepared_configuration_end_to_end(tmp_path):
+    """The data-preparation commands feed the executables: setup of the
+    one-shot 121 x 401 configuration, modeling, the direct-wave mute,
+    migration of the muted data (whose image equals the library's on the
+    same muted data), a two-iteration inversion and the viewer's figures."""
+    from fwi2d_amd import viewer
+    root = str(tmp_path)
+    mod, mig, inv = C.main(["setup", "forward-1shot", root, "--max-iteration", "2"])
+    s = C.CONFIG1
+    run("modeling", mod)
+    muted = C.main(["mute", mod])
+    assert len(muted) == 1
+    obs = np.fromfile(muted[0], np.float32).reshape(1, s.nrec, s.nt)
+    assert np.isfinite(obs).all() and np.abs(obs).max() == pytest.approx(1.0, abs=1e-6)
+
+    run("migration", mig)
+    suffix = f"{int(s.fmax)}Hz_{s.nz}x{s.nx}_{int(s.dh)}m.bin"
+    img = np.fromfile(os.path.join(viewer.read_parameter(mig, "mig_output_folder"),
+                                   f"RTM_section_{suffix}"), np.float32).reshape(s.nx, s.nz)
+    sps = np.loadtxt(viewer.read_parameter(mig, "SPS"), np.float32, delimiter=",", ndmin=2)
+    rps = np.loadtxt(viewer.read_parameter(mig, "RPS"), np.float32, delimiter=",", ndmin=2)
+    init = np.fromfile(viewer.read_parameter(mig, "model_file"), np.float32).reshape(s.nx, s.nz)
+    w, tlag = F.ricker_wavelet(s.nt, s.dt, s.fmax)
+    eng = F.Engine(s.nz, s.nx, s.dh, s.nt, s.dt, abc_nb=s.abc_nb, abc_factor=s.abc_factor,
+                   rbc_nb=s.nb, rbc_varVp=s.varVp, rbc_varVp_rel=s.varVp_rel, rbc_seed=s.seed,
+                   max_batch=1)
+    eng.set_wavelet(w, tlag)
+    eng.set_geometry(sps[:, 0], sps[:, 1], rps[:, 0], rps[:, 1])
+    eng.set_model(init)
+    eng.set_observed(obs)
+    eng.reset()
+    eng.migration()
+    ref_img, _, _ = eng.get_image()
+    eng.close()
+    assert np.array_equal(img, ref_img)
+    assert np.isfinite(img).all() and np.abs(img).max() > 0
+
+    out = run("inversion", inv)
+    assert "residuo" in out
+    res = np.loadtxt(os.path.join(viewer.read_parameter(inv, "inv_residuo_folder"),
+                                  "convergence_2_iterations.txt"))
+    assert res.shape == (3,) and np.isfinite(res).all()
+    figs = viewer.main([inv, "--out", os.path.join(root, "figures"),
+                        "--true", viewer.read_parameter(mod, "model_file")])
+    assert sorted(os.path.basename(f) for f in figs) == sorted(
+        ["model.png", "seismogram_shot_1.png", "image_RTM.png", "residuo_FWI.png",
+         "model_FWI.png"])
