@@ -95,6 +95,8 @@ NN_SIGNATURES = [
     ("fwi_nn_num_params", C.c_int, [_ctx, C.POINTER(C.c_int64)]),
     ("fwi_nn_get_params", C.c_int, [_ctx, _f32p]),
     ("fwi_nn_set_params", C.c_int, [_ctx, _f32p]),
+    ("fwi_nn_get_optimizer", C.c_int, [_ctx, _f32p, _f32p, C.POINTER(C.c_int64)]),
+    ("fwi_nn_set_optimizer", C.c_int, [_ctx, _f32p, _f32p, C.c_int64]),
     ("fwi_nn_forward", C.c_int, [_ctx, C.c_int32, _f32p, _f32p]),
     ("fwi_nn_gradient", C.c_int, [_ctx, C.c_int32, _f32p, _f32p, C.POINTER(C.c_float), _vp]),
     ("fwi_nn_train_step", C.c_int, [_ctx, C.c_int32, _f32p, _f32p, C.c_float,
@@ -341,6 +343,21 @@ class Net:
 
     def set_params(self, p):
         _check(lib().fwi_nn_set_params(self._h, np.ascontiguousarray(p, np.float32)))
+
+    def get_optimizer(self):
+        """Adam state (m, v, steps taken) in the flat parameter layout."""
+        m = np.zeros(self.nparams, np.float32)
+        v = np.zeros(self.nparams, np.float32)
+        t = C.c_int64()
+        _check(lib().fwi_nn_get_optimizer(self._h, m, v, C.byref(t)))
+        return m, v, t.value
+
+    def set_optimizer(self, m, v, steps):
+        m = np.ascontiguousarray(m, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        if m.size != self.nparams or v.size != self.nparams:
+            raise ValueError(f"Adam moments need {self.nparams} values each")
+        _check(lib().fwi_nn_set_optimizer(self._h, m, v, int(steps)))
 
     def forward(self, x):
         """x: [nb, 3, H, W] -> y: [nb, H, W]."""

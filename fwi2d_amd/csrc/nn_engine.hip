@@ -420,6 +420,38 @@ int fwi_nn_set_params(fwi_nn *n, const float *params)
     return FWI_OK;
 }
 
+int fwi_nn_get_optimizer(fwi_nn *n, float *m, float *v, int64_t *step)
+{
+    if (!n || !m || !v || !step) return fail(FWI_ERR_ARG, "null argument");
+    std::vector<float> master(n->nparams);
+    NN_TRY(hipStreamSynchronize(n->stream));
+    NN_TRY(hipMemcpy(master.data(), n->m, n->nparams * 4, hipMemcpyDeviceToHost));
+    to_external(n, master, m);
+    NN_TRY(hipMemcpy(master.data(), n->v, n->nparams * 4, hipMemcpyDeviceToHost));
+    to_external(n, master, v);
+    *step = n->t;
+    return FWI_OK;
+}
+
+int fwi_nn_set_optimizer(fwi_nn *n, const float *m, const float *v, int64_t step)
+{
+    if (!n || !m || !v) return fail(FWI_ERR_ARG, "null argument");
+    if (step < 0 || step > 0x7fffffff)
+        return fail(FWI_ERR_ARG, "Adam step count %lld out of range", (long long)step);
+    const long count = external_count(n);
+    for (long i = 0; i < count; ++i)
+        if (!std::isfinite(m[i]) || !std::isfinite(v[i]) || v[i] < 0.f)
+            return fail(FWI_ERR_ARG, "Adam moment %ld is not finite (or second moment < 0)", i);
+    std::vector<float> mm, vv;
+    from_external(n, m, mm);
+    from_external(n, v, vv);
+    NN_TRY(hipStreamSynchronize(n->stream));
+    NN_TRY(hipMemcpy(n->m, mm.data(), n->nparams * 4, hipMemcpyHostToDevice));
+    NN_TRY(hipMemcpy(n->v, vv.data(), n->nparams * 4, hipMemcpyHostToDevice));
+    n->t = (int)step;
+    return FWI_OK;
+}
+
 int fwi_nn_forward(fwi_nn *n, int32_t nb, const float *x, float *y)
 {
     TRY(check_nb(n, nb));

@@ -16,7 +16,8 @@ Three tools:
   outputs zeros gives d = -g.
 * ``train`` fits the network (BF16 MFMA kernels, Adam) to those samples; every
   input channel and the accumulated update are normalised by their maximum
-  magnitude per sample.
+  magnitude per sample.  Both sample generation (a file per training model)
+  and training (a file rewritten every epoch) can checkpoint and resume.
 * ``run_fwi`` runs plain FWI or FWI with a trial direction.  Such an iteration
   computes the gradient, the plain two-point line search and its guarded
   step, then tries the direction d at two multiples of the plain step length;
@@ -273,44 +274,102 @@ def run_fwi(fwi, direction=None, multiples=(2.0, 4.0), band_iters=None, log=None
     return hist, m
 
 
-def make_samples(setup, seeds, log=None):
+def _save_npz(path, **arrays):
+    """np.savez to path through a temporary file, so an interrupted write
+    leaves either the old file or the new one."""
+    tmp = path + ".tmp"
+    with open(tmp, "wb") as f:
+        np.savez(f, **arrays)
+    os.replace(tmp, path)
+
+
+def _setup_tag(setup):
+    return json.dumps(dataclasses.asdict(setup), sort_keys=True)
+
+
+def make_samples(setup, seeds, log=None, cache=None):
     """Plain-FWI samples: X [N, 3, nx, nz] (normalised raw gradient,
     illumination, model), Y [N, nx, nz] (the network's target: normalised
     m_end - m_k plus the plain direction g), G [N, nx, nz] (g) and the
-    per-sample scale max|m_end - m_k| / dm."""
+    per-sample scale max|m_end - m_k| / dm.
+
+    With `cache` (a folder), each seed's samples are written to
+    cache/samples_<seed>.npz as soon as its run ends, and a seed whose file
+    exists is read instead of run again, so an interrupted generation resumes
+    at the first missing seed.  A file made for another setup is refused."""
     xs, ys, gs, scales = [], [], [], []
+    shape = (setup.nx, setup.nz)
+    if cache:
+        os.makedirs(cache, exist_ok=True)
     for seed in seeds:
-        v, v0 = C.config4_models(seed, setup.nz, setup.nx)
-        fwi = Fwi(setup, v, v0)
-        m = fwi.m0.copy()
-        fwi.eng.reset()
-        inputs = []
-        for it in range(setup.iterations):
-            fwi.use_band(setup.band(it))
-            f0, g, raw, ill = fwi.gradient(m, it)
-            step, _, _, _ = fwi.plain_search(m, g, f0)
-            inputs.append((raw, ill, m.copy(), g))
-            m = m - np.float32(step) * g
-        for raw, ill, mk, gk in inputs:
-            t = (m - mk).astype(np.float32)
-            s = float(np.max(np.abs(t)))
-            if s <= 0:
-                continue
-            xs.append(normalise_inputs(raw, ill, mk))
-            ys.append(t / s + gk)
-            gs.append(gk)
-            scales.append(s / fwi.dm)
-        fwi.close()
+        path = os.path.join(cache, f"samples_{seed}.npz") if cache else None
+        cached = bool(path) and os.path.exists(path)
+        if cached:
+            with np.load(path) as z:
+                if str(z["setup"]) != _setup_tag(setup):
+                    raise ValueError(f"{path} holds samples of another setup")
+                sx, sy, sg, ss = z["x"], z["y"], z["g"], z["scale"]
+        else:
+            sx, sy, sg, ss = _seed_samples(setup, seed)
+            if path:
+                _save_npz(path, x=sx, y=sy, g=sg, scale=ss, setup=np.array(_setup_tag(setup)))
+        xs.extend(sx)
+        ys.extend(sy)
+        gs.extend(sg)
+        scales.extend(ss)
         if log:
-            log({"seed": seed, "samples": len(xs)})
+            log({"seed": seed, "samples": len(xs), "cached": cached})
+    if not xs:
+        return (np.zeros((0, 3) + shape, np.float32), np.zeros((0,) + shape, np.float32),
+                np.zeros((0,) + shape, np.float32), np.zeros(0))
     return np.stack(xs), np.stack(ys), np.stack(gs), np.array(scales)
 
 
-def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, log=None):
+def _seed_samples(setup, seed):
+    """Samples of one training model (see make_samples), as arrays."""
+    v, v0 = C.config4_models(seed, setup.nz, setup.nx)
+    fwi = Fwi(setup, v, v0)
+    m = fwi.m0.copy()
+    fwi.eng.reset()
+    inputs = []
+    for it in range(setup.iterations):
+        fwi.use_band(setup.band(it))
+        f0, g, raw, ill = fwi.gradient(m, it)
+        step, _, _, _ = fwi.plain_search(m, g, f0)
+        inputs.append((raw, ill, m.copy(), g))
+        m = m - np.float32(step) * g
+    sx, sy, sg, ss = [], [], [], []
+    for raw, ill, mk, gk in inputs:
+        t = (m - mk).astype(np.float32)
+        s = float(np.max(np.abs(t)))
+        if s <= 0:
+            continue
+        sx.append(normalise_inputs(raw, ill, mk))
+        sy.append(t / s + gk)
+        sg.append(gk)
+        ss.append(s / fwi.dm)
+    fwi.close()
+    shape = (setup.nx, setup.nz)
+    if not sx:
+        return (np.zeros((0, 3) + shape, np.float32), np.zeros((0,) + shape, np.float32),
+                np.zeros((0,) + shape, np.float32), np.zeros(0))
+    return np.stack(sx), np.stack(sy), np.stack(sg), np.array(ss)
+
+
+def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, log=None,
+          checkpoint=None):
     """Adam on the GPU; returns (params, history of (epoch, train loss,
     held-out loss), held-out indices).  With flip, each training sample is
     mirrored along x with probability 1/2 (the wave equation is symmetric
-    under x -> -x, so a mirrored sample is a valid sample)."""
+    under x -> -x, so a mirrored sample is a valid sample).
+
+    With `checkpoint` (a file path), the state after every epoch (parameters,
+    Adam moments and step count, the shuffled order, the random generator and
+    the loss history) is written there, and an existing checkpoint is resumed
+    from the epoch after it: a run stopped and resumed gives the same
+    parameters, bit for bit, as one that ran straight through.  A checkpoint
+    of another training problem (sample count, image size, batch, seed,
+    hold-out, mirroring, learning rate) is refused."""
     n, _, h, w = x.shape
     rng = np.random.default_rng(seed)
     order = rng.permutation(n)
@@ -318,7 +377,25 @@ def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, lo
     val, tr = order[:nval], order[nval:]
     net = A.Net(batch, h, w, seed)
     hist = []
-    for ep in range(epochs):
+    first = 0
+    problem = {"n": int(n), "h": int(h), "w": int(w), "batch": int(batch), "seed": int(seed),
+               "holdout": float(holdout), "flip": bool(flip), "lr": float(lr)}
+    if checkpoint and os.path.exists(checkpoint):
+        with np.load(checkpoint) as z:
+            meta = json.loads(str(z["meta"]))
+            if meta["problem"] != problem:
+                net.close()
+                raise ValueError(f"{checkpoint} is a checkpoint of another training problem "
+                                 f"({meta['problem']})")
+            net.set_params(z["params"])
+            net.set_optimizer(z["m"], z["v"], meta["steps"])
+            tr = z["order"].copy()
+        rng.bit_generator.state = meta["rng"]
+        hist = [tuple(e) for e in meta["hist"]]
+        first = meta["epoch"] + 1
+        if log:
+            log({"resumed_from": checkpoint, "epoch": meta["epoch"]})
+    for ep in range(first, epochs):
         rng.shuffle(tr)
         losses = []
         for b0 in range(0, len(tr) - batch + 1, batch):
@@ -337,6 +414,12 @@ def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, lo
         hist.append((ep, float(np.mean(losses)), float(np.sum(vl) / nval)))
         if log:
             log({"epoch": ep, "train_loss": hist[-1][1], "holdout_loss": hist[-1][2]})
+        if checkpoint:
+            m, v, steps = net.get_optimizer()
+            meta = {"problem": problem, "epoch": ep, "steps": steps, "hist": hist,
+                    "rng": rng.bit_generator.state}
+            _save_npz(checkpoint, params=net.get_params(), m=m, v=v, order=tr,
+                      meta=np.array(json.dumps(meta)))
     p = net.get_params()
     net.close()
     return p, hist, val
@@ -391,6 +474,9 @@ def main(argv=None):
     p.add_argument("--bands", default="", help="iterations per band of the comparison runs")
     p.add_argument("--short-bands", default="",
                    help="iterations per band of extra, shorter control and learned runs")
+    p.add_argument("--checkpoint", default="",
+                   help="folder for resumable sample generation (one file per training "
+                        "model) and training (one file, rewritten every epoch)")
     a = p.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     log = lambda d: print(json.dumps(d), flush=True)
@@ -402,10 +488,15 @@ def main(argv=None):
     else:
         t0 = time.perf_counter()
         ts = TRAIN_SMALL if a.train_size == "small" else TRAIN_MID
-        x, y, g, scales = make_samples(ts, range(1, 1 + a.train_models), log)
+        ck = a.checkpoint
+        x, y, g, scales = make_samples(ts, range(1, 1 + a.train_models), log,
+                                       cache=os.path.join(ck, f"samples_{a.train_size}")
+                                       if ck else None)
         t_gen = time.perf_counter() - t0
         t0 = time.perf_counter()
-        params, th, val = train(x, y, epochs=a.epochs, lr=a.lr, log=log)
+        params, th, val = train(x, y, epochs=a.epochs, lr=a.lr, log=log,
+                                checkpoint=os.path.join(ck, f"train_{a.train_size}.npz")
+                                if ck else None)
         t_train = time.perf_counter() - t0
         cos = holdout_cosines(params, x, y, g, val)
         log({"holdout_cosine_learned": cos[0], "holdout_cosine_plain": cos[1]})

@@ -35,6 +35,14 @@ int fwi_nn_num_params(fwi_nn *nn, int64_t *count);
 int fwi_nn_get_params(fwi_nn *nn, float *params);
 int fwi_nn_set_params(fwi_nn *nn, const float *params);
 
+/* Adam state in the flat parameter layout: first and second moments and the
+ * number of steps taken, so that training can stop and resume exactly.  The
+ * reference keeps no optimizer state (it trains no network); these serve the
+ * training checkpoint.  set rejects a negative step count and non-finite or
+ * negative second moments. */
+int fwi_nn_get_optimizer(fwi_nn *nn, float *m, float *v, int64_t *step);
+int fwi_nn_set_optimizer(fwi_nn *nn, const float *m, const float *v, int64_t step);
+
 /* y = net(x) for nb <= batch images. */
 int fwi_nn_forward(fwi_nn *nn, int32_t nb, const float *x, float *y);
 

@@ -104,3 +104,45 @@ def test_mid_trained_network_gains_over_plain_fwi():
     e = {k: h[-1]["model_rms_error"] for k, h in runs.items()}
     assert f["learned"] < f["plain"]
     assert e["learned"] < e["plain"]
+
+
+def test_training_resumes_bit_for_bit(tmp_path):
+    """Training stopped after one epoch and resumed from its checkpoint ends
+    with the same parameters and loss history as a run straight through."""
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((20, 3, 16, 24)).astype(np.float32)
+    y = rng.standard_normal((20, 16, 24)).astype(np.float32)
+    kw = dict(batch=4, lr=2e-3, seed=3, flip=True)
+    pa, ha, va = L.train(x, y, epochs=3, **kw)
+    ck = str(tmp_path / "train.npz")
+    L.train(x, y, epochs=1, checkpoint=ck, **kw)
+    logs = []
+    pb, hb, vb = L.train(x, y, epochs=3, checkpoint=ck, log=logs.append, **kw)
+    assert logs[0] == {"resumed_from": ck, "epoch": 0}
+    assert [e["epoch"] for e in logs[1:]] == [1, 2]
+    assert np.array_equal(pa, pb) and ha == hb and np.array_equal(va, vb)
+    # a finished checkpoint resumes to nothing left to do
+    pc, hc, _ = L.train(x, y, epochs=3, checkpoint=ck, **kw)
+    assert np.array_equal(pc, pa) and hc == ha
+    with pytest.raises(ValueError, match="another training problem"):
+        L.train(x, y, epochs=3, checkpoint=ck, **dict(kw, lr=1e-3))
+
+
+TINY = dataclasses.replace(L.TRAIN_SMALL, nz=48, nx=96, nt=900, nsrc=3, nrec=48, rbc_nb=24,
+                           abc_nb=24, band_iters=(1, 1, 1))
+
+
+def test_sample_generation_resumes_from_its_cache(tmp_path):
+    """Samples of seeds already in the cache folder are read, not recomputed,
+    and equal what a fresh run gives; a cache of another setup is refused."""
+    cache = str(tmp_path / "samples")
+    xa, ya, ga, sa = L.make_samples(TINY, [1, 2])
+    L.make_samples(TINY, [1], cache=cache)
+    logs = []
+    xb, yb, gb, sb = L.make_samples(TINY, [1, 2], cache=cache, log=logs.append)
+    assert [e["cached"] for e in logs] == [True, False]
+    assert sorted(os.listdir(cache)) == ["samples_1.npz", "samples_2.npz"]
+    for a, b in ((xa, xb), (ya, yb), (ga, gb), (sa, sb)):
+        assert a.shape == b.shape and np.array_equal(a, b)
+    with pytest.raises(ValueError, match="another setup"):
+        L.make_samples(dataclasses.replace(TINY, seed=5), [1], cache=cache)

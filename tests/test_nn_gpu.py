@@ -82,3 +82,41 @@ def test_set_get_roundtrip_and_training_reduces_loss():
     losses = [net.train_step(x, t, 3e-3) for _ in range(60)]
     print("loss", losses[0], "->", losses[-1])
     assert losses[-1] < 0.5 * losses[0]
+
+
+def test_optimizer_state_closed_form_and_resume():
+    """After one Adam step from zero moments m = (1 - b1) g, v = (1 - b2) g^2
+    (b1 = 0.9, b2 = 0.999 as float32) and the step count is 1; a fresh network given the parameters and Adam
+    state continues exactly as the original one does."""
+    x, t = small_problem(4, nb=2)
+    a = F.Net(2, 16, 24, seed=7)
+    _, g = a.gradient(x, t)
+    a.train_step(x, t, 2e-3)
+    m, v, steps = a.get_optimizer()
+    assert steps == 1
+    # the kernel's betas are the float32 constants 0.9f and 0.999f
+    b1, b2 = float(np.float32(0.9)), float(np.float32(0.999))
+    assert np.allclose(m, (1.0 - b1) * g.astype(np.float64), rtol=1e-6, atol=0)
+    assert np.allclose(v, (1.0 - b2) * g.astype(np.float64) ** 2, rtol=1e-6, atol=0)
+    params = a.get_params()
+    b = F.Net(2, 16, 24, seed=99)
+    b.set_params(params)
+    b.set_optimizer(m, v, steps)
+    for _ in range(3):
+        la, lb = a.train_step(x, t, 2e-3), b.train_step(x, t, 2e-3)
+        assert la == lb
+    assert np.array_equal(a.get_params(), b.get_params())
+    ma, va, sa = a.get_optimizer()
+    mb, vb, sb = b.get_optimizer()
+    assert sa == sb == 4 and np.array_equal(ma, mb) and np.array_equal(va, vb)
+    # invalid state is refused before it reaches the device
+    with pytest.raises(F.FwiError):
+        b.set_optimizer(m, -np.abs(v) - 1.0, steps)
+    with pytest.raises(F.FwiError):
+        b.set_optimizer(m, v, -1)
+    with pytest.raises(ValueError):
+        b.set_optimizer(m[:-1], v, steps)
+    mc, vc, sc = b.get_optimizer()            # a refused call changes nothing
+    assert sc == sb and np.array_equal(mc, mb) and np.array_equal(vc, vb)
+    a.close()
+    b.close()
