@@ -593,16 +593,24 @@ int fwi_set_geometry(fwi_ctx *c, int32_t nsrc, const float *xsrc, const float *z
     if (!c || nsrc < 1 || nrec < 1 || !xsrc || !zsrc || !xrec || !zrec)
         return fail(FWI_ERR_ARG, "bad geometry arguments");
     const float dh = c->cfg.dh;
+    // Grid indices and receiver tables of every configured grid, validated
+    // before anything in the context changes: a rejected geometry leaves the
+    // previous one in place.
+    struct Tables {
+        std::vector<int> siz, six, col_ptr, s_iz, s_id;
+    } tab[2];
     for (int kind = 0; kind < 2; ++kind) {
         GridBuf &gb = c->grid[kind];
         if (!gb.on) continue;
+        Tables &t = tab[kind];
         const int nb = gb.g.nb;
         auto index = [&](float x) { return (int)((x + 0.5f * dh) / dh) + nb; };
-        std::vector<int> siz(nsrc), six(nsrc);
+        t.siz.resize(nsrc);
+        t.six.resize(nsrc);
         for (int s = 0; s < nsrc; ++s) {
-            siz[s] = index(zsrc[s]);
-            six[s] = index(xsrc[s]);
-            if (siz[s] < 4 || siz[s] >= gb.g.nzz - 4 || six[s] < 4 || six[s] >= gb.g.nxx - 4)
+            t.siz[s] = index(zsrc[s]);
+            t.six[s] = index(xsrc[s]);
+            if (t.siz[s] < 4 || t.siz[s] >= gb.g.nzz - 4 || t.six[s] < 4 || t.six[s] >= gb.g.nxx - 4)
                 return fail(FWI_ERR_ARG, "source %d at (x=%g, z=%g) m falls outside the grid",
                             s, xsrc[s], zsrc[s]);
         }
@@ -618,30 +626,42 @@ int fwi_set_geometry(fwi_ctx *c, int32_t nsrc, const float *xsrc, const float *z
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
             return rix[a] != rix[b] ? rix[a] < rix[b] : riz[a] < riz[b];
         });
-        std::vector<int> col_ptr(gb.g.nxx + 1, 0), s_iz(nrec), s_id(nrec);
-        for (int r = 0; r < nrec; ++r) col_ptr[rix[r] + 1]++;
-        for (int j = 0; j < gb.g.nxx; ++j) col_ptr[j + 1] += col_ptr[j];
+        t.col_ptr.assign(gb.g.nxx + 1, 0);
+        t.s_iz.resize(nrec);
+        t.s_id.resize(nrec);
+        for (int r = 0; r < nrec; ++r) t.col_ptr[rix[r] + 1]++;
+        for (int j = 0; j < gb.g.nxx; ++j) t.col_ptr[j + 1] += t.col_ptr[j];
         for (int k = 0; k < nrec; ++k) {
-            s_iz[k] = riz[order[k]];
-            s_id[k] = order[k];
+            t.s_iz[k] = riz[order[k]];
+            t.s_id[k] = order[k];
         }
+    }
+    // From here on the context holds no usable geometry until every buffer
+    // is in place (a failed allocation leaves it unset, not half-updated).
+    c->have_geometry = false;
+    drop_graphs(c);
+    for (int kind = 0; kind < 2; ++kind) {
+        GridBuf &gb = c->grid[kind];
+        if (!gb.on) continue;
+        const Tables &t = tab[kind];
         void *old[] = {gb.src_iz, gb.src_ix, gb.col_ptr, gb.rec_iz, gb.rec_id};
         for (void *p : old)
             if (p) (void)hipFree(p);
         gb.src_iz = gb.src_ix = gb.col_ptr = gb.rec_iz = gb.rec_id = nullptr;
+        gb.h_col_ptr.clear();
         TRY(dalloc(&gb.src_iz, nsrc, c->stream));
         TRY(dalloc(&gb.src_ix, nsrc, c->stream));
         TRY(dalloc(&gb.col_ptr, gb.g.nxx + 1, c->stream));
         TRY(dalloc(&gb.rec_iz, nrec, c->stream));
         TRY(dalloc(&gb.rec_id, nrec, c->stream));
-        HIP_TRY(hipMemcpy(gb.src_iz, siz.data(), nsrc * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(gb.src_ix, six.data(), nsrc * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(gb.col_ptr, col_ptr.data(), col_ptr.size() * 4, hipMemcpyHostToDevice));
-        gb.h_col_ptr = col_ptr;
-        HIP_TRY(hipMemcpy(gb.rec_iz, s_iz.data(), nrec * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(gb.rec_id, s_id.data(), nrec * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gb.src_iz, t.siz.data(), nsrc * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gb.src_ix, t.six.data(), nsrc * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gb.col_ptr, t.col_ptr.data(), t.col_ptr.size() * 4,
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gb.rec_iz, t.s_iz.data(), nrec * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(gb.rec_id, t.s_id.data(), nrec * 4, hipMemcpyHostToDevice));
+        gb.h_col_ptr = t.col_ptr;
     }
-    drop_graphs(c);
     if (c->d_obs) { (void)hipFree(c->d_obs); c->d_obs = nullptr; }
     if (c->d_seis) { (void)hipFree(c->d_seis); c->d_seis = nullptr; }
     const long per_shot = (long)nrec * c->cfg.nt;

@@ -357,6 +357,35 @@ def test_errors_are_reported():
         eng.set_tuning(kernel=0, steps=4)
 
 
+def test_rejected_geometry_keeps_the_previous_one():
+    """A geometry that one grid accepts and the other rejects (a receiver
+    700 m left of the model: inside the 60-cell sponge pad, outside the
+    20-cell random pad) is refused as a whole: the previous geometry stays in
+    force for both grids and gives the same seismograms and image."""
+    v = overthrust()
+    eng = F.Engine(81, 201, 25.0, 301, 1e-3, abc_nb=60, abc_factor=0.004, rbc_nb=20,
+                   rbc_varVp_rel=0.4, rbc_seed=3, max_batch=2)
+    w, tl = F.ricker_wavelet(301, 1e-3, 30.0)
+    eng.set_wavelet(w, tl)
+    xr = np.linspace(0.0, 5000.0, 51)
+    eng.set_geometry([1500.0, 3500.0], [100.0, 100.0], xr, np.zeros(51))
+    eng.set_model(v)
+    seis = eng.modeling(F.ABC)
+    eng.set_observed(seis)
+    eng.reset()
+    eng.migration()
+    _, img, ill = eng.get_image()
+    with pytest.raises(F.FwiError) as e:
+        eng.set_geometry([1500.0], [100.0], [-700.0, 100.0], [0.0, 0.0])
+    assert "receiver 0" in str(e.value)
+    assert np.array_equal(eng.modeling(F.ABC), seis)
+    eng.reset()
+    eng.migration()
+    _, img2, ill2 = eng.get_image()
+    assert np.array_equal(img2, img) and np.array_equal(ill2, ill)
+    eng.close()
+
+
 def test_gradient_predicts_finite_difference_misfit():
     """The reference's gradient (grad += dt R v^2 lap S, inversion.cu:442) is
     the adjoint-state derivative of the misfit with respect to m = 1/v^2, up
