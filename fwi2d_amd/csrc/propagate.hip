@@ -628,7 +628,7 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
             const int i = zb + r;
             const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
             float val = leapfrog<ABC>(q[4][r], o[r], c[r], lap8r<VZ, 0>(q, ext, r), d);
-            nv[r] = (i >= 4 && i < g.nzz - 4) ? val : o[r];
+            nv[r] = ((i >= 4) & (i < g.nzz - 4)) ? val : o[r];
         }
         add_source<VZ>(nv, j, six, zb, siz, wsrc);
         if (active) {
@@ -736,7 +736,7 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
                 const int i = zb + r;
                 const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q1[4][r], o1[r], cj[r], lap8r<VZ, 0>(q1, ext, r), d);
-                n1[r] = (i >= 4 && i < g.nzz - 4) ? val : o1[r];
+                n1[r] = ((i >= 4) & (i < g.nzz - 4)) ? val : o1[r];
             }
             add_source<VZ>(n1, j, six, zb, siz, w1);
             if (own && j >= x0 && j < x1) {
@@ -775,7 +775,7 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
                 const float o2 = q1[0][r];              // u_t at column jj
                 const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q2[4][r], o2, c2[r], lap8r<VZ, 0>(q2, ext, r), d);
-                n2[r] = (i >= 4 && i < g.nzz - 4) ? val : o2;
+                n2[r] = ((i >= 4) & (i < g.nzz - 4)) ? val : o2;
             }
             add_source<VZ>(n2, jj, six, zb, siz, w2);
             if (own) {
@@ -889,7 +889,7 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
                 const int i = zb + r;
                 const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q1[C4][r], o1[r], cj[r], lap8r<VZ, C0>(q1, ext, r), d);
-                q2[C8][r] = (jin && i >= 4 && i < g.nzz - 4) ? val : o1[r];
+                q2[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o1[r];
             }
             add_source<VZ>(q2[C8], j, six, zb, siz, w1);
         }
@@ -912,7 +912,7 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
                 const float c2 = c2v[r];
                 const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q2[C4][r], o2, c2, lap8r<VZ, C0>(q2, ext, r), d);
-                q3[C8][r] = (jin && i >= 4 && i < g.nzz - 4) ? val : o2;
+                q3[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o2;
             }
             add_source<VZ>(q3[C8], jj, six, zb, siz, w2);
             if (own && jj >= x0 && jj < x1) vstore<VZ>((out1 + (long)jj * ldz) + uzl, q3[C8]);
@@ -936,7 +936,7 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
                 const float o3 = q2[C0][r];             // u_{t+1} at column jjj
                 const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
                 float val = leapfrog<ABC>(q3[C4][r], o3, c3[r], lap8r<VZ, C0>(q3, ext, r), d);
-                n3[r] = (i >= 4 && i < g.nzz - 4) ? val : o3;
+                n3[r] = ((i >= 4) & (i < g.nzz - 4)) ? val : o3;
             }
             add_source<VZ>(n3, jjj, six, zb, siz, w3);
             if (own) {
