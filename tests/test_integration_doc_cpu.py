@@ -34,3 +34,21 @@ def test_integration_cpp_examples_compile(tmp_path):
                         f"-I{os.path.join(ROOT, 'include')}", str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_integration_python_stub_matches_the_binding():
+    """The ctypes stub of INTEGRATION.md loads the library (no GPU call) and
+    its fwi_config layout equals the binding's."""
+    import ctypes as C
+    from fwi2d_amd import abi
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```python\n(.*?)```", text, re.S)
+    assert blocks
+    ns = {}
+    exec(blocks[0].replace("<repo>", ROOT), ns)
+    stub, ref = ns["Config"], abi.Config
+    assert C.sizeof(stub) == C.sizeof(ref)
+    assert [(n, t) for n, t in stub._fields_] == [(n, t) for n, t in ref._fields_]
+    for name, _ in ref._fields_:
+        assert getattr(stub, name).offset == getattr(ref, name).offset, name
+    assert callable(ns["check"]) and ns["lib"].fwi_create is not None
