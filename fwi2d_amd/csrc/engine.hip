@@ -118,9 +118,10 @@ struct fwi_ctx {
     float vmin = 0.f, vmax = 0.f;
     Quad fwd{};                // final (u_T, u_{T-1}) of the last forward pass
 
-    // launch shapes: [0] random-boundary forward, [1] reverse, [2] sponge forward
-    int vz[3] = {2, 1, 2}, xw[3] = {64, 64, 64};
-    int nsteps[3] = {2, 2, 2};              // time steps per launch
+    // launch shapes: [0] random-boundary forward, [1] reverse, [2] sponge forward;
+    // the defaults are the shapes the autotuner picks for batches of shots
+    int vz[3] = {4, 2, 4}, xw[3] = {81, 81, 81};
+    int nsteps[3] = {3, 3, 3};              // time steps per launch
     int use_graph = 1;
     std::map<long, hipGraphExec_t> graphs;
 
@@ -1022,7 +1023,6 @@ int fwi_get_tuning(fwi_ctx *c, int32_t kernel, int32_t *steps, int32_t *vz, int3
 int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, double *rev_us)
 {
     TRY(check_ready(c));
-    if (!c->grid[1].on) return fail(FWI_ERR_ARG, "autotune runs on the random-boundary grid");
     if (nslots < 1 || nslots > c->cfg.max_batch) return fail(FWI_ERR_ARG, "bad slot count");
     if (reps < 1) reps = 1;
     const int T = T_of(c);
@@ -1035,7 +1035,8 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
         // kind 2, the sponge forward, runs on its own grid with the receivers
         // sampled (step index past tlag); kind 0 at step T (no source); the
         // reverse kernel at step 0, receiver data injected
-        if (kind == 2 && !c->grid[0].on) continue;
+        if (kind == 2 && !c->grid[0].on) continue;      // kernels of a grid that is not
+        if (kind != 2 && !c->grid[1].on) continue;      // configured are not tuned
         const GridBuf &gb = c->grid[kind == 2 ? 0 : 1];
         const int tstep = (kind == 2 && c->tlag + 3 <= T) ? c->tlag + 1 : T;
         const long cells = (long)nslots * gb.g.nzz * gb.g.nxx;
@@ -1105,11 +1106,13 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
     }
     drop_graphs(c);
     for (int k = 0; k < 8; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
-    HIP_TRY(fwi::launch_fill(c->d_acc_slot, nf, 0.f, c->stream));
-    HIP_TRY(fwi::launch_fill(c->d_ill_slot, nf, 0.f, c->stream));
+    if (c->grid[1].on) {
+        HIP_TRY(fwi::launch_fill(c->d_acc_slot, nf, 0.f, c->stream));
+        HIP_TRY(fwi::launch_fill(c->d_ill_slot, nf, 0.f, c->stream));
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (fwd_us) *fwd_us = best[0];
-    if (rev_us) *rev_us = best[1];
+    if (fwd_us) *fwd_us = c->grid[1].on ? best[0] : 0.0;
+    if (rev_us) *rev_us = c->grid[1].on ? best[1] : 0.0;
     return FWI_OK;
 }
 

@@ -83,6 +83,14 @@ void Modeling::create_engine(int rbc_nb, float varVp, float varVp_rel, uint64_t 
                            geometry.nrec, geometry.xrec.data(), geometry.zrec.data()),
           "set_geometry");
     check(fwi_set_model(ctx, vp.data()), "set_model");
+    // launch shapes timed on this grid and batch (about a second at
+    // production sizes); results do not depend on them
+    if (param_bool("gpu_autotune", parameters, true)) {
+        double fwd_us = 0.0, rev_us = 0.0;
+        check(fwi_autotune(ctx, std::max(1, std::min(max_batch, geometry.nsrc)), 8, &fwd_us,
+                           &rev_us),
+              "autotune");
+    }
 }
 
 void Modeling::set_parameters()

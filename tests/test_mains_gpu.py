@@ -201,3 +201,32 @@ def test_prepared_configuration_end_to_end(tmp_path):
     assert sorted(os.path.basename(f) for f in figs) == sorted(
         ["model.png", "seismogram_shot_1.png", "image_RTM.png", "residuo_FWI.png",
          "model_FWI.png"])
+
+
+def test_autotuned_executables_write_the_same_files(tmp_path):
+    """The executables autotune their launch shapes by default
+    (gpu_autotune); with gpu_autotune = false they keep the built-in shapes.
+    Both write bit-identical seismograms and RTM sections."""
+    tmp = str(tmp_path)
+    _, _, _, _, folders, prefix, params = setup_files(tmp)
+    init_file = os.path.join(tmp, "init.bin")
+    true_file = os.path.join(tmp, "true.bin")
+    sps, rps = os.path.join(tmp, "sps.txt"), os.path.join(tmp, "rps.txt")
+    f2 = C.ensure_dirs(os.path.join(tmp, "fixed"))
+    p_mod = os.path.join(tmp, "parameters_mod_fixed.txt")
+    C.write_parameters(p_mod, SURVEY, true_file, sps, rps, f2, extra={"gpu_autotune": "false"})
+    p_mig = os.path.join(tmp, "parameters_mig_fixed.txt")
+    C.write_parameters(p_mig, SURVEY, init_file, sps, rps, f2, mig_prefix=prefix,
+                       extra={"gpu_autotune": "false"})
+    for p in (params["mod"], p_mod):
+        run("modeling", p)
+    for p in (params["mig"], p_mig):
+        run("migration", p)
+    for k in range(1, len(XS) + 1):
+        a = np.fromfile(os.path.join(folders["data"], f"{prefix}{k}.bin"), np.float32)
+        b = np.fromfile(os.path.join(f2["data"], f"{prefix}{k}.bin"), np.float32)
+        assert a.size == len(XR) * NT and np.array_equal(a, b), k
+    name = f"RTM_section_{int(FMAX)}Hz_{SURVEY.nz}x{SURVEY.nx}_{int(SURVEY.dh)}m.bin"
+    a = np.fromfile(os.path.join(folders["seismic"], name), np.float32)
+    b = np.fromfile(os.path.join(f2["seismic"], name), np.float32)
+    assert a.size == SURVEY.nz * SURVEY.nx and np.abs(a).max() > 0 and np.array_equal(a, b)
