@@ -117,8 +117,10 @@ void Modeling::show_information() const
 
 void Modeling::forward_solver()
 {
-    seismogram.assign((size_t)geometry.nsrc * geometry.nrec * nt, 0.f);
-    check(fwi_modeling(ctx, FWI_BOUNDARY_ABC, 0, geometry.nsrc, seismogram.data()), "modeling");
+    // no zero fill: every sample is written by the call (1 GB at 64 shots of
+    // 1001 receivers x 4001 samples)
+    seismogram.reset(new float[(size_t)geometry.nsrc * geometry.nrec * nt]);
+    check(fwi_modeling(ctx, FWI_BOUNDARY_ABC, 0, geometry.nsrc, seismogram.get()), "modeling");
 }
 
 void Modeling::get_seismogram()
@@ -128,7 +130,7 @@ void Modeling::get_seismogram()
         std::string file = data_folder + "seismogram_nt" + std::to_string(nt) + "_nr" +
                            std::to_string(geometry.nrec) + "_" + std::to_string(int(1e6f * dt)) +
                            "us_shot_" + std::to_string(s + 1) + ".bin";
-        export_binary_float(file, seismogram.data() + s * per, per);
+        export_binary_float(file, seismogram.get() + s * per, per);
     }
 }
 

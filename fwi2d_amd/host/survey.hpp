@@ -6,6 +6,7 @@
 #pragma once
 
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "admin.hpp"
@@ -35,7 +36,7 @@ protected:
     float vmin = 0, vmax = 0;
     std::vector<float> vp;          // interior model [nx][nz]
     std::vector<float> wavelet;
-    std::vector<float> seismogram;  // [nsrc][nrec][nt]
+    std::unique_ptr<float[]> seismogram;  // [nsrc][nrec][nt], written whole by fwi_modeling
     std::string data_folder, title;
     fwi_ctx *ctx = nullptr;
 
