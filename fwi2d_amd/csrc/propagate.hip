@@ -561,6 +561,12 @@ __device__ __forceinline__ bool tile_of(int nzseg, int nxchunk, int nslots, Tile
 }
 
 // ---------------------------------------------------------------- forward
+//
+// The rim tests of the forward kernels combine their terms with `&`, not
+// `&&`: the short-circuit form gives the compiler a branch, and it then sinks
+// each row's stencil into an exec-masked block of its own (a save/restore of
+// the exec mask per row and step), which costs scalar issue slots and keeps
+// the rows from being interleaved.
 
 // One time step: u_{t+1} = leapfrog(u_t, u_{t-1}) written over u_{t-1}
 // (the odd remainder of a pass, and a tuning alternative).
