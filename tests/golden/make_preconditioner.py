@@ -3,6 +3,7 @@ project's own kernels on plain-FWI samples.
 
     python tests/golden/make_preconditioner.py small   (GPU, ~4 minutes)
     python tests/golden/make_preconditioner.py mid     (GPU, ~8 minutes)
+    python tests/golden/make_preconditioner.py mid96   (GPU, ~15 minutes)
 
 small -> preconditioner_48models.bin: 48 seeded training models of the
 config-4 family at 128 x 384 (10 m, 16 shots), 8 plain FWI iterations each
@@ -16,6 +17,10 @@ samples (what `python -m fwi2d_amd.learned --train-size mid --runs ""`
 writes).  The full-size (401 x 1601) comparison uses it: the model families
 scale their structures with the grid, so mid-size gradients are closer in
 cells to the full-size ones.
+
+mid96 -> preconditioner_mid96.bin: the same at 200 x 800 with 96 training
+models (seeds 1-96, 768 samples; `python -m fwi2d_amd.learned --train-size
+mid --train-models 96 --runs ""`).
 """
 import os
 import sys
@@ -27,9 +32,11 @@ from fwi2d_amd import learned as L  # noqa: E402
 
 
 def main(size):
-    setup, flip, name = {"small": (L.TRAIN_SMALL, False, "preconditioner_48models.bin"),
-                         "mid": (L.TRAIN_MID, True, "preconditioner_mid48.bin")}[size]
-    x, y, g, _ = L.make_samples(setup, range(1, 49), log=print)
+    setup, flip, name, models = {
+        "small": (L.TRAIN_SMALL, False, "preconditioner_48models.bin", 48),
+        "mid": (L.TRAIN_MID, True, "preconditioner_mid48.bin", 48),
+        "mid96": (L.TRAIN_MID, True, "preconditioner_mid96.bin", 96)}[size]
+    x, y, g, _ = L.make_samples(setup, range(1, models + 1), log=print)
     params, hist, val = L.train(x, y, epochs=40, lr=1e-3, flip=flip, log=print)
     print("held-out cosine (learned, plain):", L.holdout_cosines(params, x, y, g, val))
     params.tofile(os.path.join(os.path.dirname(os.path.abspath(__file__)), name))
