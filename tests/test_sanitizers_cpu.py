@@ -13,7 +13,7 @@ ORACLE = os.path.join(ROOT, "oracle")
 HOST = os.path.join(ROOT, "fwi2d_amd", "host")
 SAN = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
        "-fno-sanitize-recover=undefined"]
-ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
+ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0:verify_asan_link_order=0",
            UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1", OMP_NUM_THREADS="1")
 
 ORACLE_DRIVER = r"""
