@@ -6,8 +6,6 @@ Any sanitizer report or leak fails the test."""
 import os
 import subprocess
 
-import pytest
-
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
 HOST = os.path.join(ROOT, "fwi2d_amd", "host")
