@@ -1,5 +1,6 @@
-"""The C++ examples in INTEGRATION.md compile against include/fwi2d.h
-(g++ -fsyntax-only), so the documented calls keep the header's signatures."""
+"""The integration guide's C++ examples compile against include/fwi2d.h
+(g++ -fsyntax-only), so the documented calls keep the header's signatures,
+and its ctypes stub matches the Python binding."""
 import os
 import re
 import subprocess
@@ -37,8 +38,8 @@ def test_integration_cpp_examples_compile(tmp_path):
 
 
 def test_integration_python_stub_matches_the_binding():
-    """The ctypes stub of INTEGRATION.md loads the library (no GPU call) and
-    its fwi_config layout equals the binding's."""
+    """The guide's ctypes stub loads the library (no GPU call) and its
+    fwi_config layout equals the binding's."""
     import ctypes as C
     from fwi2d_amd import abi
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
