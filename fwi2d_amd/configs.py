@@ -340,18 +340,14 @@ def initial_model(parameters, sigma=3.0, sources=None, receivers=None):
     nz = stoi(read_parameter(parameters, "z_samples"))
     dh = float(np.float32(read_parameter(parameters, "model_spacing")))
     zmask = float(np.float32(read_parameter(parameters, "depth_mask", "0")))
-    if sources:
-        xs, zs = _line(sources)
-        np.savetxt(read_parameter(parameters, "SPS"), np.stack([xs, zs], 1), fmt="%.2f",
-                   delimiter=",")
-    if receivers:
-        xr, zr = _line(receivers)
-        np.savetxt(read_parameter(parameters, "RPS"), np.stack([xr, zr], 1), fmt="%.2f",
-                   delimiter=",")
     v = np.fromfile(model_file, np.float32, count=nx * nz)
     if v.size != nx * nz:
         raise SystemExit(f"{model_file}: {v.size} floats, expected {nx} x {nz}")
     v = v.reshape(nx, nz)
+    lines = [(key, _line(spec)) for key, spec in (("SPS", sources), ("RPS", receivers)) if spec]
+    for key, (x, z) in lines:       # every input checked before any file is written
+        np.savetxt(read_parameter(parameters, key), np.stack([x, z], 1), fmt="%.2f",
+                   delimiter=",")
     init = smooth_slowness(v, sigma, keep_rows=int(zmask / dh))
     out = os.path.join(os.path.dirname(model_file),
                        os.path.basename(model_file).replace("true", "init"))

@@ -169,3 +169,18 @@ def test_command_line_entry_point(tmp_path):
     r = subprocess.run([sys.executable, "-m", "fwi2d_amd.configs"], capture_output=True,
                        text=True, cwd=ROOT, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_initial_model_checks_its_inputs_before_writing(tmp_path):
+    """A short model file is refused before SPS/RPS are rewritten."""
+    s, v, params = small_run(tmp_path)
+    sps = read_parameter(params, "SPS")
+    before = open(sps).read()
+    v.ravel()[: v.size // 2].tofile(read_parameter(params, "model_file"))
+    with pytest.raises(SystemExit, match="floats"):
+        C.initial_model(params, sources="0,100,3,20")
+    assert open(sps).read() == before
+    v.tofile(read_parameter(params, "model_file"))
+    with pytest.raises(ValueError):
+        C.initial_model(params, sources="0,100,20")       # malformed line spec
+    assert open(sps).read() == before
