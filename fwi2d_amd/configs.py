@@ -369,12 +369,15 @@ def mute(parameters, velocity=3600.0, delay=0.18, gstd=30.0):
     rps = np.loadtxt(read_parameter(parameters, "RPS"), np.float32, delimiter=",", ndmin=2)
     nr = len(rps)
     prefix = seismogram_prefix(nt, nr, dt)
+    srcs = [f"{folder}{prefix}{k + 1}.bin" for k in range(len(sps))]
+    for src in srcs:                # every input checked before any file is written
+        if not os.path.exists(src):
+            raise FileNotFoundError(f"{src} does not exist (run the modeling executable first)")
+        if os.path.getsize(src) != 4 * nr * nt:
+            raise SystemExit(f"{src}: {os.path.getsize(src) // 4} floats, expected {nr} x {nt}")
     out = []
-    for k in range(len(sps)):
-        src = f"{folder}{prefix}{k + 1}.bin"
+    for k, src in enumerate(srcs):
         data = np.fromfile(src, np.float32)
-        if data.size != nr * nt:
-            raise SystemExit(f"{src}: {data.size} floats, expected {nr} x {nt}")
         d = mute_direct_wave(data.reshape(nr, nt), sps[k, 0], sps[k, 1], rps[:, 0], rps[:, 1],
                              dt, float(velocity), float(delay), float(gstd))
         dst = f"{folder}input_RTM_{prefix}{k + 1}.bin"

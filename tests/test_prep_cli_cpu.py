@@ -128,10 +128,18 @@ def test_mute_options_and_missing_input(tmp_path):
     for k, path in enumerate(a):
         out = np.fromfile(path, np.float32).reshape(s.nrec, s.nt)
         assert np.allclose(out, data[k] / np.abs(data[k]).max(), rtol=1e-6, atol=0)
-    os.remove(f"{read_parameter(params, 'mod_output_folder')}"
-              f"{C.seismogram_prefix(s.nt, s.nrec, s.dt)}2.bin")
+    folder = read_parameter(params, "mod_output_folder")
+    prefix = C.seismogram_prefix(s.nt, s.nrec, s.dt)
+    for f in a:
+        os.remove(f)
+    os.remove(f"{folder}{prefix}2.bin")
     with pytest.raises(FileNotFoundError):
         C.mute(params)
+    assert not any(os.path.exists(f) for f in a)       # nothing written for shot 1 either
+    data[1, :, :10].tofile(f"{folder}{prefix}2.bin")    # short file
+    with pytest.raises(SystemExit, match="floats"):
+        C.mute(params)
+    assert not any(os.path.exists(f) for f in a)
 
 
 def test_initial_model_from_parameters(tmp_path):
