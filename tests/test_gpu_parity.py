@@ -196,6 +196,37 @@ def test_reverse_launch_shapes_bit_identical():
         assert r[4] == results[0][4]
 
 
+def test_graph_replays_match_eager():
+    # each pass is captured once into a graph and replayed afterwards; every
+    # replay, with eager host work and synchronisations between, gives the
+    # bits of the eager launches, for full (2-shot) and ragged (1-shot)
+    # batches and for a random boundary drawn per iteration
+    s = overthrust_survey(nsrc=3, abc_nb=51, abc_factor=0.004, rbc_nb=41, varVp=900.0,
+                          seed=6, max_batch=2)
+
+    def run(iteration):
+        obs = s.eng.modeling(F.ABC)
+        s.eng.set_observed(obs)
+        s.eng.set_model((s.v * 0.98).astype(np.float32))
+        s.eng.reset()
+        s.eng.migration()
+        img = s.eng.get_image()
+        s.eng.reset()
+        f = s.eng.gradient(iteration=iteration)
+        grd = s.eng.get_gradient(100.0)
+        s.eng.set_model(s.v)
+        return [obs, img[1], img[2], grd[1], grd[2], np.float64(f)]
+
+    s.eng.set_tuning(use_graph=0)
+    base = {it: run(it) for it in (1, 2)}
+    assert not np.array_equal(base[1][3], base[2][3])   # the boundary differs per iteration
+    s.eng.set_tuning(use_graph=1)
+    for rep, it in enumerate((1, 2, 1, 2)):
+        for k, (a, b) in enumerate(zip(run(it), base[it])):
+            assert np.array_equal(a, b), (rep, it, k)
+        s.eng.synchronize()
+
+
 def test_batching_bit_identical():
     a = overthrust_survey(nsrc=5, rbc_nb=41, varVp=800.0, seed=5, max_batch=5)
     b = overthrust_survey(nsrc=5, rbc_nb=41, varVp=800.0, seed=5, max_batch=2)
