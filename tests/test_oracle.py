@@ -10,7 +10,8 @@ index, or a transposed operand fails:
                receiver / injection timing against a brute-force statement
   reverse      exact time reversal of the leapfrog (illumination identity),
                image = sum of S*R against forward fields
-  wavelet      DFT pair identity, filter modulus sqrt(2 pi f) and phase pi/4
+  wavelet      DFT pair identity, a library FFT (numpy's pocketfft) at the
+               wavelet lengths, filter modulus sqrt(2 pi f) and phase pi/4
   sponge       closed form of the Cerjan table and the corner rule
   pad / index  brute force against numpy edge padding, half-cell rounding
   post / step  closed forms of -d2/dz2 and of the parabola vertex
@@ -246,6 +247,25 @@ def test_dft_pair():
         y = np.zeros(n)
         ob.lib().or_dft_c2r(n, re, im, y)
         assert np.abs(y / n - x).max() < 1e-12
+
+
+def test_dft_matches_a_library_fft_at_wavelet_lengths():
+    # the unnormalised r2c / c2r pair of src/modeling/modeling.cu:54-78 at the wavelet
+    # lengths used (odd and even), against pocketfft (numpy); the c2r input
+    # has arbitrary imaginary parts at DC and, for even n, at Nyquist, which
+    # a Hermitian c2r transform ignores
+    rng = np.random.default_rng(3)
+    for n in (2000, 2001, 4149):
+        x = rng.standard_normal(n)
+        re, im = np.zeros(n // 2 + 1), np.zeros(n // 2 + 1)
+        ob.lib().or_dft_r2c(n, x, re, im)
+        X = np.fft.rfft(x)
+        assert np.abs(re + 1j * im - X).max() < 1e-13 * np.abs(X).max()
+        re, im = rng.standard_normal(n // 2 + 1), rng.standard_normal(n // 2 + 1)
+        y = np.zeros(n)
+        ob.lib().or_dft_c2r(n, re, im, y)
+        ref = np.fft.irfft(re + 1j * im, n) * n
+        assert np.abs(y - ref).max() < 1e-13 * np.abs(ref).max()
 
 
 def ricker(nt, dt, fmax):
