@@ -1,6 +1,6 @@
-"""CPU tests of bench.py's command line: it measures one GPU only and says so
-when asked for more, and it refuses unknown workloads, both before anything
-touches a GPU or loads the library."""
+"""CPU tests of bench.py: it measures one GPU only and says so when asked for
+more, and it refuses unknown workloads, both before anything touches a GPU or
+loads the library; its CPU-baseline leg runs and describes its sample."""
 import os
 import subprocess
 import sys
@@ -24,3 +24,24 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 def test_unknown_workload_is_refused():
     r = run("--workload", "no-such-workload")
     assert r.returncode == 2 and "invalid choice" in r.stderr and r.stdout == ""
+
+
+def test_cpu_baseline_leg_on_a_tiny_workload():
+    # the reported CPU baseline (FP32 OpenMP statement of the reference
+    # stencil) runs the same RTM per shot and states its sample and threads
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    import bench
+    import fwi2d_amd as F
+    from fwi2d_amd import configs as C
+    s = C.Survey("tiny", 24, 40, 10.0, 80, 1e-3, 30.0, 12, 2, 8, 20.0, 10.0)
+    v = C.flat_layers(s.nz, s.nx, [1500.0, 2000.0])
+    assert v.shape == (s.nx, s.nz)
+    xs, zs, xr, zr = s.acquisition()
+    w, tlag = F.ricker_wavelet(s.nt, s.dt, s.fmax)
+    obs = np.random.default_rng(0).standard_normal((s.nsrc, s.nrec, s.nt)).astype(np.float32)
+    out = bench.cpu_baseline(s, v, xs, zs, xr, zr, w, tlag, obs, 2, 2)
+    assert out["kind"] == "port" and out["cores"] == 2 and out["unit"] == "Gcell-updates/s"
+    assert out["value"] > 0
+    assert out["sample"].startswith("2 of 2 shots")
+    assert f"{s.nz + 2 * s.nb}x{s.nx + 2 * s.nb} cells x {s.nt + tlag} steps" in out["sample"]
