@@ -18,6 +18,7 @@ FWI_ERR_STATE = 2
 FWI_ERR_HIP = 3
 FWI_ERR_NOMEM = 4
 FWI_ERR_NUMERIC = 5
+FWI_ERR_INTERNAL = 6
 
 ABC = 0
 RBC = 1

@@ -10,6 +10,7 @@
 #include "fwi2d.h"
 #include "kernels.h"
 #include "dsp.h"
+#include "layout.h"
 #include "status.h"
 
 #include <hip/hip_runtime.h>
@@ -56,7 +57,7 @@ using fwi::fail;
     } while (0)
 
 constexpr float kCflLimit = 0.5546f;   // 2-D leapfrog limit of the 8th-order stencil
-constexpr long kGuard = 1024;           // zeroed floats after each step-kernel buffer
+using fwi::kGuard;
 
 struct GridBuf {
     fwi::Grid g{};
@@ -94,7 +95,8 @@ struct fwi_ctx {
     fwi_config cfg{};
     hipStream_t stream = nullptr;
     GridBuf grid[2];
-    long fstride_max = 0;
+    fwi::Layout lay{};         // buffer sizes and fill extents
+    std::map<const void *, long> cap;   // floats allocated per device buffer
     float kc = 0.f;            // dt^2 / dh^2
     float idh2 = 0.f;
 
@@ -186,6 +188,28 @@ int dalloc(T **p, size_t n, hipStream_t stream)
     e = hipMemsetAsync(*p, 0, n * sizeof(T), stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return fail(FWI_ERR_HIP, "zero-fill failed: %s", hipGetErrorString(e));
+    return FWI_OK;
+}
+
+// Zeroed device allocation of n floats owned by the context, recorded with
+// its size so that every fill can be checked against it.
+int cdalloc(fwi_ctx *c, float **p, long n)
+{
+    TRY(dalloc(p, (size_t)n, c->stream));
+    c->cap[*p] = n;
+    return FWI_OK;
+}
+
+// Zero-fill of the first n floats of a context buffer.  An extent beyond the
+// buffer's allocation is an engine bug: it fails with FWI_ERR_INTERNAL
+// instead of writing past the buffer.
+int fill(fwi_ctx *c, float *p, long n)
+{
+    const auto it = c->cap.find(p);
+    const long have = it == c->cap.end() ? -1 : it->second;
+    if (n > have)
+        return fail(FWI_ERR_INTERNAL, "fill of %ld floats exceeds the buffer's %ld", n, have);
+    HIP_TRY(fwi::launch_fill(p, n, 0.f, c->stream));
     return FWI_OK;
 }
 
@@ -389,7 +413,7 @@ int forward_batch(fwi_ctx *c, int kind, int shot0, int nslots, bool record, uint
 {
     GridBuf &gb = c->grid[kind];
     TRY(set_shot0(c, shot0));
-    const long nf = (long)nslots * gb.g.fstride;
+    const long nf = c->lay.pass_fill(kind, nslots);
     if (kind == FWI_BOUNDARY_RBC) {
         const fwi_config &k = c->cfg;
         float vlo, vhi;
@@ -400,7 +424,7 @@ int forward_batch(fwi_ctx *c, int kind, int shot0, int nslots, bool record, uint
     }
     // all four buffers start at zero in this grid's layout (rim and padding
     // cells are never written and must read as 0)
-    for (int k = 0; k < 4; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
+    for (int k = 0; k < 4; ++k) TRY(fill(c, c->d_f[k], nf));
     fwi::FwdStep a = fwd_args(c, kind, record);
     Quad q{{c->d_f[0], c->d_f[1], c->d_f[2], c->d_f[3]}, 1, 0};   // u_0 in b[1], u_{-1} = 0
     HIP_TRY(fwi::launch_fwd_init(a, q.b[q.cur], nslots, c->stream));
@@ -418,8 +442,8 @@ int reverse_batch(fwi_ctx *c, int shot0, int nslots, int mode)
 {
     GridBuf &gb = c->grid[FWI_BOUNDARY_RBC];
     TRY(set_shot0(c, shot0));
-    const long nf = (long)nslots * gb.g.fstride;
-    for (int k = 4; k < 8; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
+    const long nf = c->lay.pass_fill(FWI_BOUNDARY_RBC, nslots);
+    for (int k = 4; k < 8; ++k) TRY(fill(c, c->d_f[k], nf));
     fwi::RevStep a = rev_args(c, mode);
     // source wavefield: cur = u_{T-1}, old = u_T (the forward's old / cur)
     Quad sq = c->fwd;
@@ -456,14 +480,12 @@ int setup_grid(fwi_ctx *c, int kind, int nb)
     gb.on = nb > 0;
     if (!gb.on) return FWI_OK;
     if (nb < 4) return fail(FWI_ERR_ARG, "pad width %d < 4 cells (stencil rim)", nb);
-    gb.g.nb = nb;
-    gb.g.nzz = c->cfg.nz + 2 * nb;
-    gb.g.nxx = c->cfg.nx + 2 * nb;
-    gb.g.ldz = (gb.g.nzz + 3) & ~3;
-    long fs = (long)gb.g.ldz * gb.g.nxx;
-    fs = (fs + 63) & ~63L;     // 256 B aligned slots
-    gb.g.fstride = fs;
-    c->fstride_max = std::max(c->fstride_max, fs);
+    const fwi::GridDims &d = c->lay.grid[kind];
+    gb.g.nb = d.nb;
+    gb.g.nzz = d.nzz;
+    gb.g.nxx = d.nxx;
+    gb.g.ldz = d.ldz;
+    gb.g.fstride = d.fstride;
     return FWI_OK;
 }
 
@@ -493,6 +515,7 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     c->cfg = *cfg;
     c->kc = (float)((double)cfg->dt * cfg->dt / ((double)cfg->dh * cfg->dh));
     c->idh2 = (float)(1.0 / ((double)cfg->dh * cfg->dh));
+    c->lay = fwi::make_layout(*cfg);
     int r = setup_grid(c, FWI_BOUNDARY_ABC, cfg->abc_nb);
     if (r == FWI_OK) r = setup_grid(c, FWI_BOUNDARY_RBC, cfg->rbc_nb);
     if (r != FWI_OK) { delete c; return r; }
@@ -501,7 +524,6 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
 
-    const long S = cfg->max_batch;
     const long n_int = (long)cfg->nz * cfg->nx;
     r = FWI_OK;
     if (r == FWI_OK) r = dalloc(&c->d_vp, n_int, c->stream);
@@ -511,8 +533,7 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     // every buffer a step kernel reads carries kGuard zeroed floats after its
     // last slot: the lanes of a wave past the last row of the last column
     // load there (the loads are not masked per lane)
-    for (int k = 0; k < 8 && r == FWI_OK; ++k)
-        r = dalloc(&c->d_f[k], S * c->fstride_max + kGuard, c->stream);
+    for (int k = 0; k < 8 && r == FWI_OK; ++k) r = cdalloc(c, &c->d_f[k], c->lay.wavefield());
     for (int kind = 0; kind < 2 && r == FWI_OK; ++kind)
         if (c->grid[kind].on) r = dalloc(&c->grid[kind].vpad, c->grid[kind].g.fstride, c->stream);
     if (r == FWI_OK && c->grid[0].on) {
@@ -522,16 +543,15 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
         if (r == FWI_OK)
             r = hipMemcpy(c->d_b1d, b1d.data(), b1d.size() * 4, hipMemcpyHostToDevice) == hipSuccess
                     ? FWI_OK : fail(FWI_ERR_HIP, "b1d upload");
-        if (r == FWI_OK) r = dalloc(&c->d_cabc, c->grid[0].g.fstride + kGuard, c->stream);
+        if (r == FWI_OK) r = cdalloc(c, &c->d_cabc, c->lay.abc_coef());
     }
     if (r == FWI_OK && c->grid[1].on) {
-        const long fs = c->grid[1].g.fstride;
-        r = dalloc(&c->d_crb, S * fs + kGuard, c->stream);
-        if (r == FWI_OK) r = dalloc(&c->d_acc_slot, S * fs + kGuard, c->stream);
-        if (r == FWI_OK) r = dalloc(&c->d_ill_slot, S * fs + kGuard, c->stream);
-        if (r == FWI_OK) r = dalloc(&c->d_image, fs, c->stream);
-        if (r == FWI_OK) r = dalloc(&c->d_grad, fs, c->stream);
-        if (r == FWI_OK) r = dalloc(&c->d_illum, fs, c->stream);
+        r = cdalloc(c, &c->d_crb, c->lay.rb_coef());
+        if (r == FWI_OK) r = cdalloc(c, &c->d_acc_slot, c->lay.slot_acc());
+        if (r == FWI_OK) r = cdalloc(c, &c->d_ill_slot, c->lay.slot_acc());
+        if (r == FWI_OK) r = cdalloc(c, &c->d_image, c->lay.shot_sum());
+        if (r == FWI_OK) r = cdalloc(c, &c->d_grad, c->lay.shot_sum());
+        if (r == FWI_OK) r = cdalloc(c, &c->d_illum, c->lay.shot_sum());
     }
     if (r == FWI_OK) r = dalloc(&c->d_partials, c->npartials, c->stream);
     if (r == FWI_OK) r = dalloc(&c->d_sum, 1, c->stream);
@@ -786,10 +806,10 @@ int fwi_reset(fwi_ctx *c, int32_t what)
 {
     if (!c) return fail(FWI_ERR_ARG, "null context");
     if (!c->grid[1].on) return FWI_OK;
-    const long fs = c->grid[1].g.fstride;
-    if (what & FWI_RESET_IMAGE) HIP_TRY(fwi::launch_fill(c->d_image, fs, 0.f, c->stream));
-    if (what & FWI_RESET_GRADIENT) HIP_TRY(fwi::launch_fill(c->d_grad, fs, 0.f, c->stream));
-    if (what & FWI_RESET_ILLUM) HIP_TRY(fwi::launch_fill(c->d_illum, fs, 0.f, c->stream));
+    const long fs = c->lay.shot_sum();
+    if (what & FWI_RESET_IMAGE) TRY(fill(c, c->d_image, fs));
+    if (what & FWI_RESET_GRADIENT) TRY(fill(c, c->d_grad, fs));
+    if (what & FWI_RESET_ILLUM) TRY(fill(c, c->d_illum, fs));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return FWI_OK;
 }
@@ -937,27 +957,51 @@ int fwi_guarded_step(const double *alpha, const double *f, const float *m, const
     return FWI_OK;
 }
 
-int fwi_rb_velocity(fwi_ctx *c, int32_t shot, uint32_t iteration, float *out)
+}  // extern "C"
+
+namespace {
+
+// Random-boundary velocity of one shot written to the device buffer d_out
+// (nzz * nxx floats) and copied to the host array out.
+int rb_velocity_via(fwi_ctx *c, int32_t shot, uint32_t iteration, float *d_out, float *out)
 {
-    TRY(check_ready(c));
-    TRY(check_shots(c, shot, 1));
-    GridBuf &gb = c->grid[1];
-    if (!gb.on) return fail(FWI_ERR_ARG, "random-boundary grid not configured");
+    const GridBuf &gb = c->grid[1];
     const long ncell = (long)gb.g.nzz * gb.g.nxx;
     TRY(set_shot0(c, shot));
     const fwi_config &k = c->cfg;
     float vlo, vhi;
     rb_bounds(c, &vlo, &vhi);
-    float *d_out = nullptr;
-    TRY(dalloc(&d_out, ncell, c->stream));
     hipError_t e = fwi::launch_rb(gb.vpad, gb.g, 1, c->d_shot0, iteration, k.rbc_seed,
                                   k.rbc_varVp, k.rbc_varVp_rel, vlo, vhi, c->kc, nullptr, d_out,
                                   c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, ncell * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(d_out);
     if (e != hipSuccess) return fail(FWI_ERR_HIP, "rb velocity: %s", hipGetErrorString(e));
     return FWI_OK;
+}
+
+int check_rb_velocity(fwi_ctx *c, int32_t shot, const float *out)
+{
+    TRY(check_ready(c));
+    TRY(check_shots(c, shot, 1));
+    if (!c->grid[1].on) return fail(FWI_ERR_ARG, "random-boundary grid not configured");
+    if (!out) return fail(FWI_ERR_ARG, "null output");
+    return FWI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fwi_rb_velocity(fwi_ctx *c, int32_t shot, uint32_t iteration, float *out)
+{
+    TRY(check_rb_velocity(c, shot, out));
+    const long ncell = (long)c->grid[1].g.nzz * c->grid[1].g.nxx;
+    float *d_out = nullptr;
+    TRY(dalloc(&d_out, ncell, c->stream));
+    const int r = rb_velocity_via(c, shot, iteration, d_out, out);
+    (void)hipFree(d_out);
+    return r;
 }
 
 int fwi_grid_shape(fwi_ctx *c, int32_t boundary, int32_t *nzz, int32_t *nxx, int32_t *nb)
@@ -1026,8 +1070,8 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
     if (nslots < 1 || nslots > c->cfg.max_batch) return fail(FWI_ERR_ARG, "bad slot count");
     if (reps < 1) reps = 1;
     const int T = T_of(c);
-    const long nf = (long)nslots * c->fstride_max;
-    for (int k = 0; k < 8; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
+    const long nf = c->lay.tune_fill(nslots);
+    for (int k = 0; k < 8; ++k) TRY(fill(c, c->d_f[k], nf));
     TRY(set_shot0(c, 0));
     const int vzs[3] = {1, 2, 4};
     double best[3] = {1e30, 1e30, 1e30};
@@ -1105,10 +1149,12 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
         c->xw[kind] = bxw;
     }
     drop_graphs(c);
-    for (int k = 0; k < 8; ++k) HIP_TRY(fwi::launch_fill(c->d_f[k], nf, 0.f, c->stream));
+    for (int k = 0; k < 8; ++k) TRY(fill(c, c->d_f[k], nf));
     if (c->grid[1].on) {
-        HIP_TRY(fwi::launch_fill(c->d_acc_slot, nf, 0.f, c->stream));
-        HIP_TRY(fwi::launch_fill(c->d_ill_slot, nf, 0.f, c->stream));
+        // the reverse launches accumulated into the per-slot image and
+        // illumination of the random-boundary grid
+        TRY(fill(c, c->d_acc_slot, c->lay.acc_fill(nslots)));
+        TRY(fill(c, c->d_ill_slot, c->lay.acc_fill(nslots)));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (fwd_us) *fwd_us = c->grid[1].on ? best[0] : 0.0;

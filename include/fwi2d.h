@@ -30,6 +30,7 @@ extern "C" {
 #define FWI_ERR_HIP     3   /* HIP runtime failure (message has detail) */
 #define FWI_ERR_NOMEM   4   /* device or host allocation failed         */
 #define FWI_ERR_NUMERIC 5   /* CFL violated, non-positive model, ...    */
+#define FWI_ERR_INTERNAL 6  /* an engine extent check failed (a bug)     */
 
 #define FWI_BOUNDARY_ABC 0  /* Cerjan sponge on the abc_nb-padded grid  */
 #define FWI_BOUNDARY_RBC 1  /* seeded random boundary on the rbc_nb grid */

@@ -45,3 +45,27 @@ def test_cpu_baseline_leg_on_a_tiny_workload():
     assert out["value"] > 0
     assert out["sample"].startswith("2 of 2 shots")
     assert f"{s.nz + 2 * s.nb}x{s.nx + 2 * s.nb} cells x {s.nt + tlag} steps" in out["sample"]
+
+
+def test_roofline_record_is_a_fraction_of_the_hbm_peak():
+    # the reverse kernel's record from its event profile (3-step launches,
+    # 64 shots of 601 x 1301 cells): before the counter passes its bytes are
+    # the compulsory 52 B per cell and launch; after them, the measured
+    # traffic.  The per-step-equivalent figure (22 B per cell-update of the
+    # three steps) stays under its own key and may exceed 1.
+    sys.path.insert(0, ROOT)
+    import bench
+    cells = 64 * 601 * 1301
+    n = 100
+    ms = n * 0.7095                              # 709.5 us per launch
+    r = bench.roof(ms, n, n * cells * 3 * 2.0, bench.BYTES_REV, 5400.0, kind=1, steps=3)
+    assert r["compulsory_bytes_per_launch"] == cells * 52.0
+    assert abs(r["frac"] - cells * 52.0 / 709.5e-6 / 8e12) < 1e-4
+    pse = r["per_step_equivalent"]
+    assert pse["algorithmic_bytes_per_launch"] == cells * 3 * 2 * 22.0
+    assert pse["frac"] > 1.0 > r["frac"]
+    bench.set_traffic(r, 1.894e9, 1.246e9)
+    assert r["traffic"] == 3.140e9
+    assert abs(r["frac"] - 3.140e9 / 709.5e-6 / 8e12) < 1e-4 and r["frac"] <= 1.0
+    assert abs(r["achieved"] - r["frac"] * 8000.0) < 0.1
+    assert abs(r["traffic_over_compulsory"] - 3.140e9 / (cells * 52.0)) < 1e-4

@@ -237,6 +237,36 @@ def test_batching_bit_identical():
     assert np.array_equal(s3[0], sa[3])
 
 
+def test_autotune_with_a_wider_sponge_pad():
+    """The reference's own parameters.txt pads the sponge grid wider than the
+    random-boundary grid (abc 1250 m -> 51 cells, rbc 1000 m -> 41 cells at
+    25 m).  The autotuner runs the kernels of both grids and then zeroes what
+    they touched, each buffer to its own grid's extent (every fill is checked
+    against its allocation); the passes after it give the bits of the passes
+    before it."""
+    s = overthrust_survey(nsrc=4, abc_nb=51, abc_factor=0.004, rbc_nb=41, varVp=1000.0,
+                          seed=12, max_batch=4)
+    obs = s.eng.modeling(F.ABC)
+    s.eng.set_observed(obs)
+    s.eng.set_model((s.v * 0.98).astype(np.float32))
+
+    def passes():
+        s.eng.reset()
+        s.eng.migration()
+        img = s.eng.get_image()
+        s.eng.reset()
+        f = s.eng.gradient(iteration=1)
+        grd = s.eng.get_gradient(100.0)
+        return [img[1], img[2], grd[1], grd[2], np.float64(f), s.eng.modeling(F.ABC)]
+
+    before = passes()
+    fwd_us, rev_us = s.eng.autotune(4, reps=1)
+    assert fwd_us > 0 and rev_us > 0
+    after = passes()
+    for k, (a, b) in enumerate(zip(after, before)):
+        assert np.array_equal(a, b), k
+
+
 def migration_oracle(s, obs):
     nb = s.rbc_nb
     siz, six, riz, rix = s.idx(nb)
