@@ -73,6 +73,7 @@ SIGNATURES = [
     ("fwi_guarded_step", C.c_int, [_f64p, _f64p, _f32p, _f32p, C.c_int64, C.c_double,
                                    C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     ("fwi_rb_velocity", C.c_int, [_ctx, C.c_int32, C.c_uint32, _f32p]),
+    ("fwi_rb_velocity_noalloc", C.c_int, [_ctx, C.c_int32, C.c_uint32, _f32p]),
     ("fwi_get_accumulator", C.c_int, [_ctx, C.c_int32, _f32p]),
     ("fwi_set_accumulator", C.c_int, [_ctx, C.c_int32, _f32p]),
     ("fwi_grid_shape", C.c_int, [_ctx, C.c_int32, C.POINTER(C.c_int32),
@@ -268,10 +269,14 @@ class Engine:
     def set_accumulator(self, what, values):
         _check(lib().fwi_set_accumulator(self._h, what, np.ascontiguousarray(values, np.float32)))
 
-    def rb_velocity(self, shot, iteration=0):
+    def rb_velocity(self, shot, iteration=0, noalloc=False):
+        """Random-boundary velocity [nxx, nzz] of a shot's pass; noalloc uses
+        the entry point that stages through the library's static device
+        buffer instead of allocating one."""
         nzz, nxx, _ = self.grid_shape(RBC)
         out = np.zeros((nxx, nzz), np.float32)
-        _check(lib().fwi_rb_velocity(self._h, shot, iteration, out))
+        fn = lib().fwi_rb_velocity_noalloc if noalloc else lib().fwi_rb_velocity
+        _check(fn(self._h, shot, iteration, out))
         return out
 
     def grid_shape(self, boundary):

@@ -21,12 +21,21 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 namespace {
 thread_local std::string g_last_error;
 }
+
+// Staging buffer of fwi_rb_velocity_noalloc: one copy per GPU, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, not device
+// memory: host code uses only the address hipGetSymbolAddress returns on each
+// device (scratch_on_device below), never the name.  Every call on one GPU
+// shares the array, so calls that overlap on one GPU race.
+constexpr long kScratchFloats = 1L << 21;    // 8 MiB: padded grids up to 2,097,152 cells
+static __device__ __align__(256) float scratch[kScratchFloats];
 
 int fwi::fail(int code, const char *fmt, ...)
 {
@@ -127,6 +136,8 @@ struct fwi_ctx {
     int use_graph = 1;
     std::map<long, hipGraphExec_t> graphs;
 
+    float *scratch = nullptr;  // this device's copy of `scratch` (fwi_rb_velocity_noalloc)
+
     bool profile = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double prof_ms[2] = {0, 0};
@@ -137,6 +148,31 @@ struct fwi_ctx {
 namespace {
 
 int T_of(const fwi_ctx *c) { return c->cfg.nt + c->tlag; }
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept per device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_device(float **out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(FWI_ERR_HIP, "device ordinal %d outside [0, %d)", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FWI_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return FWI_OK;
+}
 
 fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g, int kind)
 {
@@ -555,6 +591,7 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     }
     if (r == FWI_OK) r = dalloc(&c->d_partials, c->npartials, c->stream);
     if (r == FWI_OK) r = dalloc(&c->d_sum, 1, c->stream);
+    if (r == FWI_OK) r = scratch_on_device(&c->scratch);
     if (r != FWI_OK) { fwi_destroy(c); return r; }
     // default launch shapes (the autotuner refines them, see the fields)
     *out = c;
@@ -1002,6 +1039,19 @@ int fwi_rb_velocity(fwi_ctx *c, int32_t shot, uint32_t iteration, float *out)
     const int r = rb_velocity_via(c, shot, iteration, d_out, out);
     (void)hipFree(d_out);
     return r;
+}
+
+int fwi_rb_velocity_noalloc(fwi_ctx *c, int32_t shot, uint32_t iteration, float *out)
+{
+    TRY(check_rb_velocity(c, shot, out));
+    const long ncell = (long)c->grid[1].g.nzz * c->grid[1].g.nxx;
+    if (ncell > kScratchFloats)
+        return fail(FWI_ERR_ARG,
+                    "random-boundary grid of %ld cells exceeds the %ld-cell staging buffer of "
+                    "fwi_rb_velocity_noalloc (use fwi_rb_velocity)", ncell, kScratchFloats);
+    if (!c->scratch) return fail(FWI_ERR_STATE, "no staging buffer on this context's device");
+    // the kernel writes every one of the ncell values: nothing to clear first
+    return rb_velocity_via(c, shot, iteration, c->scratch, out);
 }
 
 int fwi_grid_shape(fwi_ctx *c, int32_t boundary, int32_t *nzz, int32_t *nxx, int32_t *nb)

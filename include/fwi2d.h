@@ -171,6 +171,16 @@ int fwi_set_accumulator(fwi_ctx *ctx, int32_t what, const float *in);
  * of shot `shot` in FWI iteration `iteration` uses (migration: iteration 0). */
 int fwi_rb_velocity(fwi_ctx *ctx, int32_t shot, uint32_t iteration, float *vp_padded_out);
 
+/* fwi_rb_velocity without a device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The value is staged in a static
+ * device buffer of the library, one per GPU (8 MiB, held until the process
+ * ends), so the random-boundary grid may hold at most 2,097,152 cells
+ * (nzz * nxx); a larger grid fails with FWI_ERR_ARG.  Calls that overlap on
+ * one GPU (from several threads or contexts) share the buffer and race. */
+int fwi_rb_velocity_noalloc(fwi_ctx *ctx, int32_t shot, uint32_t iteration,
+                            float *vp_padded_out);
+
 /* Padded grid of a boundary kind: nzz, nxx, nb. */
 int fwi_grid_shape(fwi_ctx *ctx, int32_t boundary, int32_t *nzz, int32_t *nxx, int32_t *nb);
 
