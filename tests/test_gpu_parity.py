@@ -61,15 +61,16 @@ class Survey:
         stream = (int(iteration) << 32) | int(shot)
         return ob.rb_velocity(vpad, self.rbc_nb, ab, rel, vlo, vhi, self.seed, stream)
 
-    def oracle_forward(self, shot, boundary, vpad=None, model=None):
+    def oracle_forward(self, shot, boundary, vpad=None, model=None, precision="f64"):
         nb = self.abc_nb if boundary == F.ABC else self.rbc_nb
         siz, six, riz, rix = self.idx(nb)
         if vpad is None:
             vpad = ob.expand(self.v if model is None else model, nb)
         b1d = ob.abc_b1d(nb, self.abc_factor) if boundary == F.ABC else None
-        return ob.forward(vpad, self.dh, self.dt, self.nt, self.tlag, self.w.astype(np.float64),
+        w = self.w.astype(np.float64 if precision == "f64" else np.float32)
+        return ob.forward(vpad, self.dh, self.dt, self.nt, self.tlag, w,
                           int(siz[shot]), int(six[shot]), riz, rix, abc=(boundary == F.ABC),
-                          b1d=b1d, nb=nb)
+                          b1d=b1d, nb=nb, precision=precision)
 
 
 def overthrust_survey(nt=1001, nsrc=3, **kw):
@@ -335,21 +336,26 @@ def test_dense_receiver_column_matches_oracle():
             assert np.array_equal(raw, first), (st, vz, xw)
 
 
-def gradient_oracle(s, obs, model, iteration, zmask):
+def gradient_oracle(s, obs, model, iteration, zmask, precision="f64"):
+    """FP64 oracle of the FWI gradient pass and its post-processing;
+    precision="f32" runs the oracle's FP32 restatement instead (the
+    reference's own arithmetic order in single precision)."""
     nb = s.rbc_nb
     siz, six, riz, rix = s.idx(nb)
     nxx, nzz = s.nx + 2 * nb, s.nz + 2 * nb
-    acc = np.zeros((nxx, nzz))
-    ill = np.zeros((nxx, nzz))
+    dtype = np.float64 if precision == "f64" else np.float32
+    acc = np.zeros((nxx, nzz), dtype)
+    ill = np.zeros((nxx, nzz), dtype)
     sumsq = 0.0
     for k in range(len(s.xsrc)):
-        cal, _, _ = s.oracle_forward(k, F.ABC, model=model)
-        res = obs[k].astype(np.float64) - cal
-        sumsq += float(np.sum(res * res))
+        cal, _, _ = s.oracle_forward(k, F.ABC, model=model, precision=precision)
+        res = obs[k].astype(dtype) - cal
+        sumsq += float(np.sum(res.astype(np.float64) ** 2))
         vrb = s.rb_oracle(k, iteration)
-        _, P, Q = s.oracle_forward(k, F.RBC, vpad=vrb)
-        ob.reverse(vrb, s.dh, s.dt, s.nt, s.tlag, P, Q, riz, rix, res, 1, acc=acc, illum=ill)
-    crop = lambda a: np.ascontiguousarray(a[nb:nb + s.nx, nb:nb + s.nz])
+        _, P, Q = s.oracle_forward(k, F.RBC, vpad=vrb, precision=precision)
+        ob.reverse(vrb, s.dh, s.dt, s.nt, s.tlag, P, Q, riz, rix, res, 1, acc=acc, illum=ill,
+                   precision=precision)
+    crop = lambda a: np.ascontiguousarray(a[nb:nb + s.nx, nb:nb + s.nz], np.float64)
     g, il = crop(acc), crop(ill)
     return ob.gradient_post(g, il, s.dh, zmask), g, il, sumsq
 
