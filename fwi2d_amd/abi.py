@@ -76,6 +76,8 @@ SIGNATURES = [
     ("fwi_rb_velocity_noalloc", C.c_int, [_ctx, C.c_int32, C.c_uint32, _f32p]),
     ("fwi_get_accumulator", C.c_int, [_ctx, C.c_int32, _f32p]),
     ("fwi_set_accumulator", C.c_int, [_ctx, C.c_int32, _f32p]),
+    ("fwi_get_config", C.c_int, [_ctx, C.POINTER(Config)]),
+    ("fwi_geometry_size", C.c_int, [_ctx, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("fwi_grid_shape", C.c_int, [_ctx, C.c_int32, C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("fwi_synchronize", C.c_int, [_ctx]),
@@ -94,6 +96,8 @@ NN_SIGNATURES = [
     ("fwi_nn_create", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                 C.POINTER(C.c_void_p)]),
     ("fwi_nn_destroy", C.c_int, [_ctx]),
+    ("fwi_nn_get_shape", C.c_int, [_ctx, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32)]),
     ("fwi_nn_num_params", C.c_int, [_ctx, C.POINTER(C.c_int64)]),
     ("fwi_nn_get_params", C.c_int, [_ctx, _f32p]),
     ("fwi_nn_set_params", C.c_int, [_ctx, _f32p]),
@@ -104,6 +108,37 @@ NN_SIGNATURES = [
     ("fwi_nn_train_step", C.c_int, [_ctx, C.c_int32, _f32p, _f32p, C.c_float,
                                     C.POINTER(C.c_float)]),
     ("fwi_nn_mfma_probe", C.c_int, [_f32p, _f32p, _f32p]),
+]
+
+TRIAL_NONE, TRIAL_PLAIN, TRIAL_NETWORK, TRIAL_GIVEN = 0, 1, 2, 3
+
+
+class UpdateOpts(C.Structure):
+    """fwi_update_opts (include/fwi2d_update.h)."""
+    _fields_ = [("alpha1", C.c_double), ("alpha2", C.c_double), ("m_min", C.c_double),
+                ("zmask", C.c_float), ("trial", C.c_int32), ("net", C.c_void_p),
+                ("direction", C.c_void_p), ("nmultiples", C.c_int32),
+                ("multiples", C.c_double * 4)]
+
+
+class UpdateInfo(C.Structure):
+    """fwi_update_info (include/fwi2d_update.h)."""
+    _fields_ = [("f", C.c_double * 3), ("step", C.c_double), ("rule", C.c_int32),
+                ("plain_misfit", C.c_double), ("trial_misfit", C.c_double),
+                ("trial_multiple", C.c_double), ("taken", C.c_int32), ("passes", C.c_int32)]
+
+    def as_dict(self):
+        return {"f": list(self.f), "step": self.step, "rule": self.rule,
+                "plain_misfit": self.plain_misfit, "trial_misfit": self.trial_misfit,
+                "trial_multiple": self.trial_multiple, "taken": self.taken,
+                "passes": self.passes}
+
+
+UPDATE_SIGNATURES = [
+    ("fwi_update_model", C.c_int, [_ctx, C.POINTER(UpdateOpts), C.c_double, _f32p, _vp, _vp,
+                                   _f32p, C.POINTER(UpdateInfo)]),
+    ("fwi_trial_direction", C.c_int, [_ctx, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                      _f32p, _f32p, _f32p, _f32p, _f32p]),
 ]
 
 _lib = None
@@ -118,7 +153,7 @@ def lib():
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
                 f"g.build()'` or `make -C fwi2d_amd/csrc`")
         L = C.CDLL(LIB_PATH)
-        for name, res, args in SIGNATURES + NN_SIGNATURES:
+        for name, res, args in SIGNATURES + NN_SIGNATURES + UPDATE_SIGNATURES:
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -297,6 +332,37 @@ class Engine:
                                     C.byref(c)))
         return st.value, a.value, b.value, c.value
 
+    def get_config(self):
+        cfg = Config()
+        _check(lib().fwi_get_config(self._h, C.byref(cfg)))
+        return cfg
+
+    def update_model(self, m, g, f0, alpha1, alpha2, m_min, zmask=0.0, trial=TRIAL_NONE,
+                     net=None, direction=None, multiples=(2.0, 4.0), raw=None, illum=None):
+        """fwi_update_model: the line search, the step and the update of
+        m (slowness squared, [nx, nz]; a new array is returned) with an
+        optional trial direction.  Returns (new m, info dict)."""
+        m2 = np.array(m, np.float32, order="C", copy=True)
+        g = np.ascontiguousarray(g, np.float32)
+        raw = None if raw is None else np.ascontiguousarray(raw, np.float32)
+        illum = None if illum is None else np.ascontiguousarray(illum, np.float32)
+        d = None if direction is None else np.ascontiguousarray(direction, np.float32)
+        for a in (g, raw, illum, d):
+            if a is not None and a.shape != m2.shape:
+                raise ValueError("m, g, raw gradient, illumination and direction must have "
+                                 "one shape")
+        o = UpdateOpts()
+        o.alpha1, o.alpha2, o.m_min, o.zmask, o.trial = alpha1, alpha2, m_min, zmask, trial
+        o.net = net._h if net is not None else None
+        o.direction = _p(d)
+        o.nmultiples = len(multiples)
+        for k, v in enumerate(multiples):
+            o.multiples[k] = v
+        info = UpdateInfo()
+        _check(lib().fwi_update_model(self._h, C.byref(o), float(f0), g, _p(raw), _p(illum),
+                                      m2, C.byref(info)))
+        return m2, info.as_dict()
+
     def autotune(self, nslots, reps=10):
         f, r = C.c_double(), C.c_double()
         _check(lib().fwi_autotune(self._h, nslots, reps, C.byref(f), C.byref(r)))
@@ -341,6 +407,15 @@ class Net:
             self.close()
         except Exception:
             pass
+
+    def direction(self, nx, nz, dh, zmask, raw, illum, m, g):
+        """fwi_trial_direction: the network's search direction [nx, nz]."""
+        d = np.zeros((nx, nz), np.float32)
+        arr = [np.ascontiguousarray(a, np.float32) for a in (raw, illum, m, g)]
+        if any(a.shape != (nx, nz) for a in arr):
+            raise ValueError(f"inputs must be {nx} x {nz}")
+        _check(lib().fwi_trial_direction(self._h, nx, nz, dh, zmask, *arr, d))
+        return d
 
     def get_params(self):
         p = np.zeros(self.nparams, np.float32)

@@ -1054,6 +1054,21 @@ int fwi_rb_velocity_noalloc(fwi_ctx *c, int32_t shot, uint32_t iteration, float 
     return rb_velocity_via(c, shot, iteration, c->scratch, out);
 }
 
+int fwi_get_config(fwi_ctx *c, fwi_config *out)
+{
+    if (!c || !out) return fail(FWI_ERR_ARG, "null argument");
+    *out = c->cfg;
+    return FWI_OK;
+}
+
+int fwi_geometry_size(fwi_ctx *c, int32_t *nsrc, int32_t *nrec)
+{
+    if (!c) return fail(FWI_ERR_ARG, "null context");
+    if (nsrc) *nsrc = c->have_geometry ? c->nsrc : 0;
+    if (nrec) *nrec = c->have_geometry ? c->nrec : 0;
+    return FWI_OK;
+}
+
 int fwi_grid_shape(fwi_ctx *c, int32_t boundary, int32_t *nzz, int32_t *nxx, int32_t *nb)
 {
     if (!c || boundary < 0 || boundary > 1) return fail(FWI_ERR_ARG, "bad argument");

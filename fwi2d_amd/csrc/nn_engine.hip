@@ -391,6 +391,15 @@ int fwi_nn_create(int32_t batch, int32_t height, int32_t width, uint64_t seed, f
     return FWI_OK;
 }
 
+int fwi_nn_get_shape(fwi_nn *n, int32_t *batch, int32_t *height, int32_t *width)
+{
+    if (!n) return fail(FWI_ERR_ARG, "null network");
+    if (batch) *batch = n->B;
+    if (height) *height = n->H;
+    if (width) *width = n->W;
+    return FWI_OK;
+}
+
 int fwi_nn_num_params(fwi_nn *n, int64_t *count)
 {
     if (!n || !count) return fail(FWI_ERR_ARG, "null argument");

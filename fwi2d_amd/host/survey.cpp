@@ -177,6 +177,39 @@ void Migration::export_seismic()
 
 // ------------------------------------------------------------------ Inversion
 
+Inversion::~Inversion()
+{
+    if (net) fwi_nn_destroy(net);
+}
+
+// inv_trial_direction = none | plain | network, inv_trial_network = file of
+// the network parameters (fwi2d_nn.h layout, raw float32),
+// inv_trial_multiples = k1,k2,... (1 to 4 multiples of the plain step).
+void Inversion::set_trial()
+{
+    const std::string kind = catch_parameter("inv_trial_direction", parameters, "none");
+    if (kind == "none") return;
+    if (kind == "plain") trial = FWI_TRIAL_PLAIN;
+    else if (kind == "network") trial = FWI_TRIAL_NETWORK;
+    else throw std::runtime_error("Error: inv_trial_direction must be none, plain or network, "
+                                  "not '" + kind + "'");
+    for (const std::string &k : split(catch_parameter("inv_trial_multiples", parameters, "2,4"), ','))
+        multiples.push_back(std::stod(k));
+    if (multiples.empty() || multiples.size() > 4)
+        throw std::runtime_error("Error: inv_trial_multiples takes 1 to 4 values");
+    if (trial != FWI_TRIAL_NETWORK) return;
+    const int H = nx + (4 - nx % 4) % 4, W = nz + (4 - nz % 4) % 4;
+    check(fwi_nn_create(1, H, W, 0, &net), "network");
+    int64_t np = 0;
+    check(fwi_nn_num_params(net, &np), "network size");
+    std::vector<float> p((size_t)np);
+    const std::string file = catch_parameter("inv_trial_network", parameters);
+    import_binary_float(file, p.data(), (long)np);
+    check(fwi_nn_set_params(net, p.data()), "network parameters");
+    raw_grad.resize((size_t)nx * nz);
+    raw_illum.resize((size_t)nx * nz);
+}
+
 void Inversion::set_parameters()
 {
     title = "Full Waveform Inversion";
@@ -203,6 +236,7 @@ void Inversion::set_parameters()
     set_properties();
     vmin0 = vmin;
     vmax0 = vmax;
+    set_trial();
     create_engine(rbc_nb, rbc_varVp, rbc_varVp_rel, seed);
     // observed data stay resident on the device (read once, not every pass)
     const long per = (long)geometry.nrec * nt;
@@ -292,7 +326,9 @@ void Inversion::compute_gradient()
     if (iteration < max_iteration) {
         check(fwi_gradient(ctx, 0, geometry.nsrc, (uint32_t)iteration, &sum_res), "gradient");
         double gmax = 0.0;
-        check(fwi_get_gradient(ctx, zmask, gradient.data(), nullptr, nullptr, &gmax),
+        const bool raw = trial == FWI_TRIAL_NETWORK;   // the network reads the raw fields
+        check(fwi_get_gradient(ctx, zmask, gradient.data(), raw ? raw_grad.data() : nullptr,
+                               raw ? raw_illum.data() : nullptr, &gmax),
               "get_gradient");
         std::cout << "iteration " << iteration << ": gradient pass done, max|g| = " << gmax
                   << "\n";
@@ -317,50 +353,40 @@ double Inversion::min_slowness2() const
     return 1.0 / (vmax_stable * vmax_stable);
 }
 
-double Inversion::linesearch(float alpha)
-{
-    // a trial model that cannot be propagated (m <= 0 or faster than the
-    // stability limit) has no misfit: +inf, which the step rule skips
-    const float m_min = (float)min_slowness2();
-    std::vector<float> trial(model.size());
-    for (size_t k = 0; k < model.size(); ++k) {
-        const float m = model[k] - alpha * gradient[k];
-        if (!(m > 0.f && m >= m_min)) return std::numeric_limits<double>::infinity();
-        trial[k] = 1.0f / std::sqrt(m);
-    }
-    check(fwi_set_model(ctx, trial.data()), "set_model (trial)");
-    double s = 0.0;
-    check(fwi_misfit(ctx, 0, geometry.nsrc, &s), "misfit");
-    return std::sqrt(s);
-}
-
 void Inversion::optimization()
 {
+    // trial steps 0.02 and 0.05 of 1/vmin^2 - 1/vmax^2 of the initial model
+    // (inversion.cu:288-301)
     const double dm = 1.0 / ((double)vmin0 * vmin0) - 1.0 / ((double)vmax0 * vmax0);
-    const double a[3] = {0.0, 0.02 * dm, 0.05 * dm};
-    double f[3];
-    f[0] = residuo.back();
-    f[1] = linesearch((float)a[1]);
-    f[2] = linesearch((float)a[2]);
-    int32_t rule = 0;
-    check(fwi_guarded_step(a, f, model.data(), gradient.data(), (int64_t)model.size(),
-                           min_slowness2(), &step, &rule),
-          "step");
+    fwi_update_opts o{};
+    o.alpha1 = 0.02 * dm;
+    o.alpha2 = 0.05 * dm;
+    o.m_min = min_slowness2();
+    o.zmask = zmask;
+    o.trial = trial;
+    o.net = net;
+    o.nmultiples = (int32_t)multiples.size();
+    for (size_t k = 0; k < multiples.size(); ++k) o.multiples[k] = multiples[k];
+    check(fwi_update_model(ctx, &o, residuo.back(), gradient.data(),
+                           raw_grad.empty() ? nullptr : raw_grad.data(),
+                           raw_illum.empty() ? nullptr : raw_illum.data(), model.data(), &last),
+          "update");
     static const char *names[3] = {"parabola vertex", "best trial step", "no decrease, no step"};
-    std::cout << "line search: f = " << f[0] << ", " << f[1] << ", " << f[2] << " -> step "
-              << step << " (" << names[rule] << ")\n";
+    std::cout << "line search: f = " << last.f[0] << ", " << last.f[1] << ", " << last.f[2]
+              << " -> step " << last.step << " (" << names[last.rule] << ")\n";
+    if (trial != FWI_TRIAL_NONE && last.step > 0.0) {
+        const auto prec = std::cout.precision(12);
+        std::cout << "trial direction: misfit " << last.trial_misfit << " at " << last.trial_multiple
+                  << " x step against " << last.plain_misfit << " for the plain step -> "
+                  << (last.taken ? "trial taken" : "plain step") << "\n";
+        std::cout.precision(prec);
+    }
 }
 
 void Inversion::update_model()
 {
-    std::vector<float> v(model.size());
-    for (size_t k = 0; k < model.size(); ++k) {
-        model[k] = model[k] - (float)step * gradient[k];   // positive by the step guard
-        v[k] = 1.0f / std::sqrt(model[k]);
-    }
-    vp = v;
-    // the accepted model is the one the next gradient uses
-    check(fwi_set_model(ctx, vp.data()), "set_model");
+    // fwi_update_model has updated m and left the model in the engine
+    for (size_t k = 0; k < model.size(); ++k) vp[k] = 1.0f / std::sqrt(model[k]);
     export_binary_float(output_folder + "model_FWI_iteration_" + std::to_string(iteration) + "_" +
                             fname_suffix(fmax, nz, nx, dh),
                         vp.data(), (long)vp.size());

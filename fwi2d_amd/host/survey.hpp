@@ -11,6 +11,7 @@
 
 #include "admin.hpp"
 #include "fwi2d.h"
+#include "fwi2d_update.h"
 
 namespace fwi {
 
@@ -60,12 +61,17 @@ private:
 
 class Inversion : public Modeling {
 public:
+    ~Inversion() override;
     bool converged = false;
     void set_parameters();          // inversion.cu:3-53
     void compute_gradient();        // inversion.cu:111-166
     void check_convergence();       // inversion.cu:275-284
-    void optimization();            // inversion.cu:286-308
-    void update_model();            // inversion.cu:344-358
+    // The line search, the step and m <- m - step g (inversion.cu:286-342,
+    // 344-349) in fwi_update_model; with inv_trial_direction = plain or
+    // network it also tries the trial direction and keeps it only when its
+    // forward-modeled misfit is below the plain update's.
+    void optimization();
+    void update_model();            // model file of the iteration (inversion.cu:350-358)
     void export_convergence();      // inversion.cu:360-372
     void export_final_model();      // inversion.cu:374-379
     // Checkpoint / resume (no reference counterpart): after every model
@@ -86,13 +92,20 @@ private:
     float zmask = 0;
     float rbc_varVp = 0, rbc_varVp_rel = 0;
     int rbc_nb = 0;
-    double sum_res = 0, step = 0;
+    double sum_res = 0;
     float vmin0 = 0, vmax0 = 0;     // extremes of the initial model (dm of the line search)
     std::string input_folder, input_prefix, output_folder, residuo_folder;
     std::string checkpoint_file;
     bool resume = false;
-    double linesearch(float alpha);  // inversion.cu:310-342
+    // trial direction (no reference counterpart): FWI_TRIAL_NONE, _PLAIN or
+    // _NETWORK, the network and its raw inputs, the multiples of the step
+    int trial = FWI_TRIAL_NONE;
+    fwi_nn *net = nullptr;
+    std::vector<double> multiples;
+    std::vector<float> raw_grad, raw_illum;
+    fwi_update_info last{};         // record of the last update
     double min_slowness2() const;
+    void set_trial();
 };
 
 }  // namespace fwi

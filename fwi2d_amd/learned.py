@@ -1,30 +1,31 @@
-"""Learned-preconditioner FWI on top of the engine and the network ABI.
+"""Learned-preconditioner FWI: sample generation, training and the
+comparison runs, over the engine, the network and the C++ model update.
 
-Three tools:
+The FWI iteration itself is C++ (include/fwi2d_update.h, used by the
+inversion executable as well): the gradient pass and its post-processing
+(fwi_gradient, fwi_get_gradient), then fwi_update_model -- the reference's
+two-point parabolic line search and guarded step (inversion.cu:286-358) and,
+for the trial runs, a second direction d tried at multiples of the plain
+step and taken only when its forward-modeled misfit is below the
+forward-modeled misfit of the plain update.  This module only orchestrates:
 
-* ``make_samples`` runs plain FWI (the reference's algorithm:
-  inversion.cu:111-358, with the accepted model used next and the guarded
-  step) on seeded synthetic models and records, per iteration k, the input
-  (raw gradient, illumination, model m_k) and the accepted updates that the
-  plain run still makes after iteration k, m_end - m_k.  A single accepted
-  update -step*g is collinear with the post-processed gradient g, so it would
-  only teach the network the post-processing; the accumulated updates carry
-  what the remaining iterations add.
-* The network is a learned correction of the plain direction: the search
-  direction is d = -g + net(x) (then scaled to max |d| = 1), and the training
-  target is the normalised accumulated update minus -g.  A network that
-  outputs zeros gives d = -g.
+* ``make_samples`` runs plain FWI on seeded synthetic models and records,
+  per iteration k, the input (raw gradient, illumination, model m_k) and the
+  accepted updates that the plain run still makes after iteration k,
+  m_end - m_k.  A single accepted update -step*g is collinear with the
+  post-processed gradient g, so it would only teach the network the
+  post-processing; the accumulated updates carry what the remaining
+  iterations add.
+* The network is a learned correction of the plain direction: the trial
+  direction is d = -g + net(x) (fwi_trial_direction), and the training target
+  is the normalised accumulated update minus -g.  A network that outputs zeros
+  gives d = -g, the control (FWI_TRIAL_PLAIN).
 * ``train`` fits the network (BF16 MFMA kernels, Adam) to those samples; every
   input channel and the accumulated update are normalised by their maximum
   magnitude per sample.  Both sample generation (a file per training model)
   and training (a file rewritten every epoch) can checkpoint and resume.
-* ``run_fwi`` runs plain FWI or FWI with a trial direction.  Such an iteration
-  computes the gradient, the plain two-point line search and its guarded
-  step, then tries the direction d at two multiples of the plain step length;
-  the better trial is taken only when its forward-modeled misfit is below the
-  misfit the plain line search predicts for its own step, otherwise the plain
-  step is taken.  Trial directions get the gradient's depth mask.  ``plain_direction`` (d = -g) is the control that isolates
-  what the network adds to the longer trial steps.
+* ``run_fwi`` runs plain FWI or FWI with a trial direction over a multiscale
+  band schedule and records the misfit trajectory.
 
 Multiscale bands rebuild the wavelet and re-model the observed data per band
 from the true model (a synthetic inverse-crime setup).
@@ -97,6 +98,12 @@ COMPARE_SMALL = dataclasses.replace(TRAIN_SMALL, nsrc=32, nrec=384, seed=5,
 TRAIN_MID = FwiSetup(nz=200, nx=800, dh=10.0, nt=3126, dt=8e-4, nsrc=64, nrec=400,
                      zsrc=20.0, zrec=10.0, rbc_nb=100, abc_nb=80, varVp_rel=0.4, seed=4,
                      peaks=(4.0, 8.0, 12.0), band_iters=(3, 3, 2), zmask=60.0)
+# The training configuration at its stated size (256 x 768 at 10 m, 96
+# shots, 8 plain iterations per model over the 4 / 8 / 12 Hz bands); the
+# record length and pads follow the 200 x 800 setup (2.5 s at 0.8 ms).
+TRAIN_FULL = FwiSetup(nz=256, nx=768, dh=10.0, nt=3126, dt=8e-4, nsrc=96, nrec=768,
+                      zsrc=20.0, zrec=10.0, rbc_nb=100, abc_nb=80, varVp_rel=0.4, seed=4,
+                      peaks=(4.0, 8.0, 12.0), band_iters=(3, 3, 2), zmask=60.0)
 # The comparison configuration at full size: 401 x 1601 at 10 m, 200-cell
 # random boundary, 256 shots, 1601 receivers, nt 5001 at 0.6 ms, bands
 # 4 / 8 / 12 Hz (20 iterations: 7 / 7 / 6).
@@ -116,7 +123,7 @@ def slowness2(v):
 
 class Fwi:
     """One survey on one engine: observed data per band, the gradient pass,
-    the misfit of a model and the plain line search."""
+    the misfit of a model and the C++ model update."""
 
     def __init__(self, setup, v_true, v_init, autotune=True):
         s = self.setup = setup
@@ -136,8 +143,9 @@ class Fwi:
             self.bands.append((w, tlag, self.eng.modeling(A.ABC)))
         self.v_true = np.asarray(v_true, np.float32)
         self.m0 = slowness2(v_init)
+        # line-search trial steps of the reference (inversion.cu:288-301)
         self.dm = float(1.0 / float(np.min(v_init)) ** 2 - 1.0 / float(np.max(v_init)) ** 2)
-        self.alphas = np.array([0.0, 0.02 * self.dm, 0.05 * self.dm])
+        self.alphas = (0.02 * self.dm, 0.05 * self.dm)
         vstable = 0.999 * 0.5546 * s.dh / s.dt     # fastest velocity the propagator accepts
         self.m_min = 1.0 / (vstable * vstable)
         self.current = None
@@ -166,22 +174,14 @@ class Fwi:
         g, raw, ill, _ = self.eng.get_gradient(self.setup.zmask)
         return float(np.sqrt(s2)), g, raw, ill
 
-    def plain_search(self, m, g, f0):
-        """Two trial misfits, the guarded step and the misfit the line search
-        predicts for it (parabola value for the vertex, the trial's own misfit
-        otherwise)."""
-        a = self.alphas
-        f = np.array([f0, self.misfit(m - np.float32(a[1]) * g),
-                      self.misfit(m - np.float32(a[2]) * g)])
-        step, rule = A.guarded_step(a, f, m, g, self.m_min)
-        if rule == 0:
-            coef = np.polyfit(a, f, 2)
-            pred = float(np.polyval(coef, step))
-        elif rule == 1:
-            pred = float(f[int(np.argmin(np.abs(a - step)))])
-        else:
-            pred = float(f0)
-        return step, rule, pred, f
+    def update(self, m, g, f0, trial=A.TRIAL_NONE, net=None, direction=None,
+               multiples=(2.0, 4.0), raw=None, ill=None):
+        """fwi_update_model: (new m, info) of the plain update, or of the
+        trial direction when it passes the acceptance rule."""
+        return self.eng.update_model(m, g, f0, self.alphas[0], self.alphas[1], self.m_min,
+                                     zmask=self.setup.zmask, trial=trial, net=net,
+                                     direction=direction, multiples=multiples, raw=raw,
+                                     illum=ill)
 
 
 def normalise_inputs(raw, ill, m):
@@ -193,85 +193,77 @@ def normalise_inputs(raw, ill, m):
     return x
 
 
-class Direction:
-    """The network as a search-direction generator for [nx, nz] grids.  The
-    network runs on the grid padded (edge values repeated) to multiples of 4
-    (its two 2x2 pools) and its output is cropped back."""
-
-    def __init__(self, nx, nz, params=None, seed=0, batch=1):
-        self.nx, self.nz = nx, nz
-        self.px, self.pz = (-nx) % 4, (-nz) % 4
-        self.net = A.Net(batch, nx + self.px, nz + self.pz, seed)
-        if params is not None:
-            self.net.set_params(params)
-
-    def close(self):
-        self.net.close()
-
-    def __call__(self, raw, ill, m, g):
-        x = normalise_inputs(raw, ill, m)
-        if self.px or self.pz:
-            x = np.pad(x, ((0, 0), (0, self.px), (0, self.pz)), mode="edge")
-        y = self.net.forward(np.ascontiguousarray(x)[None])[0][:self.nx, :self.nz]
-        d = y - g
-        s = float(np.max(np.abs(d)))
-        return d / s if s > 0 else d
+def network(nx, nz, params=None, seed=0):
+    """The network for [nx, nz] models: its image is the model grid edge-padded
+    to multiples of 4 (its two 2x2 pools), the shape fwi_trial_direction
+    expects."""
+    net = A.Net(1, nx + (-nx) % 4, nz + (-nz) % 4, seed)
+    if params is not None:
+        net.set_params(params)
+    return net
 
 
-def plain_direction(raw, ill, m, g):
-    """Control: the plain search direction -g (max |g| = 1) in place of the
-    network, with the same trial multiples and acceptance rule."""
-    return -g
+RUNS = {"plain": A.TRIAL_NONE, "plain_short": A.TRIAL_NONE, "control": A.TRIAL_PLAIN,
+        "learned": A.TRIAL_NETWORK}
 
 
-def run_fwi(fwi, direction=None, multiples=(2.0, 4.0), band_iters=None, log=None):
-    """Plain FWI (direction=None) or FWI with a trial direction and the
-    acceptance rule.  band_iters overrides the setup's iterations per band.
-    Returns a history list with the misfit f0 of every iteration (at its
-    band), the wall time since the start, the rule taken and the model error,
-    plus the final misfit and the final model."""
+def run_fwi(fwi, trial=A.TRIAL_NONE, net=None, direction=None, multiples=(2.0, 4.0),
+            band_iters=None, log=None):
+    """Plain FWI (trial FWI_TRIAL_NONE) or FWI with a trial direction and the
+    acceptance rule (FWI_TRIAL_PLAIN: the control d = -g; FWI_TRIAL_NETWORK:
+    d = -g + net(x); FWI_TRIAL_GIVEN: d = direction(raw, ill, m, g), a
+    callable).  band_iters overrides the setup's iterations per band.
+
+    Returns (history, trajectory, final model).  history has one entry per
+    iteration (the misfit f0 of its starting model, the update's record, wall
+    time, model error) and a last entry with the final model's misfit.  The
+    trajectory lists every forward-modeled misfit of a model the run adopts,
+    with the wall time at which it was known: f0 after each gradient pass,
+    the adopted model's misfit as soon as the update has modeled it (the
+    trial runs always do; the plain run when its step is a line-search
+    trial), and the final misfit.  time_to_misfit reads it."""
     s = fwi.setup if band_iters is None else dataclasses.replace(fwi.setup,
                                                                  band_iters=tuple(band_iters))
     its = s.iterations
     m = fwi.m0.copy()
     fwi.eng.reset()                       # illumination accumulates from here on
-    hist = []
+    hist, traj = [], []
+    rms = lambda mm: float(np.sqrt(np.mean((velocity(mm) - fwi.v_true) ** 2)))
     t0 = time.perf_counter()
     for it in range(its):
-        fwi.use_band(s.band(it))
+        b = s.band(it)
+        fwi.use_band(b)
         f0, g, raw, ill = fwi.gradient(m, it)
-        step, rule, pred, f = fwi.plain_search(m, g, f0)
-        taken = "plain"
-        new = m - np.float32(step) * g
-        if direction is not None and step > 0.0:
+        traj.append({"time": time.perf_counter() - t0, "misfit": f0, "band": b})
+        d = None
+        if trial == A.TRIAL_GIVEN:
             d = direction(raw, ill, m, g)
-            # the gradient's depth mask (rows izm < i < nz-1 only) applies to
-            # any trial direction: the shallow zone stays as the plain FWI keeps it
-            izm = int(np.float32(s.zmask) / np.float32(s.dh))
-            d = d.copy()
-            d[:, :izm + 1] = 0.0
-            d[:, s.nz - 1:] = 0.0
-            best = None
-            for k in multiples:
-                trial = m + np.float32(k * step) * d
-                fl = fwi.misfit(trial)
-                if best is None or fl < best[0]:
-                    best = (fl, trial)
-            if best is not None and np.isfinite(best[0]) and best[0] < pred:
-                new, taken = best[1], "learned"
-        m = new
-        err = float(np.sqrt(np.mean((velocity(m) - fwi.v_true) ** 2)))
-        hist.append({"iteration": it, "band": s.band(it), "misfit": f0,
-                     "time": time.perf_counter() - t0, "step": step, "rule": rule,
-                     "taken": taken, "model_rms_error": err})
+        m, info = fwi.update(m, g, f0, trial=trial, net=net, direction=d, multiples=multiples,
+                             raw=raw, ill=ill)
+        t = time.perf_counter() - t0
+        known = None
+        if info["taken"]:
+            known = info["trial_misfit"]
+        elif trial != A.TRIAL_NONE:
+            known = info["plain_misfit"]
+        elif info["rule"] == 1:
+            known = info["f"][1] if info["step"] == fwi.alphas[0] else info["f"][2]
+        elif info["rule"] == 2:
+            known = f0
+        if known is not None and np.isfinite(known):
+            traj.append({"time": t, "misfit": float(known), "band": b})
+        hist.append(dict(info, iteration=it, band=b, misfit=f0, time=t,
+                         taken="trial" if info["taken"] else "plain", model_rms_error=rms(m)))
         if log:
             log(hist[-1])
-    fwi.use_band(s.band(its - 1))
+    b = s.band(its - 1)
+    fwi.use_band(b)
     fend = fwi.misfit(m)
-    hist.append({"iteration": its, "band": s.band(its - 1), "misfit": fend,
-                 "time": time.perf_counter() - t0, "step": 0.0, "rule": -1, "taken": "end",
-                 "model_rms_error": float(np.sqrt(np.mean((velocity(m) - fwi.v_true) ** 2)))})
-    return hist, m
+    t = time.perf_counter() - t0
+    traj.append({"time": t, "misfit": fend, "band": b})
+    hist.append({"iteration": its, "band": b, "misfit": fend, "time": t, "taken": "end",
+                 "model_rms_error": rms(m)})
+    return hist, traj, m
 
 
 def _save_npz(path, **arrays):
@@ -335,9 +327,8 @@ def _seed_samples(setup, seed):
     for it in range(setup.iterations):
         fwi.use_band(setup.band(it))
         f0, g, raw, ill = fwi.gradient(m, it)
-        step, _, _, _ = fwi.plain_search(m, g, f0)
         inputs.append((raw, ill, m.copy(), g))
-        m = m - np.float32(step) * g
+        m, _ = fwi.update(m, g, f0)
     sx, sy, sg, ss = [], [], [], []
     for raw, ill, mk, gk in inputs:
         t = (m - mk).astype(np.float32)
@@ -427,15 +418,17 @@ def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, lo
 
 def holdout_cosines(params, x, y, g, idx):
     """Mean cosine between the accumulated update (y - g) and the learned
-    direction, and between it and the plain direction -g, over samples idx."""
+    direction net(x) - g, and between it and the plain direction -g, over
+    samples idx."""
     n, _, h, w = x.shape
-    d = Direction(h, w, params)
+    net = A.Net(1, h, w, 0)
+    net.set_params(params)
     cl, cp = [], []
     for i in idx:
         t = y[i] - g[i]
-        cl.append(cosine(d.net.forward(x[i][None])[0] - g[i], t))
+        cl.append(cosine(net.forward(x[i][None])[0] - g[i], t))
         cp.append(cosine(-g[i], t))
-    d.close()
+    net.close()
     return float(np.mean(cl)), float(np.mean(cp))
 
 
@@ -444,15 +437,33 @@ def cosine(a, b):
     return float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
 
 
-def time_to_misfit(hist, target, band):
-    """Wall time at which a run's misfit in `band` first reaches `target`
-    (misfits of different bands measure different data and are not
-    compared), or None.  The misfit of an iteration's starting model counts
-    at the end of that iteration (its gradient pass measures it)."""
-    for h in hist:
-        if h["band"] == band and h["misfit"] <= target:
-            return h["time"]
+def time_to_misfit(traj, target, band):
+    """Wall time at which a run's misfit trajectory (run_fwi) first reaches
+    `target` in `band` (misfits of different bands measure different data
+    and are not compared), or None."""
+    for e in traj:
+        if e["band"] == band and e["misfit"] <= target:
+            return e["time"]
     return None
+
+
+def summarise(runs, trajs):
+    """Per run: final misfit (last band), model error, wall time, and the time
+    at which its trajectory first reaches plain FWI's final misfit, also as a
+    fraction of plain FWI's wall time."""
+    ref = runs.get("plain")
+    target = ref[-1]["misfit"] if ref else None
+    band = ref[-1]["band"] if ref else None
+    t_plain = ref[-1]["time"] if ref else None
+    out = {}
+    for name, h in runs.items():
+        t = time_to_misfit(trajs[name], target, band) if ref else None
+        out[name] = {"iterations": len(h) - 1, "final_misfit": h[-1]["misfit"],
+                     "final_model_rms_error": h[-1]["model_rms_error"],
+                     "time_s": h[-1]["time"], "time_to_plain_final_misfit_s": t,
+                     "time_ratio": (t / t_plain) if (t is not None and t_plain) else None,
+                     "trial_steps_taken": sum(e.get("taken") == "trial" for e in h)}
+    return {"plain_final_misfit": target, "plain_time_s": t_plain, "runs": out}
 
 
 def main(argv=None):
@@ -460,20 +471,24 @@ def main(argv=None):
     p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     p.add_argument("--out", default="learned_run")
     p.add_argument("--train-models", type=int, default=48)
-    p.add_argument("--train-size", choices=("small", "mid"), default="small",
-                   help="training-sample survey: 128 x 384 (16 shots) or 200 x 800 (64 shots)")
+    p.add_argument("--train-size", choices=("small", "mid", "full"), default="small",
+                   help="training survey: 128 x 384 (16 shots), 200 x 800 (64 shots) or the "
+                        "stated 256 x 768 (96 shots)")
+    p.add_argument("--train-first-seed", type=int, default=1)
     p.add_argument("--epochs", type=int, default=40)
     p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--compare-seed", type=int, default=10005)
     p.add_argument("--params", default="", help="trained parameters (skips generation and training)")
     p.add_argument("--size", choices=("small", "full"), default="small",
                    help="comparison at the reduced size or at the full config-5 size")
-    p.add_argument("--runs", default="plain,control,learned",
-                   help="comma-separated runs of the comparison (plain, control, learned); "
-                        "empty: train only")
-    p.add_argument("--bands", default="", help="iterations per band of the comparison runs")
+    p.add_argument("--runs", default="plain,plain_short,control,learned",
+                   help="comma-separated runs of the comparison (plain, plain_short, control, "
+                        "learned); empty: train only")
+    p.add_argument("--bands", default="", help="iterations per band of the plain run "
+                                               "(default: the setup's, 7,7,6)")
     p.add_argument("--short-bands", default="",
-                   help="iterations per band of extra, shorter control and learned runs")
+                   help="iterations per band of the plain_short, control and learned runs "
+                        "(default: --bands)")
     p.add_argument("--checkpoint", default="",
                    help="folder for resumable sample generation (one file per training "
                         "model) and training (one file, rewritten every epoch)")
@@ -487,9 +502,10 @@ def main(argv=None):
         cos = (None, None)
     else:
         t0 = time.perf_counter()
-        ts = TRAIN_SMALL if a.train_size == "small" else TRAIN_MID
+        ts = {"small": TRAIN_SMALL, "mid": TRAIN_MID, "full": TRAIN_FULL}[a.train_size]
         ck = a.checkpoint
-        x, y, g, scales = make_samples(ts, range(1, 1 + a.train_models), log,
+        seeds = range(a.train_first_seed, a.train_first_seed + a.train_models)
+        x, y, g, scales = make_samples(ts, seeds, log,
                                        cache=os.path.join(ck, f"samples_{a.train_size}")
                                        if ck else None)
         t_gen = time.perf_counter() - t0
@@ -501,50 +517,42 @@ def main(argv=None):
         cos = holdout_cosines(params, x, y, g, val)
         log({"holdout_cosine_learned": cos[0], "holdout_cosine_plain": cos[1]})
     params.tofile(os.path.join(a.out, "preconditioner_params.bin"))
+    train_report = {"samples": int(len(x)), "generation_s": t_gen, "training_s": t_train,
+                    "holdout_cosine": {"learned": cos[0], "plain_direction": cos[1]},
+                    "train_loss_first_last": [th[0][1], th[-1][1]],
+                    "holdout_loss_first_last": [th[0][2], th[-1][2]]}
     if not a.runs:                    # training only
-        log({"samples": int(len(x)), "generation_s": t_gen, "training_s": t_train,
-             "holdout_cosine": {"learned": cos[0], "plain_direction": cos[1]}})
+        with open(os.path.join(a.out, "report.json"), "w") as f:
+            json.dump(train_report, f, indent=1)
+        log(train_report)
         return
 
     s = COMPARE_SMALL if a.size == "small" else COMPARE_FULL
-    if a.bands:
-        s = dataclasses.replace(s, band_iters=tuple(int(k) for k in a.bands.split(",")))
+    long_bands = tuple(int(k) for k in a.bands.split(",")) if a.bands else s.band_iters
+    short_bands = tuple(int(k) for k in a.short_bands.split(",")) if a.short_bands else long_bands
     v, v0 = C.config5_models(a.compare_seed, s.nz, s.nx)
     t0 = time.perf_counter()
     fwi = Fwi(s, v, v0)
     log({"setup_s": time.perf_counter() - t0, "grid": [s.nz, s.nx], "shots": s.nsrc,
-         "band_iters": list(s.band_iters)})
-    d = Direction(s.nx, s.nz, params)
-    arms = {"plain": None, "control": plain_direction, "learned": d}
-    runs = {}
+         "bands": list(long_bands), "short_bands": list(short_bands)})
+    net = network(s.nx, s.nz, params)
+    runs, trajs = {}, {}
     for name in a.runs.split(","):
-        runs[name] = run_fwi(fwi, arms[name], log=lambda e, n=name: log(dict(e, run=n)))[0]
-    if a.short_bands:
-        bi = tuple(int(k) for k in a.short_bands.split(","))
-        runs["control_short"] = run_fwi(fwi, plain_direction, band_iters=bi, log=log)[0]
-        runs["learned_short"] = run_fwi(fwi, d, band_iters=bi, log=log)[0]
-    d.close()
+        if name not in RUNS:
+            raise SystemExit(f"unknown run '{name}' (one of {', '.join(RUNS)})")
+        bands = long_bands if name == "plain" else short_bands
+        h, tr, _ = run_fwi(fwi, RUNS[name], net=net, band_iters=bands,
+                           log=lambda e, n=name: log(dict(e, run=n)))
+        runs[name], trajs[name] = h, tr
+    net.close()
     fwi.close()
-    ref = runs.get("plain")
-    target = ref[-1]["misfit"] if ref else None
-    t_plain = ref[-1]["time"] if ref else None
-    summary = {}
-    for name, h in runs.items():
-        t = time_to_misfit(h, target, ref[-1]["band"]) if ref else None
-        summary[name] = {"final_misfit": h[-1]["misfit"], "time_s": h[-1]["time"],
-                         "final_model_rms_error": h[-1]["model_rms_error"],
-                         "time_to_plain_final_misfit_s": t,
-                         "time_ratio": (t / t_plain) if (t and t_plain) else None,
-                         "trial_steps_taken": sum(e["taken"] == "learned" for e in h)}
-    report = {"samples": int(len(x)), "generation_s": t_gen, "training_s": t_train,
-              "holdout_cosine": {"learned": cos[0], "plain_direction": cos[1]},
-              "train_loss_first_last": [th[0][1], th[-1][1]],
-              "holdout_loss_first_last": [th[0][2], th[-1][2]],
-              "plain_final_misfit": target, "plain_time_s": t_plain,
-              "summary": summary, "runs": runs}
+    summary = summarise(runs, trajs)
+    report = dict(train_report, compare_seed=a.compare_seed, grid=[s.nz, s.nx], shots=s.nsrc,
+                  bands=list(long_bands), short_bands=list(short_bands), summary=summary,
+                  runs=runs, trajectories=trajs)
     with open(os.path.join(a.out, "report.json"), "w") as f:
         json.dump(report, f, indent=1)
-    log({k: v for k, v in report.items() if k != "runs"})
+    log({k: v for k, v in report.items() if k not in ("runs", "trajectories")})
 
 
 if __name__ == "__main__":

@@ -181,6 +181,13 @@ int fwi_rb_velocity(fwi_ctx *ctx, int32_t shot, uint32_t iteration, float *vp_pa
 int fwi_rb_velocity_noalloc(fwi_ctx *ctx, int32_t shot, uint32_t iteration,
                             float *vp_padded_out);
 
+/* The configuration the context was created with. */
+int fwi_get_config(fwi_ctx *ctx, fwi_config *out);
+
+/* Source and receiver counts of the current geometry (0 before
+ * fwi_set_geometry). */
+int fwi_geometry_size(fwi_ctx *ctx, int32_t *nsrc, int32_t *nrec);
+
 /* Padded grid of a boundary kind: nzz, nxx, nb. */
 int fwi_grid_shape(fwi_ctx *ctx, int32_t boundary, int32_t *nzz, int32_t *nxx, int32_t *nb);
 

@@ -31,6 +31,8 @@ typedef struct fwi_nn fwi_nn;
  * He-uniform weights (zero biases). */
 int fwi_nn_create(int32_t batch, int32_t height, int32_t width, uint64_t seed, fwi_nn **out);
 int fwi_nn_destroy(fwi_nn *nn);
+/* The batch and image size the network was created for. */
+int fwi_nn_get_shape(fwi_nn *nn, int32_t *batch, int32_t *height, int32_t *width);
 int fwi_nn_num_params(fwi_nn *nn, int64_t *count);
 int fwi_nn_get_params(fwi_nn *nn, float *params);
 int fwi_nn_set_params(fwi_nn *nn, const float *params);

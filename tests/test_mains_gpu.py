@@ -230,3 +230,42 @@ def test_autotuned_executables_write_the_same_files(tmp_path):
     a = np.fromfile(os.path.join(folders["seismic"], name), np.float32)
     b = np.fromfile(os.path.join(f2["seismic"], name), np.float32)
     assert a.size == SURVEY.nz * SURVEY.nx and np.abs(a).max() > 0 and np.array_equal(a, b)
+
+
+def test_inversion_trial_directions(tmp_path):
+    """inv_trial_direction = plain (the control, d = -g) and = network with a
+    network of zero weights (d = net(x) - g = -g) run the same iterations and
+    write the same files, bit for bit; both log the trial's misfit against
+    the plain step's and never keep a trial whose misfit is not below it."""
+    tmp = str(tmp_path)
+    _, _, _, _, folders, prefix, params = setup_files(tmp)
+    run("modeling", params["mod"])
+    init_file = os.path.join(tmp, "init.bin")
+    sps, rps = os.path.join(tmp, "sps.txt"), os.path.join(tmp, "rps.txt")
+    net = F.abi.Net(1, 204, 84, 0)           # 201 x 81 edge-padded to multiples of 4
+    zeros = os.path.join(tmp, "zero_network.bin")
+    np.zeros(net.nparams, np.float32).tofile(zeros)
+    net.close()
+    outs = {}
+    for name, extra in (("plain", {"inv_trial_direction": "plain"}),
+                        ("network", {"inv_trial_direction": "network",
+                                     "inv_trial_network": zeros,
+                                     "inv_trial_multiples": "2,4"})):
+        f = C.ensure_dirs(os.path.join(tmp, name))
+        f["data"] = folders["data"]
+        p = os.path.join(tmp, f"parameters_{name}.txt")
+        C.write_parameters(p, SURVEY, init_file, sps, rps, f, max_iteration=3,
+                           depth_mask=100.0, inv_prefix=prefix, extra=extra)
+        out = run("inversion", p)
+        trials = [l for l in out.splitlines() if l.startswith("trial direction")]
+        assert len(trials) >= 1
+        for line in trials:
+            words = line.split()
+            fl, fp = float(words[3]), float(words[9])
+            assert ("trial taken" in line) == (fl < fp), line
+        suffix = f"{int(FMAX)}Hz_{SURVEY.nz}x{SURVEY.nx}_{int(SURVEY.dh)}m.bin"
+        outs[name] = (np.fromfile(os.path.join(f["models"], f"final_model_FWI_{suffix}"),
+                                  np.float32),
+                      open(os.path.join(f["residuo"], "convergence_3_iterations.txt")).read())
+    assert np.array_equal(outs["plain"][0], outs["network"][0])
+    assert outs["plain"][1] == outs["network"][1]
