@@ -121,6 +121,9 @@ def slowness2(v):
     return (1.0 / (v * v)).astype(np.float32)
 
 
+_TUNED = {}        # launch shapes per setup (autotuned once per process)
+
+
 class Fwi:
     """One survey on one engine: observed data per band, the gradient pass,
     the misfit of a model and the C++ model update."""
@@ -133,14 +136,23 @@ class Fwi:
         self.eng.set_geometry(*s.acquisition())
         self.eng.set_model(v_true)
         if autotune:
-            w, tlag = A.ricker_wavelet(s.nt, s.dt, 3.0 * s.peaks[-1])
-            self.eng.set_wavelet(w, tlag)
-            self.eng.autotune(s.nsrc, reps=4)
-        self.bands = []
-        for peak in s.peaks:
-            w, tlag = A.ricker_wavelet(s.nt, s.dt, 3.0 * peak)
-            self.eng.set_wavelet(w, tlag)
-            self.bands.append((w, tlag, self.eng.modeling(A.ABC)))
+            # launch shapes depend on the grid and the batch only: the first
+            # survey of a setup times them, later ones reuse them
+            key = _setup_tag(s)
+            if key not in _TUNED:
+                w, tlag = A.ricker_wavelet(s.nt, s.dt, 3.0 * s.peaks[-1])
+                self.eng.set_wavelet(w, tlag)
+                self.eng.autotune(s.nsrc, reps=4)
+                _TUNED[key] = [self.eng.get_tuning(k)[:3] for k in range(3)]
+            for k, (st, vz, xw) in enumerate(_TUNED[key]):
+                self.eng.set_tuning(vz, xw, kernel=k, steps=st)
+        # observed data of every band the schedule uses, modeled from the true
+        # model before any run starts (a band a run adds later is modeled
+        # when it is first used)
+        self.bands = [None] * len(s.peaks)
+        for b, n in enumerate(s.band_iters):
+            if n > 0:
+                self._model_band(b)
         self.v_true = np.asarray(v_true, np.float32)
         self.m0 = slowness2(v_init)
         # line-search trial steps of the reference (inversion.cu:288-301)
@@ -153,7 +165,18 @@ class Fwi:
     def close(self):
         self.eng.close()
 
+    def _model_band(self, b):
+        """Wavelet and observed data of band b (the engine holds the true
+        model while this runs)."""
+        w, tlag = A.ricker_wavelet(self.setup.nt, self.setup.dt, 3.0 * self.setup.peaks[b])
+        self.eng.set_wavelet(w, tlag)
+        self.bands[b] = (w, tlag, self.eng.modeling(A.ABC))
+        self.current = None
+
     def use_band(self, b):
+        if self.bands[b] is None:
+            self.eng.set_model(self.v_true)
+            self._model_band(b)
         if b != self.current:
             w, tlag, obs = self.bands[b]
             self.eng.set_wavelet(w, tlag)

@@ -13,7 +13,13 @@
   differences of the misfit along localized model perturbations agree with
   <raw gradient, dm> in sign and up to one common factor (as the small-size
   test in test_gpu_parity.py).
+* Config 4's training workload at its stated shape (samples of plain FWI at
+  256 x 768 with 96 shots, Adam steps at batch 16) and config 5's comparison
+  workload at its full size (one iteration each of plain FWI, the control,
+  the learned run and a zero network), with the acceptance rule's
+  never-worse property checked bit for bit.
 """
+import dataclasses
 import os
 import subprocess
 import time
@@ -215,3 +221,75 @@ def test_config3_full_workload(tmp_path):
     assert np.all(np.sign(r) == np.sign(r[0]))
     assert np.abs(r).max() / np.abs(r).min() <= 1.5
     eng.close()
+
+
+def test_config4_shape_samples_and_training():
+    """Config 4's training workload at its stated shape: plain-FWI samples of
+    two seeded training models at 256 x 768 (96 shots, 8 iterations over the
+    4 / 8 / 12 Hz bands), then Adam steps of the network at batch 16 on those
+    768 x 256 samples, which lower the training loss."""
+    from fwi2d_amd import abi as A
+    from fwi2d_amd import learned as L
+    s = L.TRAIN_FULL
+    assert (s.nz, s.nx, s.nsrc, s.iterations) == (256, 768, 96, 8)
+    t0 = time.perf_counter()
+    x, y, g, scales = L.make_samples(s, [1, 2])
+    print(f"{len(x)} samples from 2 models in {time.perf_counter() - t0:.1f} s", flush=True)
+    assert x.shape[1:] == (3, 768, 256) and y.shape[1:] == (768, 256)
+    assert len(x) >= 12 and np.isfinite(x).all() and np.isfinite(y).all()
+    assert np.all(scales > 0)
+    idx = np.arange(16) % len(x)
+    net = A.Net(16, 768, 256, 0)
+    t0 = time.perf_counter()
+    losses = [net.train_step(x[idx], y[idx], 1e-3) for _ in range(6)]
+    print(f"losses {['%.4e' % v for v in losses]} ({time.perf_counter() - t0:.2f} s)", flush=True)
+    net.close()
+    assert losses[-1] < losses[0]
+
+
+def test_config5_shape_plain_control_learned():
+    """Config 5's comparison workload at its full size (401 x 1601 at 10 m,
+    200-cell random pad, 256 shots, 1601 receivers, nt 5001 at 0.6 ms), one
+    iteration in the 4 Hz band of each run from the same starting model:
+    * plain FWI, the control (acceptance rule along -g), the learned run
+      (network trained at 200 x 800) and the learned run with a zero network;
+    * the zero network reproduces the control bit for bit;
+    * the trial runs are never worse than the plain step: the plain update
+      they measured is plain FWI's own update (its misfit equals plain FWI's
+      next misfit, bit for bit), and the model they keep has a misfit no
+      larger than that."""
+    from fwi2d_amd import abi as A
+    from fwi2d_amd import learned as L
+    s = dataclasses.replace(L.COMPARE_FULL, band_iters=(1, 0, 0))
+    v, v0 = C.config5_models(10005, s.nz, s.nx)
+    t0 = time.perf_counter()
+    fwi = L.Fwi(s, v, v0)
+    print(f"setup {time.perf_counter() - t0:.1f} s", flush=True)
+    params = np.fromfile(os.path.join(ROOT, "tests", "golden", "preconditioner_mid96.bin"),
+                         np.float32)
+    net = L.network(s.nx, s.nz, params)
+    zero = L.network(s.nx, s.nz)
+    zero.set_params(np.zeros(zero.nparams, np.float32))
+    runs = {}
+    for name, trial, n in (("plain", A.TRIAL_NONE, None), ("control", A.TRIAL_PLAIN, None),
+                           ("learned", A.TRIAL_NETWORK, net),
+                           ("zero_network", A.TRIAL_NETWORK, zero)):
+        h, _, m = L.run_fwi(fwi, trial, net=n)
+        runs[name] = (h, m)
+        e = h[0]
+        print(f"{name:12s} f0 {e['misfit']:.6e} step {e['step']:.4e} rule {e['rule']} "
+              f"plain {e.get('plain_misfit', float('nan')):.6e} "
+              f"trial {e.get('trial_misfit', float('inf')):.6e} taken {e['taken']} -> "
+              f"{h[-1]['misfit']:.6e} ({h[-1]['time']:.1f} s)", flush=True)
+    net.close()
+    zero.close()
+    fwi.close()
+    hp = runs["plain"][0]
+    for name in ("control", "learned", "zero_network"):
+        h = runs[name][0]
+        assert h[0]["misfit"] == hp[0]["misfit"]
+        assert h[0]["plain_misfit"] == hp[-1]["misfit"], name
+        assert h[-1]["misfit"] <= hp[-1]["misfit"], name
+    assert [(e["misfit"], e["taken"]) for e in runs["zero_network"][0]] == \
+        [(e["misfit"], e["taken"]) for e in runs["control"][0]]
+    assert np.array_equal(runs["zero_network"][1], runs["control"][1])
