@@ -4,6 +4,7 @@ project's own kernels on plain-FWI samples.
     python tests/golden/make_preconditioner.py small   (GPU, ~4 minutes)
     python tests/golden/make_preconditioner.py mid     (GPU, ~8 minutes)
     python tests/golden/make_preconditioner.py mid96   (GPU, ~15 minutes)
+    python tests/golden/make_preconditioner.py config4 (GPU, ~16 minutes)
 
 small -> preconditioner_48models.bin: 48 seeded training models of the
 config-4 family at 128 x 384 (10 m, 16 shots), 8 plain FWI iterations each
@@ -21,6 +22,13 @@ cells to the full-size ones.
 mid96 -> preconditioner_mid96.bin: the same at 200 x 800 with 96 training
 models (seeds 1-96, 768 samples; `python -m fwi2d_amd.learned --train-size
 mid --train-models 96 --runs ""`).
+
+config4 -> preconditioner_config4.bin: the training configuration at its
+stated grid and acquisition (256 x 768 at 10 m, 96 shots, 768 receivers,
+8 plain FWI iterations per model over 4/8/12 Hz; fwi2d_amd.learned.TRAIN_FULL)
+on 64 of its 256 models (seeds 1-64: what one 20-minute GPU session
+generates), 40 epochs of Adam at lr 1e-3, batch 16, mirrored samples
+(`python -m fwi2d_amd.learned --train-size full --train-models 64 --runs ""`).
 """
 import os
 import sys
@@ -35,7 +43,8 @@ def main(size):
     setup, flip, name, models = {
         "small": (L.TRAIN_SMALL, False, "preconditioner_48models.bin", 48),
         "mid": (L.TRAIN_MID, True, "preconditioner_mid48.bin", 48),
-        "mid96": (L.TRAIN_MID, True, "preconditioner_mid96.bin", 96)}[size]
+        "mid96": (L.TRAIN_MID, True, "preconditioner_mid96.bin", 96),
+        "config4": (L.TRAIN_FULL, True, "preconditioner_config4.bin", 64)}[size]
     x, y, g, _ = L.make_samples(setup, range(1, models + 1), log=print)
     params, hist, val = L.train(x, y, epochs=40, lr=1e-3, flip=flip, log=print)
     print("held-out cosine (learned, plain):", L.holdout_cosines(params, x, y, g, val))

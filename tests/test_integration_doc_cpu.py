@@ -1,6 +1,6 @@
-"""The integration guide's C++ examples compile against include/fwi2d.h
-(g++ -fsyntax-only), so the documented calls keep the header's signatures,
-and its ctypes stub matches the Python binding."""
+"""The integration guide's C++ examples compile against include/fwi2d.h and
+include/fwi2d_update.h (g++ -fsyntax-only), so the documented calls keep the
+headers' signatures, and its ctypes stub matches the Python binding."""
 import os
 import re
 import subprocess
@@ -11,6 +11,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # reference classes (Modeling, Migration, Inversion members).
 PRELUDE = r"""
 #include "fwi2d.h"
+#include "fwi2d_update.h"
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <vector>
@@ -19,7 +21,7 @@ void example(int nz, int nx, float dh, int nt, float dt, int nb, float abc_facto
              Geometry *geometry, int nsrc, const float *xsrc, const float *zsrc, int nrec,
              const float *xrec, const float *zrec, const float *Vp_nx_nz, const float *data,
              float *image_nx_nz, uint32_t iteration, float depth_mask, float *g_nx_nz,
-             const double *alpha, const double *f, const float *m_nx_nz, double m_min)
+             const double *alpha, const double *f, float *m_nx_nz, double m_min, double dm)
 {
 """
 
@@ -28,7 +30,8 @@ def test_integration_cpp_examples_compile(tmp_path):
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     blocks = re.findall(r"```cpp\n(.*?)```", text, re.S)
     assert len(blocks) >= 3
-    body = "\n".join(b.replace('#include "fwi2d.h"', "") for b in blocks)
+    body = "\n".join(b.replace('#include "fwi2d.h"', "").replace('#include "fwi2d_update.h"', "")
+                     for b in blocks)
     src = tmp_path / "example.cpp"
     src.write_text(PRELUDE + body + "\n}\n")
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wno-unused-variable",
