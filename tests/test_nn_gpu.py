@@ -36,10 +36,11 @@ def test_forward_matches_oracle():
     yo, _ = no.forward(p.astype(np.float64), x.astype(np.float64), rnd=True)
     err = rel_l2(y, yo)
     print(f"forward relL2 (bf16 emulated) {err:.3e}")
-    assert err < 1e-2
+    assert err < 1e-6
     yo64, _ = no.forward(p.astype(np.float64), x.astype(np.float64), rnd=False)
+    # bf16 activations against an FP64 network: a few units of bf16's 2^-8
     print(f"forward relL2 (exact FP64) {rel_l2(y, yo64):.3e}")
-    assert rel_l2(y, yo64) < 5e-2
+    assert rel_l2(y, yo64) < 2e-2
 
 
 def test_gradient_matches_oracle():
@@ -49,15 +50,55 @@ def test_gradient_matches_oracle():
     loss, g = net.gradient(x, t)
     lo, go, _ = no.loss_and_grad(p.astype(np.float64), x.astype(np.float64),
                                  t.astype(np.float64), rnd=True)
-    assert abs(loss - lo) <= 1e-2 * lo
+    print(f"loss {loss:.8e} oracle {lo:.8e}")
+    assert abs(loss - lo) <= 1e-5 * lo
     # per layer relative error of the weight and bias gradients
     o = 0
     for k, (cin, cout) in enumerate(no.ARCH):
         n = cout * 9 * cin + cout
         e = rel_l2(g[o:o + n], go[o:o + n])
         print(f"layer {k + 1} gradient relL2 {e:.3e}")
-        assert e < 3e-2, k
+        assert e < 1e-3, k
         o += n
+
+
+def test_gradient_matches_finite_differences_per_layer():
+    """The GPU gradient against central differences of the GPU loss, one
+    direction per layer: u = the layer's gradient, normalised; the difference
+    (L(p + eps u) - L(p - eps u)) / 2 eps must equal <g, u> within 5 %.  The
+    loss of a bf16 network moves in bf16 steps, so eps is 0.3 % of the layer's
+    weight norm, where the difference stands above that rounding (the
+    bf16-emulated oracle gives ratios within 2.8 % of 1 for these parameters
+    and inputs)."""
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((2, 3, 16, 24)).astype(np.float32)
+    t = (0.1 * rng.standard_normal((2, 16, 24))).astype(np.float32)
+    parts = []
+    for cin, cout in no.ARCH:                     # He-uniform weights, zero biases
+        lim = np.sqrt(6.0 / (9 * cin))
+        parts += [rng.uniform(-lim, lim, cout * 9 * cin), np.zeros(cout)]
+    p = np.concatenate(parts).astype(np.float32).astype(np.float64)
+    net = F.Net(2, 16, 24, seed=0)
+    net.set_params(p.astype(np.float32))
+    _, g = net.gradient(x, t)
+    g = g.astype(np.float64)
+    o = 0
+    for k, (cin, cout) in enumerate(no.ARCH):
+        n = cout * 9 * cin + cout
+        u = np.zeros_like(p)
+        u[o:o + n] = g[o:o + n]
+        u /= np.linalg.norm(u)
+        eps = 3e-3 * np.linalg.norm(p[o:o + n - cout])
+        net.set_params((p + eps * u).astype(np.float32))
+        lp = net.gradient(x, t)[0]
+        net.set_params((p - eps * u).astype(np.float32))
+        lm = net.gradient(x, t)[0]
+        fd = (lp - lm) / (2 * eps)
+        gd = float(g @ u)
+        print(f"layer {k + 1}: finite difference {fd:.5e}, <g, u> {gd:.5e}, ratio {fd / gd:.4f}")
+        assert abs(fd / gd - 1.0) <= 0.05, k
+        o += n
+    net.close()
 
 
 def test_adam_step_closed_form():
