@@ -515,9 +515,22 @@ def main(argv=None):
     p.add_argument("--checkpoint", default="",
                    help="folder for resumable sample generation (one file per training "
                         "model) and training (one file, rewritten every epoch)")
+    p.add_argument("--merge", nargs="+", default=None, metavar="REPORT",
+                   help="summarise the runs of several report.json files of the same "
+                        "comparison (runs made in separate sessions) and exit")
     a = p.parse_args(argv)
-    os.makedirs(a.out, exist_ok=True)
     log = lambda d: print(json.dumps(d), flush=True)
+    if a.merge:
+        runs, trajs = {}, {}
+        for path in a.merge:
+            with open(path) as f:
+                r = json.load(f)
+            runs.update(r["runs"])
+            trajs.update(r["trajectories"])
+        summary = summarise(runs, trajs)
+        log(summary)
+        return summary
+    os.makedirs(a.out, exist_ok=True)
 
     if a.params:
         params = np.fromfile(a.params, np.float32)

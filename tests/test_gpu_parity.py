@@ -557,3 +557,64 @@ def test_full_size_invariants():
     print(f"seismogram linearity relL2 {e_seis:.3e} (equal: {np.array_equal(obs2, 2.0 * obs[:8])})")
     assert e_seis <= SEIS_TOL
     eng.close()
+
+
+def test_one_fwi_iteration_matches_oracle():
+    """One whole plain FWI iteration (inversion.cu:111-358) against the FP64
+    oracle on the overthrust model, 2 shots: the gradient pass and its
+    post-processing, then the C++ update (fwi_update_model: misfits of the two
+    line-search models m - alpha_k g, the parabola and the step rule, and
+    m - step g).  The oracle models the same two trial models from its own
+    direction, and its parabola vertex (or_parabola_step) gives the step; the
+    test requires the same rule, the step within 1e-3 and the model update
+    m_new - m within 1e-3 (relative L2)."""
+    s = overthrust_survey(nsrc=2, abc_nb=51, abc_factor=0.004, rbc_nb=41, varVp=1000.0,
+                          seed=21, max_batch=2)
+    from fwi2d_amd import configs as C
+    true_v = s.v.copy()
+    obs = np.stack([s.oracle_forward(k, F.ABC)[0] for k in range(2)]).astype(np.float32)
+    init = C.smooth_slowness(true_v, 6.0)
+    s.v = init
+    s.eng.set_model(init)
+    s.eng.set_observed(obs)
+    s.eng.reset()
+    zmask = 100.0
+    f0 = float(np.sqrt(s.eng.gradient(iteration=1)))
+    g, _, _, _ = s.eng.get_gradient(zmask)
+    m = (1.0 / init.astype(np.float64) ** 2).astype(np.float32)
+    dm = 1.0 / float(init.min()) ** 2 - 1.0 / float(init.max()) ** 2
+    a1, a2 = 0.02 * dm, 0.05 * dm
+    vstable = 0.999 * 0.5546 * s.dh / s.dt
+    m_new, info = s.eng.update_model(m, g, f0, a1, a2, 1.0 / vstable ** 2)
+
+    rg, _, _, rsum = gradient_oracle(s, obs, init, 1, zmask)
+    rf = [np.sqrt(rsum)]
+    m64 = 1.0 / init.astype(np.float64) ** 2
+    for a in (a1, a2):
+        v = 1.0 / np.sqrt(m64 - a * rg)
+        rf.append(np.sqrt(sum(float(np.sum((obs[k].astype(np.float64) -
+                                            s.oracle_forward(k, F.ABC, model=v)[0]) ** 2))
+                              for k in range(2))))
+    alpha = np.array([0.0, a1, a2])
+    print(f"misfits GPU {info['f']} oracle {rf}; rule {info['rule']} step {info['step']:.6e}")
+    for k in range(3):
+        assert abs(info["f"][k] - rf[k]) <= 1e-5 * rf[k], k
+    curv = ((rf[2] - rf[0]) / a2 - (rf[1] - rf[0]) / a1) / (a2 - a1)
+    vertex = ob.parabola_step(alpha, np.array(rf))
+    if curv > 0 and 0 < vertex <= 4 * a2:
+        ref_rule, ref_step = 0, vertex
+    else:
+        k = 2 if rf[2] < rf[1] else 1
+        ref_rule, ref_step = 1, alpha[k]
+    print(f"oracle rule {ref_rule} step {ref_step:.6e}")
+    assert info["rule"] == ref_rule
+    assert abs(info["step"] - ref_step) <= 1e-3 * ref_step
+    upd, rupd = m_new.astype(np.float64) - m, -ref_step * rg
+    e = rel_l2(upd, rupd)
+    print(f"model update relL2 {e:.3e}")
+    assert e <= 1e-3
+    # the engine holds the updated model on return
+    held = s.eng.misfit()
+    s.eng.set_model((1.0 / np.sqrt(m_new)).astype(np.float32))
+    assert s.eng.misfit() == held
+    s.eng.close()

@@ -60,3 +60,34 @@ def test_model_families_are_seeded():
     # the starting model is smoother than the true model
     tv = lambda a: np.abs(np.diff(a, axis=1)).sum()
     assert tv(s5) < tv(v5)
+
+
+def test_reports_of_separate_sessions_merge(tmp_path):
+    """Runs of one comparison made in separate sessions summarise together:
+    the target is plain FWI's final misfit in its last band, and each run's
+    time is the first point of its trajectory at or below it."""
+    import json
+    a = {"runs": {"plain": [{"misfit": 9.0, "band": 1, "time": 10.0, "taken": "plain",
+                             "model_rms_error": 5.0},
+                            {"misfit": 4.0, "band": 1, "time": 20.0, "taken": "end",
+                             "model_rms_error": 4.0}]},
+         "trajectories": {"plain": [{"time": 10.0, "misfit": 9.0, "band": 1},
+                                    {"time": 20.0, "misfit": 4.0, "band": 1}]}}
+    b = {"runs": {"learned": [{"misfit": 9.0, "band": 1, "time": 4.0, "taken": "trial",
+                               "model_rms_error": 4.5},
+                              {"misfit": 3.5, "band": 1, "time": 9.0, "taken": "end",
+                               "model_rms_error": 3.0}]},
+         "trajectories": {"learned": [{"time": 3.0, "misfit": 9.0, "band": 1},
+                                      {"time": 4.0, "misfit": 3.9, "band": 1},
+                                      {"time": 9.0, "misfit": 3.5, "band": 1}]}}
+    paths = []
+    for name, r in (("a", a), ("b", b)):
+        p = tmp_path / f"{name}.json"
+        p.write_text(json.dumps(r))
+        paths.append(str(p))
+    s = L.main(["--merge"] + paths)
+    assert s["plain_final_misfit"] == 4.0 and s["plain_time_s"] == 20.0
+    assert s["runs"]["learned"]["time_to_plain_final_misfit_s"] == 4.0
+    assert s["runs"]["learned"]["time_ratio"] == 0.2
+    assert s["runs"]["learned"]["trial_steps_taken"] == 1
+    assert s["runs"]["plain"]["time_ratio"] == 1.0
