@@ -27,7 +27,11 @@
 //    are staged in LDS once per launch (a second build of each kernel reads
 //    them from global memory, for launches whose windows do not fit);
 //  * blocks are renumbered so that each XCD receives a contiguous run of
-//    tiles: neighbouring chunks share their halo columns in that XCD's L2.
+//    tiles, and in the three-step kernels odd x-chunks march right to left:
+//    two neighbouring chunks then reach the columns they share at the same
+//    time, which keeps the second read of those halo columns in the caches
+//    (the stencil is symmetric and each sum is commutative, so the mirrored
+//    march gives the same bits).
 //
 // FP32 operation order: the Laplacian's nearest ring is summed as
 // differences from the centre cell (the farther rings as sums minus 4u) and
@@ -824,6 +828,12 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
     const bool own = active && (zb >= zs + 8) && (zb < zs + 64 * VZ - 8);
     const int x0 = 4 + id.xc * xw;
     const int x1 = min(x0 + xw, nxx - 4);
+    // odd chunks march right to left: neighbouring chunks then read their
+    // shared halo columns at the same time (column c of the march is PC(c))
+    // odd chunks march right to left (column c of the march is PC(c))
+    const bool back = (id.xc & 1) != 0;
+    const int mir = x0 + x1 - 1;
+    auto PC = [&](int c) { return back ? mir - c : c; };
 
     const long soff = (long)id.slot * g.fstride;
     const float *cur = a.cur + soff;
@@ -853,14 +863,14 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
     float q3[9][VZ];     // u_{t+2}
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-        ldcolc<VZ>(q1[m], cur, x0 - 12 + m, nxx, ldz, zl);
+        ldcolc<VZ>(q1[m], cur, PC(x0 - 12 + m), nxx, ldz, zl);
         vzero<VZ>(q2[m]);
         vzero<VZ>(q3[m]);
     }
     float on[VZ], cn[VZ], pn[VZ];
-    ldcolc<VZ>(on, old, x0 - 8, nxx, ldz, zl);
-    ldcolc<VZ>(cn, cc, x0 - 8, nxx, ldz, zl);
-    float4 h = ldhaloc<VZ>(cur, x0 - 8, nxx, ldz, zs, lane);
+    ldcolc<VZ>(on, old, PC(x0 - 8), nxx, ldz, zl);
+    ldcolc<VZ>(cn, cc, PC(x0 - 8), nxx, ldz, zl);
+    float4 h = ldhaloc<VZ>(cur, PC(x0 - 8), nxx, ldz, zs, lane);
     __shared__ float cring[9][64 * VZ];              // c of the last nine columns
     const int jend = x1 + 8;
     int j = x0 - 8;
@@ -876,20 +886,21 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
         vstore<VZ>(&cring[C8][lane * VZ], cj);
-        ldcolc<VZ>(pn, cur, j + 5, nxx, ldz, zl);
-        ldcolc<VZ>(on, old, j + 1, nxx, ldz, zl);
-        ldcolc<VZ>(cn, cc, j + 1, nxx, ldz, zl);
+        ldcolc<VZ>(pn, cur, PC(j + 5), nxx, ldz, zl);
+        ldcolc<VZ>(on, old, PC(j + 1), nxx, ldz, zl);
+        ldcolc<VZ>(cn, cc, PC(j + 1), nxx, ldz, zl);
         // ---- step 1: u_{t+1} at column j
         {
-            const bool jin = (j >= 4) && (j < nxx - 4);
+            const int pj = PC(j);
+            const bool jin = (pj >= 4) && (pj < nxx - 4);
             float dxj = 1.f;
             if constexpr (ABC) {
                 bool in;
-                dxj = staged ? st.dx[j - xa] : pad_factor(a.b1d, j, nxx, g.nb, in);
+                dxj = staged ? st.dx[pj - xa] : pad_factor(a.b1d, pj, nxx, g.nb, in);
             }
             float ext[VZ + 8];
             zext<VZ>(ext, q1[C4], h);
-            h = ldhaloc<VZ>(cur, j + 1, nxx, ldz, zs, lane);
+            h = ldhaloc<VZ>(cur, PC(j + 1), nxx, ldz, zs, lane);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
@@ -897,16 +908,18 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
                 float val = leapfrog<ABC>(q1[C4][r], o1[r], cj[r], lap8r<VZ, C0>(q1, ext, r), d);
                 q2[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o1[r];
             }
-            add_source<VZ>(q2[C8], j, six, zb, siz, w1);
+            add_source<VZ>(q2[C8], pj, six, zb, siz, w1);
         }
         // ---- step 2: u_{t+2} at column jj = j - 4 (columns before x0 - 4 are not used)
         {
-            const int jj = j - 4;
-            const bool jin = (jj >= 4) && (jj < nxx - 4);
+            const int jj = j - 4, pjj = PC(jj);
+            const bool jin = (pjj >= 4) && (pjj < nxx - 4);
             float dxj = 1.f;
             if constexpr (ABC) {
                 bool in;
-                dxj = staged ? st.dx[max(jj - xa, 0)] : pad_factor(a.b1d, jj, nxx, g.nb, in);
+                // columns outside the window (the march's first four) are not used
+                dxj = staged ? st.dx[min(max(pjj - xa, 0), x1 + 7 - xa)]
+                             : pad_factor(a.b1d, pjj, nxx, g.nb, in);
             }
             float ext[VZ + 8], c2v[VZ];
             zext_local<VZ>(ext, q2[C4]);
@@ -920,16 +933,17 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
                 float val = leapfrog<ABC>(q2[C4][r], o2, c2, lap8r<VZ, C0>(q2, ext, r), d);
                 q3[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o2;
             }
-            add_source<VZ>(q3[C8], jj, six, zb, siz, w2);
-            if (own && jj >= x0 && jj < x1) vstore<VZ>((out1 + (long)jj * ldz) + uzl, q3[C8]);
+            add_source<VZ>(q3[C8], pjj, six, zb, siz, w2);
+            if (own && jj >= x0 && jj < x1) vstore<VZ>((out1 + (long)pjj * ldz) + uzl, q3[C8]);
         }
         // ---- step 3: u_{t+3} at column jjj = j - 8 (an owned column)
         const int jjj = j - 8;
         if (jjj >= x0) {
+            const int pjjj = PC(jjj);
             float dxj = 1.f;
             if constexpr (ABC) {
                 bool in;
-                dxj = staged ? st.dx[jjj - xa] : pad_factor(a.b1d, jjj, nxx, g.nb, in);
+                dxj = staged ? st.dx[pjjj - xa] : pad_factor(a.b1d, pjjj, nxx, g.nb, in);
             }
             float c3[VZ];
             vload<VZ>(c3, &cring[C0][lane * VZ]);
@@ -944,16 +958,16 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
                 float val = leapfrog<ABC>(q3[C4][r], o3, c3[r], lap8r<VZ, C0>(q3, ext, r), d);
                 n3[r] = ((i >= 4) & (i < g.nzz - 4)) ? val : o3;
             }
-            add_source<VZ>(n3, jjj, six, zb, siz, w3);
+            add_source<VZ>(n3, pjjj, six, zb, siz, w3);
             if (own) {
-                vstore<VZ>((out2 + (long)jjj * ldz) + uzl, n3);
+                vstore<VZ>((out2 + (long)pjjj * ldz) + uzl, n3);
                 // the column's three samples: u_{t+1} (q2), u_{t+2} (q3), u_{t+3}
                 if (rec1 || rec2 || rec3) {
                     if constexpr (staged)
-                        sample3_staged<VZ, ABC>(st, a.nt, q2[C0], q3[C4], n3, dxj, zm1, jjj - xa, zb,
+                        sample3_staged<VZ, ABC>(st, a.nt, q2[C0], q3[C4], n3, dxj, zm1, pjjj - xa, zb,
                                                 seis, t + 1 - a.tlag, rec1, rec2, rec3);
                     else
-                        sample3_receivers<VZ, ABC>(a, q2[C0], q3[C4], n3, jjj, zb, seis,
+                        sample3_receivers<VZ, ABC>(a, q2[C0], q3[C4], n3, pjjj, zb, seis,
                                                    t + 1 - a.tlag, rec1, rec2, rec3);
                 }
             }
@@ -1281,6 +1295,10 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
     const bool own = active && (zb >= zs + 8) && (zb < zs + 64 * VZ - 8);
     const int x0 = 4 + id.xc * xw;
     const int x1 = min(x0 + xw, nxx - 4);
+    // odd chunks march right to left (column c of the march is PC(c))
+    const bool back = (id.xc & 1) != 0;
+    const int mir = x0 + x1 - 1;
+    auto PC = [&](int c) { return back ? mir - c : c; };
 
     const long soff = (long)id.slot * g.fstride;
     const float *scur = a.scur + soff;
@@ -1311,19 +1329,19 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
     float pa[9][VZ], pi[9][VZ];      // image / illumination partial sums
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-        ldcolc<VZ>(qs1[m], scur, x0 - 12 + m, nxx, ldz, zl);
-        ldcolc<VZ>(qr1[m], rcur, x0 - 12 + m, nxx, ldz, zl);
+        ldcolc<VZ>(qs1[m], scur, PC(x0 - 12 + m), nxx, ldz, zl);
+        ldcolc<VZ>(qr1[m], rcur, PC(x0 - 12 + m), nxx, ldz, zl);
         vzero<VZ>(qs2[m]); vzero<VZ>(qr2[m]);
         vzero<VZ>(qs3[m]); vzero<VZ>(qr3[m]);
         vzero<VZ>(pa[m]); vzero<VZ>(pi[m]);
     }
     float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ], an[VZ], in[VZ];
-    ldcolc<VZ>(osn, sold, x0 - 8, nxx, ldz, zl);
-    ldcolc<VZ>(orn, rold, x0 - 8, nxx, ldz, zl);
-    ldcolc<VZ>(cn, cc, x0 - 8, nxx, ldz, zl);
+    ldcolc<VZ>(osn, sold, PC(x0 - 8), nxx, ldz, zl);
+    ldcolc<VZ>(orn, rold, PC(x0 - 8), nxx, ldz, zl);
+    ldcolc<VZ>(cn, cc, PC(x0 - 8), nxx, ldz, zl);
     vzero<VZ>(an); vzero<VZ>(in);
-    float4 hs = ldhaloc<VZ>(scur, x0 - 8, nxx, ldz, zs, lane);
-    float4 hr = ldhaloc<VZ>(rcur, x0 - 8, nxx, ldz, zs, lane);
+    float4 hs = ldhaloc<VZ>(scur, PC(x0 - 8), nxx, ldz, zs, lane);
+    float4 hr = ldhaloc<VZ>(rcur, PC(x0 - 8), nxx, ldz, zs, lane);
     __shared__ float cring[9][64 * VZ];              // c of the last nine columns
     const int jend = x1 + 8;                         // window [x0 - 8, x1 + 8)
     int j = x0 - 8;
@@ -1343,23 +1361,24 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
             pa[C8][r] = an[r]; pi[C8][r] = in[r];
         }
         vstore<VZ>(&cring[C8][lane * VZ], cj);
-        ldcolc<VZ>(psn, scur, j + 5, nxx, ldz, zl);
-        ldcolc<VZ>(prn, rcur, j + 5, nxx, ldz, zl);
-        ldcolc<VZ>(osn, sold, j + 1, nxx, ldz, zl);
-        ldcolc<VZ>(orn, rold, j + 1, nxx, ldz, zl);
-        ldcolc<VZ>(cn, cc, j + 1, nxx, ldz, zl);
+        ldcolc<VZ>(psn, scur, PC(j + 5), nxx, ldz, zl);
+        ldcolc<VZ>(prn, rcur, PC(j + 5), nxx, ldz, zl);
+        ldcolc<VZ>(osn, sold, PC(j + 1), nxx, ldz, zl);
+        ldcolc<VZ>(orn, rold, PC(j + 1), nxx, ldz, zl);
+        ldcolc<VZ>(cn, cc, PC(j + 1), nxx, ldz, zl);
         if (j + 1 >= x0 && j + 1 < x1) {             // image / illumination of owned columns
-            vload<VZ>(an, (acc + (long)(j + 1) * ldz) + uzl);
-            vload<VZ>(in, (ill + (long)(j + 1) * ldz) + uzl);
+            vload<VZ>(an, (acc + (long)PC(j + 1) * ldz) + uzl);
+            vload<VZ>(in, (ill + (long)PC(j + 1) * ldz) + uzl);
         }
         // ---- step 1 at column j
-        const bool jin1 = (j >= 4) && (j < nxx - 4);
+        const int pj = PC(j);
+        const bool jin1 = (pj >= 4) && (pj < nxx - 4);
         {
             float es[VZ + 8], er[VZ + 8];
             zext<VZ>(es, qs1[C4], hs);
             zext<VZ>(er, qr1[C4], hr);
-            hs = ldhaloc<VZ>(scur, j + 1, nxx, ldz, zs, lane);
-            hr = ldhaloc<VZ>(rcur, j + 1, nxx, ldz, zs, lane);
+            hs = ldhaloc<VZ>(scur, PC(j + 1), nxx, ldz, zs, lane);
+            hr = ldhaloc<VZ>(rcur, PC(j + 1), nxx, ldz, zs, lane);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
@@ -1377,13 +1396,13 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj1 && jin1) {
-                if constexpr (staged) inject_staged<VZ, 0>(st, a.idh2, qr2[C8], j - xa, zb);
-                else inject_receivers<VZ>(a, qr2[C8], j, zb, dat, a.nt - 2 - k);
+                if constexpr (staged) inject_staged<VZ, 0>(st, a.idh2, qr2[C8], pj - xa, zb);
+                else inject_receivers<VZ>(a, qr2[C8], pj, zb, dat, a.nt - 2 - k);
             }
         }
         // ---- step 2 at column jj = j - 4 (columns before x0 - 4 are not used)
-        const int jj = j - 4;
-        const bool jin2 = (jj >= 4) && (jj < nxx - 4);
+        const int jj = j - 4, pjj = PC(jj);
+        const bool jin2 = (pjj >= 4) && (pjj < nxx - 4);
         {
             float es[VZ + 8], er[VZ + 8], c2v[VZ];
             zext_local<VZ>(es, qs2[C4]);
@@ -1409,17 +1428,18 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj2 && jin2 && jj >= x0 - 4) {
-                if constexpr (staged) inject_staged<VZ, 1>(st, a.idh2, qr3[C8], jj - xa, zb);
-                else inject_receivers<VZ>(a, qr3[C8], jj, zb, dat, a.nt - 3 - k);
+                if constexpr (staged) inject_staged<VZ, 1>(st, a.idh2, qr3[C8], pjj - xa, zb);
+                else inject_receivers<VZ>(a, qr3[C8], pjj, zb, dat, a.nt - 3 - k);
             }
             if (own && jj >= x0 && jj < x1) {
-                vstore<VZ>((sout1 + (long)jj * ldz) + uzl, qs3[C8]);
-                vstore<VZ>((rout1 + (long)jj * ldz) + uzl, qr3[C8]);
+                vstore<VZ>((sout1 + (long)pjj * ldz) + uzl, qs3[C8]);
+                vstore<VZ>((rout1 + (long)pjj * ldz) + uzl, qr3[C8]);
             }
         }
         // ---- step 3 at column jjj = j - 8 (an owned column)
         const int jjj = j - 8;
         if (jjj >= x0) {
+            const int pjjj = PC(jjj);
             float c3[VZ];
             vload<VZ>(c3, &cring[C0][lane * VZ]);
             float es[VZ + 8], er[VZ + 8];
@@ -1447,11 +1467,11 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 }
             }
             if (inj3) {
-                if constexpr (staged) inject_staged<VZ, 2>(st, a.idh2, nr3, jjj - xa, zb);
-                else inject_receivers<VZ>(a, nr3, jjj, zb, dat, a.nt - 4 - k);
+                if constexpr (staged) inject_staged<VZ, 2>(st, a.idh2, nr3, pjjj - xa, zb);
+                else inject_receivers<VZ>(a, nr3, pjjj, zb, dat, a.nt - 4 - k);
             }
             if (own) {
-                const long off = (long)jjj * ldz;
+                const long off = (long)pjjj * ldz;
                 vstore<VZ>((sout2 + off) + uzl, ns3);
                 vstore<VZ>((rout2 + off) + uzl, nr3);
                 vstore<VZ>((acc + off) + uzl, fa);

@@ -1,6 +1,7 @@
 """CPU model of the launch geometry and global addressing of the three-step
 step kernels (`fwd3_kernel`, `rev3_kernel` in fwi2d_amd/csrc/propagate.hip),
-with the engine's buffer layout (`setup_grid`, `kGuard` in engine.hip).
+with the engine's buffer layout (`setup_grid`, `kGuard` in engine.hip);
+odd x-chunks march right to left, their columns mirrored within the chunk.
 
 Checks, for every tile and lane of a launch:
 * the owned rows of the z segments and the owned columns of the x chunks
@@ -33,7 +34,8 @@ def zsegments(nzz, vz, steps):
 
 
 def tiles(nzz, nxx, vz, xw):
-    """(zs, zb[lanes], own[lanes], x0, x1) of each tile of a slot."""
+    """(zs, zb[lanes], own[lanes], x0, x1, back) of each tile of a slot;
+    back: the chunk marches right to left (odd chunks)."""
     lanes = np.arange(64)
     ldz, _ = layout(nzz, nxx)
     for zseg in range(zsegments(nzz, vz, 3)):
@@ -42,7 +44,7 @@ def tiles(nzz, nxx, vz, xw):
         own = (zb < ldz) & (zb >= zs + 8) & (zb < zs + 64 * vz - 8)
         for xc in range((nxx - 8 + xw - 1) // xw):
             x0 = 4 + xc * xw
-            yield zs, zb, own, x0, min(x0 + xw, nxx - 4)
+            yield zs, zb, own, x0, min(x0 + xw, nxx - 4), bool(xc & 1)
 
 
 def accesses(nzz, nxx, vz, xw, reverse, halo_lo=4):
@@ -54,14 +56,19 @@ def accesses(nzz, nxx, vz, xw, reverse, halo_lo=4):
     out = []
     cl = lambda c: min(max(c, 0), nxx - 1)
     ch = lambda c: min(max(c, halo_lo), nxx - 5)
-    for zs, zb, own, x0, x1 in tiles(nzz, nxx, vz, xw):
+    for zs, zb, own, x0, x1, back in tiles(nzz, nxx, vz, xw):
         zl = np.maximum(zb, 0)
         hz = np.where(lanes < 32, zs - 4, zs + 64 * vz)
         span = lambda c0, c1, rows, n: (c0 * ldz + rows.min(), c1 * ldz + rows.max() + n - 1)
-        out.append(("field",) + span(cl(x0 - 12), cl(x1 + 12), zl, vz))   # u_t / S_cur, R_cur
-        out.append(("field",) + span(cl(x0 - 8), cl(x1 + 8), zl, vz))     # u_old
-        out.append(("c",) + span(cl(x0 - 8), cl(x1 + 8), zl, vz))         # coefficients
-        out.append(("field",) + span(ch(x0 - 8), ch(x1 + 8), hz, 4))      # float4 halo
+        # the march's column range [a, b] as grid columns (mirrored for odd chunks)
+        pc = lambda c: (x0 + x1 - 1 - c) if back else c
+        cols = lambda a, b: (min(pc(a), pc(b)), max(pc(a), pc(b)))
+        c0, c1 = cols(x0 - 12, x1 + 12)
+        out.append(("field",) + span(cl(c0), cl(c1), zl, vz))              # u_t / S_cur, R_cur
+        c0, c1 = cols(x0 - 8, x1 + 8)
+        out.append(("field",) + span(cl(c0), cl(c1), zl, vz))              # u_old
+        out.append(("c",) + span(cl(c0), cl(c1), zl, vz))                 # coefficients
+        out.append(("field",) + span(ch(c0), ch(c1), hz, 4))               # float4 halo
         if reverse:
             out.append(("field",) + span(x0, x1 - 1, zl, vz))              # image, illum
         if own.any():
@@ -77,7 +84,7 @@ SHAPES = [(601, 1301), (163, 283), (183, 303), (120, 180), (9, 9), (13, 40), (80
 def test_owned_cells_cover_the_grid_once(nzz, nxx, vz):
     for xw in (3, 8, 81, 144, 1000):
         count = np.zeros((nxx, nzz), np.int32)
-        for zs, zb, own, x0, x1 in tiles(nzz, nxx, vz, xw):
+        for zs, zb, own, x0, x1, _ in tiles(nzz, nxx, vz, xw):
             rows = (zb[own][:, None] + np.arange(vz)[None, :]).ravel()
             rows = rows[rows < nzz]
             count[x0:x1, rows] += 1
