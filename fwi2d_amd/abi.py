@@ -41,7 +41,24 @@ class Config(C.Structure):
         ("abc_nb", C.c_int32), ("abc_factor", C.c_float),
         ("rbc_nb", C.c_int32), ("rbc_varVp", C.c_float), ("rbc_varVp_rel", C.c_float),
         ("rbc_seed", C.c_uint64), ("max_batch", C.c_int32),
+        ("rbc_kind", C.c_int32), ("rbc_ratio", C.c_float),
     ]
+
+
+RBC_UNIFORM, RBC_GAUSSIAN = 0, 1
+
+
+def rb_blobs(nz, nx, dh, rbc_nb, varVp, ratio, seed, shot, iteration=0):
+    """Blobs of the FWI_RBC_GAUSSIAN boundary of one shot and iteration, in
+    sample order, as an [n, 5] array (x, z, 16 r^2, 0.5 / r^2, amplitude);
+    host only (fwi_rb_blobs)."""
+    cfg = Config(nz, nx, dh, 2, 1e-3, 0, 0.0, rbc_nb, varVp, 0.0, seed, 1, RBC_GAUSSIAN, ratio)
+    n = C.c_int64()
+    _check(lib().fwi_rb_blobs(C.byref(cfg), shot, iteration, np.zeros(5, np.float32), 0,
+                              C.byref(n)))
+    out = np.zeros(5 * max(n.value, 1), np.float32)
+    _check(lib().fwi_rb_blobs(C.byref(cfg), shot, iteration, out, n.value, C.byref(n)))
+    return out[:5 * n.value].reshape(n.value, 5)
 
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -74,6 +91,8 @@ SIGNATURES = [
                                    C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     ("fwi_rb_velocity", C.c_int, [_ctx, C.c_int32, C.c_uint32, _f32p]),
     ("fwi_rb_velocity_noalloc", C.c_int, [_ctx, C.c_int32, C.c_uint32, _f32p]),
+    ("fwi_rb_blobs", C.c_int, [C.POINTER(Config), C.c_int32, C.c_uint32, _f32p, C.c_int64,
+                               C.POINTER(C.c_int64)]),
     ("fwi_get_accumulator", C.c_int, [_ctx, C.c_int32, _f32p]),
     ("fwi_set_accumulator", C.c_int, [_ctx, C.c_int32, _f32p]),
     ("fwi_get_config", C.c_int, [_ctx, C.POINTER(Config)]),
@@ -209,9 +228,10 @@ class Engine:
     """One survey context on the GPU (thin owner of an ``fwi_ctx``)."""
 
     def __init__(self, nz, nx, dh, nt, dt, abc_nb=0, abc_factor=0.0, rbc_nb=0,
-                 rbc_varVp=0.0, rbc_varVp_rel=0.0, rbc_seed=0, max_batch=1):
+                 rbc_varVp=0.0, rbc_varVp_rel=0.0, rbc_seed=0, max_batch=1,
+                 rbc_kind=RBC_UNIFORM, rbc_ratio=0.0):
         cfg = Config(nz, nx, dh, nt, dt, abc_nb, abc_factor, rbc_nb, rbc_varVp,
-                     rbc_varVp_rel, rbc_seed, max_batch)
+                     rbc_varVp_rel, rbc_seed, max_batch, rbc_kind, rbc_ratio)
         h = C.c_void_p()
         _check(lib().fwi_create(C.byref(cfg), C.byref(h)))
         self._h = h

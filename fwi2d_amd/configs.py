@@ -237,8 +237,12 @@ def write_parameters(path, s, model_file, sps, rps, folders, max_iteration=5, de
         f"mig_rbc_length = {(s.nb - 1) * s.dh + 0.5 * s.dh}",
         f"mig_input_folder = {folders['data']}", f"mig_input_prefix = {mig_prefix}",
         f"mig_output_folder = {folders['seismic']}", f"rbc_seed = {s.seed}",
+        "rbc_kind = uniform",           # the configurations' boundary (seeded +-40 % ramp)
     ]
-    for k, v in (extra or {}).items():
+    extra = dict(extra or {})
+    # a key of `extra` replaces the line of the same key (the parser reads the first)
+    lines = [l for l in lines if l.split(" = ")[0] not in extra]
+    for k, v in extra.items():
         lines.append(f"{k} = {v}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")

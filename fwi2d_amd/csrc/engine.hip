@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -23,6 +24,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -137,6 +139,17 @@ struct fwi_ctx {
     std::map<long, hipGraphExec_t> graphs;
 
     float *scratch = nullptr;  // this device's copy of `scratch` (fwi_rb_velocity_noalloc)
+
+    // FWI_RBC_GAUSSIAN with blobs: their shape, the host sampling buffers (one
+    // per host thread), the table staged in pinned memory and its device copy
+    fwi::RbShape rbs{};
+    std::vector<fwi::RbWork> rbwork;
+    std::vector<uint64_t> rb_streams;
+    long blob_cap = 0;         // blobs the tables hold (max_batch streams)
+    fwi::RbBlob *h_blob = nullptr, *d_blob = nullptr;
+    int32_t *h_bstart = nullptr, *d_bstart = nullptr;
+    hipEvent_t rb_copied = nullptr;   // the last table upload has read the staging
+    bool rb_pending = false;
 
     bool profile = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -407,8 +420,9 @@ void rb_bounds(const fwi_ctx *c, float *vlo, float *vhi)
 {
 #pragma clang fp contract(off)
     const fwi_config &k = c->cfg;
-    const float lo_scale = 1.0f - k.rbc_varVp_rel;
-    const float hi_scale = 1.0f + k.rbc_varVp_rel;
+    const float rel = (k.rbc_kind == FWI_RBC_UNIFORM) ? k.rbc_varVp_rel : 0.0f;
+    const float lo_scale = 1.0f - rel;
+    const float hi_scale = 1.0f + rel;
     const float lo = c->vmin * lo_scale;
     const float hi = c->vmax * hi_scale;
     *vlo = lo - k.rbc_varVp;
@@ -443,21 +457,57 @@ int set_shot0(fwi_ctx *c, int shot0)
     return FWI_OK;
 }
 
+// Random-boundary velocity of slots [0, nslots) (shots shot0 + slot, which
+// the device scalar d_shot0 already holds) into c_out (coefficients, per
+// slot) or v_out (velocities): the uniform kind from the seed alone; the
+// Gaussian kind also samples the slots' blobs on the host, stages their table
+// in pinned memory and uploads it on the context stream first.
+int rb_generate(fwi_ctx *c, int shot0, int nslots, uint32_t iteration, float *c_out,
+                float *v_out)
+{
+    const GridBuf &gb = c->grid[1];
+    const fwi_config &k = c->cfg;
+    float vlo, vhi;
+    rb_bounds(c, &vlo, &vhi);
+    if (k.rbc_kind == FWI_RBC_UNIFORM) {
+        HIP_TRY(fwi::launch_rb(gb.vpad, gb.g, nslots, c->d_shot0, iteration, k.rbc_seed,
+                               k.rbc_varVp, k.rbc_varVp_rel, vlo, vhi, c->kc, c_out, v_out,
+                               c->stream));
+        return FWI_OK;
+    }
+    const fwi::RbShape &rs = c->rbs;
+    if (rs.nbx > 0) {
+        if (nslots > c->cfg.max_batch) return fail(FWI_ERR_INTERNAL, "blob table: %d slots", nslots);
+        // the staging buffers are rewritten only after the previous upload read them
+        if (c->rb_pending) HIP_TRY(hipEventSynchronize(c->rb_copied));
+        c->rb_pending = false;
+        for (int q = 0; q < nslots; ++q)
+            c->rb_streams[q] = ((uint64_t)iteration << 32) | (uint32_t)(shot0 + q);
+        const long n = fwi::rb_blob_table(rs, k.rbc_seed, c->rb_streams.data(), nslots,
+                                          c->rbwork, c->h_blob, c->blob_cap, c->h_bstart);
+        if (n < 0) return fail(FWI_ERR_INTERNAL, "blob table exceeds its %ld blobs", c->blob_cap);
+        const long nstart = (long)nslots * (rs.nbx * rs.nbz + 1);
+        if (n > 0)
+            HIP_TRY(hipMemcpyAsync(c->d_blob, c->h_blob, n * sizeof(fwi::RbBlob),
+                                   hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->d_bstart, c->h_bstart, nstart * sizeof(int32_t),
+                               hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipEventRecord(c->rb_copied, c->stream));
+        c->rb_pending = true;
+    }
+    HIP_TRY(fwi::launch_rbg(gb.vpad, gb.g, nslots, c->d_shot0, iteration, k.rbc_seed,
+                            k.rbc_varVp, vlo, vhi, c->kc, k.dh, c->d_blob, c->d_bstart, rs.nbx,
+                            rs.nbz, rs.bin, c_out, v_out, c->stream));
+    return FWI_OK;
+}
+
 // Forward pass of one batch.  The wavefield lives in d_f[0..3]; its final
 // pair (u_T, u_{T-1}) is left in c->fwd (used by the reverse pass).
 int forward_batch(fwi_ctx *c, int kind, int shot0, int nslots, bool record, uint32_t iteration)
 {
-    GridBuf &gb = c->grid[kind];
     TRY(set_shot0(c, shot0));
     const long nf = c->lay.pass_fill(kind, nslots);
-    if (kind == FWI_BOUNDARY_RBC) {
-        const fwi_config &k = c->cfg;
-        float vlo, vhi;
-        rb_bounds(c, &vlo, &vhi);
-        HIP_TRY(fwi::launch_rb(gb.vpad, gb.g, nslots, c->d_shot0, iteration, k.rbc_seed,
-                               k.rbc_varVp, k.rbc_varVp_rel, vlo, vhi, c->kc, c->d_crb, nullptr,
-                               c->stream));
-    }
+    if (kind == FWI_BOUNDARY_RBC) TRY(rb_generate(c, shot0, nslots, iteration, c->d_crb, nullptr));
     // all four buffers start at zero in this grid's layout (rim and padding
     // cells are never written and must read as 0)
     for (int k = 0; k < 4; ++k) TRY(fill(c, c->d_f[k], nf));
@@ -533,6 +583,39 @@ const char *fwi_last_error(void) { return g_last_error.c_str(); }
 
 const char *fwi_version(void) { return "fwi2d-mi355x 0.1 (gfx950)"; }
 
+namespace {
+
+// Blob tables of the Gaussian boundary, sized once for max_batch streams of
+// the random-boundary grid (rb_blob_bound blobs each), and the host sampling
+// buffers, reserved so that sampling allocates nothing afterwards.
+int setup_blobs(fwi_ctx *c)
+{
+    const fwi::Grid &g = c->grid[1].g;
+    c->rbs = fwi::rb_shape(g.nzz, g.nxx, g.nb, c->cfg.dh, c->cfg.rbc_ratio, c->cfg.rbc_varVp);
+    if (c->rbs.nbx == 0) return FWI_OK;
+    const long per = fwi::rb_blob_bound(c->rbs);
+    const long slots = c->cfg.max_batch;
+    const long nstart = slots * ((long)c->rbs.nbx * c->rbs.nbz + 1);
+    const double bytes = (double)per * slots * sizeof(fwi::RbBlob) + (double)nstart * 4;
+    if (bytes > 512.0 * 1024 * 1024 || (double)per * slots > INT32_MAX)
+        return fail(FWI_ERR_ARG, "rbc_ratio %g m is too small for this grid (%ld blobs per shot)",
+                    c->cfg.rbc_ratio, per);
+    c->blob_cap = per * slots;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int threads = (int)std::max(1u, std::min(16u, hw));
+    c->rbwork.resize(threads);
+    for (auto &w : c->rbwork) w.reserve(c->rbs);
+    c->rb_streams.resize(slots);
+    HIP_TRY(hipMalloc(&c->d_blob, c->blob_cap * sizeof(fwi::RbBlob)));
+    HIP_TRY(hipMalloc(&c->d_bstart, nstart * sizeof(int32_t)));
+    HIP_TRY(hipHostMalloc(&c->h_blob, c->blob_cap * sizeof(fwi::RbBlob), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&c->h_bstart, nstart * sizeof(int32_t), hipHostMallocDefault));
+    HIP_TRY(hipEventCreateWithFlags(&c->rb_copied, hipEventDisableTiming));
+    return FWI_OK;
+}
+
+}  // namespace
+
 int fwi_create(const fwi_config *cfg, fwi_ctx **out)
 {
     if (!cfg || !out) return fail(FWI_ERR_ARG, "null argument");
@@ -543,6 +626,10 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     if (cfg->max_batch < 1) return fail(FWI_ERR_ARG, "max_batch must be >= 1");
     if (cfg->abc_nb <= 0 && cfg->rbc_nb <= 0)
         return fail(FWI_ERR_ARG, "neither an abc nor an rbc grid is configured");
+    if (cfg->rbc_kind != FWI_RBC_UNIFORM && cfg->rbc_kind != FWI_RBC_GAUSSIAN)
+        return fail(FWI_ERR_ARG, "unknown rbc_kind %d", cfg->rbc_kind);
+    if (cfg->rbc_kind == FWI_RBC_GAUSSIAN && (!(cfg->rbc_ratio >= 0.f) || !(cfg->rbc_varVp >= 0.f)))
+        return fail(FWI_ERR_ARG, "rbc_ratio and rbc_varVp must be >= 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(FWI_ERR_HIP, "no HIP device available");
@@ -592,6 +679,8 @@ int fwi_create(const fwi_config *cfg, fwi_ctx **out)
     if (r == FWI_OK) r = dalloc(&c->d_partials, c->npartials, c->stream);
     if (r == FWI_OK) r = dalloc(&c->d_sum, 1, c->stream);
     if (r == FWI_OK) r = scratch_on_device(&c->scratch);
+    if (r == FWI_OK && c->grid[1].on && cfg->rbc_kind == FWI_RBC_GAUSSIAN)
+        r = setup_blobs(c);
     if (r != FWI_OK) { fwi_destroy(c); return r; }
     // default launch shapes (the autotuner refines them, see the fields)
     *out = c;
@@ -614,6 +703,11 @@ int fwi_destroy(fwi_ctx *c)
         for (void *p : gp)
             if (p) (void)hipFree(p);
     }
+    if (c->d_blob) (void)hipFree(c->d_blob);
+    if (c->d_bstart) (void)hipFree(c->d_bstart);
+    if (c->h_blob) (void)hipHostFree(c->h_blob);
+    if (c->h_bstart) (void)hipHostFree(c->h_bstart);
+    if (c->rb_copied) (void)hipEventDestroy(c->rb_copied);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1005,13 +1099,8 @@ int rb_velocity_via(fwi_ctx *c, int32_t shot, uint32_t iteration, float *d_out, 
     const GridBuf &gb = c->grid[1];
     const long ncell = (long)gb.g.nzz * gb.g.nxx;
     TRY(set_shot0(c, shot));
-    const fwi_config &k = c->cfg;
-    float vlo, vhi;
-    rb_bounds(c, &vlo, &vhi);
-    hipError_t e = fwi::launch_rb(gb.vpad, gb.g, 1, c->d_shot0, iteration, k.rbc_seed,
-                                  k.rbc_varVp, k.rbc_varVp_rel, vlo, vhi, c->kc, nullptr, d_out,
-                                  c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, ncell * 4, hipMemcpyDeviceToHost, c->stream);
+    TRY(rb_generate(c, shot, 1, iteration, nullptr, d_out));
+    hipError_t e = hipMemcpyAsync(out, d_out, ncell * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(FWI_ERR_HIP, "rb velocity: %s", hipGetErrorString(e));
     return FWI_OK;
@@ -1029,6 +1118,27 @@ int check_rb_velocity(fwi_ctx *c, int32_t shot, const float *out)
 }  // namespace
 
 extern "C" {
+
+int fwi_rb_blobs(const fwi_config *cfg, int32_t shot, uint32_t iteration, float *out,
+                 int64_t cap, int64_t *count)
+{
+    if (!cfg || !count || (cap > 0 && !out)) return fail(FWI_ERR_ARG, "null argument");
+    if (cfg->nz < 1 || cfg->nx < 1 || !(cfg->dh > 0.f) || cfg->rbc_nb <= 0 || cap < 0)
+        return fail(FWI_ERR_ARG, "bad grid or capacity");
+    const fwi::GridDims d = fwi::grid_dims(cfg->nz, cfg->nx, cfg->rbc_nb);
+    const fwi::RbShape rs =
+        fwi::rb_shape(d.nzz, d.nxx, d.nb, cfg->dh, cfg->rbc_ratio, cfg->rbc_varVp);
+    fwi::RbWork w;
+    w.reserve(rs);
+    fwi::rb_blobs(rs, cfg->rbc_seed, ((uint64_t)iteration << 32) | (uint32_t)shot, w);
+    *count = (int64_t)w.blobs.size();
+    for (int64_t k = 0; k < *count && k < cap; ++k) {
+        const fwi::RbBlob &b = w.blobs[k];
+        float *o = out + 5 * k;
+        o[0] = b.x; o[1] = b.z; o[2] = b.c2; o[3] = b.h; o[4] = b.a;
+    }
+    return FWI_OK;
+}
 
 int fwi_rb_velocity(fwi_ctx *c, int32_t shot, uint32_t iteration, float *out)
 {

@@ -1,6 +1,7 @@
 // kernels.h -- launch interface of the HIP kernels (device code lives in
 // propagate.hip).  Host-only declarations: plain structs and launchers.
 #pragma once
+#include "rbc.h"
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -117,6 +118,12 @@ hipError_t launch_coef(const float *v, const Grid &g, float k, float *c, hipStre
 hipError_t launch_rb(const float *vpad, const Grid &g, int nslots, const int *shot0,
                      uint32_t iteration, uint64_t seed, float amp_abs, float amp_rel,
                      float vlo, float vhi, float k, float *c, float *v_out, hipStream_t s);
+// The reference-style random boundary (Gaussian ramps and the blob table of
+// rbc.h, one table segment per slot) of nslots slots; like launch_rb.
+hipError_t launch_rbg(const float *vpad, const Grid &g, int nslots, const int *shot0,
+                      uint32_t iteration, uint64_t seed, float sigma, float vlo, float vhi,
+                      float k, float dh, const RbBlob *blob, const int32_t *bstart, int nbx,
+                      int nbz, float bin, float *c, float *v_out, hipStream_t s);
 // residual = obs[shot0+slot] - cal (in place in cal), FP64 partial sums.
 hipError_t launch_residual(const float *obs, float *cal, const int *shot0, long per_shot,
                            int nslots, double *partials, int nblocks, hipStream_t s);

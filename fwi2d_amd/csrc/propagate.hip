@@ -1622,6 +1622,125 @@ __global__ void rb_kernel(const float *vpad, Grid g, int nslots, const int *shot
     }
 }
 
+// Standard normal draw of pad cell `cell` for ramp k (0: x pad, 1: z pad):
+// the Irwin-Hall sum of twelve 16-bit uniforms, u = (v + 1/2) / 2^16, from
+// three hashes, minus 6.  Integer sum, then one exact scaling: the same bits
+// on any IEEE machine.  Mean 0, variance 1, bounded by +-6.
+__device__ __forceinline__ float rb_normal(uint64_t key, uint64_t cell, int k)
+{
+    uint32_t sum = 0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const uint64_t h = mix64(key + (6 * cell + 3 * (uint64_t)k + (uint64_t)m + 1) * kGolden);
+        sum += (uint32_t)(h & 0xFFFF) + (uint32_t)((h >> 16) & 0xFFFF) +
+               (uint32_t)((h >> 32) & 0xFFFF) + (uint32_t)(h >> 48);
+    }
+    return (float)((int32_t)(2 * sum) + 12 - 786432) * (1.0f / 131072.0f);
+}
+
+// e^-q for q >= 0 with one rounding per operation: n = (int)(q log2 e),
+// y = q - n ln 2 (Cody-Waite: ln 2 split into a short head and a tail, two
+// fmaf), e^-q = 2^-n e^-y, the second factor by its Taylor polynomial of
+// degree 9 (truncation below 1.1e-8 on [0, ln 2)), evaluated by Horner with
+// fmaf.
+__device__ __forceinline__ float rb_expneg(float q)
+{
+#pragma clang fp contract(off)
+    const int n = (int)(q * 1.44269504f);
+    const float fn = (float)n;
+    float y = fmaf(-fn, 0.693145752f, q);      // q - n ln 2, ln 2 split in two
+    y = fmaf(-fn, 1.42860682e-6f, y);
+    float p = -2.75573192e-6f;
+    p = fmaf(p, y, 2.48015873e-5f);
+    p = fmaf(p, y, -1.98412698e-4f);
+    p = fmaf(p, y, 1.38888889e-3f);
+    p = fmaf(p, y, -8.33333333e-3f);
+    p = fmaf(p, y, 4.16666667e-2f);
+    p = fmaf(p, y, -1.66666667e-1f);
+    p = fmaf(p, y, 0.5f);
+    p = fmaf(p, y, -1.0f);
+    p = fmaf(p, y, 1.0f);
+    return ldexpf(p, -n);
+}
+
+// The reference's random boundary with seeded draws (FWI_RBC_GAUSSIAN;
+// random_boundary_bg, modeling.cu:487-524, and random_boundary_gp,
+// modeling.cu:467-485).  Pad cells only:
+//   val = base [+ (fx sigma) N0 if x pad] [+ (fz sigma) N1 if z pad]
+//   then, over the 3 x 3 blob bins around the cell (x-major, z inner) and
+//   the blobs of each bin in table order, for d^2 < 16 r^2:
+//   val += a * e^-(d^2 * 0.5 / r^2)
+//   val = min(max(val, vlo), vhi)
+// with x = j dh, z = i dh and the ramps of rb_kernel.  FMA contraction is
+// off; every operation is one correctly rounded IEEE operation, so a
+// sequential C statement of the same steps gives the same bits.
+__global__ void rbg_kernel(const float *vpad, Grid g, int nslots, const int *shot0,
+                           uint32_t iteration, uint64_t seed, float sigma, float vlo, float vhi,
+                           float kc, float dh, const RbBlob *blob, const int32_t *bstart,
+                           int nbx, int nbz, float bin, float *c, float *v_out)
+{
+#pragma clang fp contract(off)
+    const long ncell = (long)g.nzz * g.nxx;
+    const int slot = blockIdx.y;
+    const uint64_t stream = ((uint64_t)iteration << 32) | (uint32_t)(*shot0 + slot);
+    const uint64_t key = mix64(seed ^ mix64(stream + kGolden));
+    const float fnb = (float)g.nb;
+    const int nbins = nbx * nbz;
+    const int32_t *st = bstart + (long)slot * (nbins + 1);
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < ncell;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % g.nzz), j = (int)(idx / g.nzz);
+        const float base = vpad[(long)j * g.ldz + i];
+        const bool xpad = (j < g.nb) || (j >= g.nxx - g.nb);
+        const bool zpad = (i < g.nb) || (i >= g.nzz - g.nb);
+        float val = base;
+        if (xpad || zpad) {
+            const uint64_t cell = (uint64_t)idx;
+            if (xpad) {
+                const int dj = (j < g.nb) ? j : (g.nxx - 1 - j);
+                const float q = (float)dj / fnb;
+                const float f = 1.0f - q;
+                const float fs = f * sigma;
+                const float d = fs * rb_normal(key, cell, 0);
+                val = val + d;
+            }
+            if (zpad) {
+                const int di = (i < g.nb) ? i : (g.nzz - 1 - i);
+                const float q = (float)di / fnb;
+                const float f = 1.0f - q;
+                const float fs = f * sigma;
+                const float d = fs * rb_normal(key, cell, 1);
+                val = val + d;
+            }
+            if (nbins > 0) {
+                const float x = (float)j * dh, z = (float)i * dh;
+                const int cbx = min((int)(x / bin), nbx - 1);
+                const int cbz = min((int)(z / bin), nbz - 1);
+                for (int bx = max(cbx - 1, 0); bx <= min(cbx + 1, nbx - 1); ++bx)
+                    for (int bz = max(cbz - 1, 0); bz <= min(cbz + 1, nbz - 1); ++bz) {
+                        const int b = bz + bx * nbz;
+                        for (int k = st[b]; k < st[b + 1]; ++k) {
+                            const RbBlob B = blob[k];
+                            const float dx = x - B.x;
+                            const float dz = z - B.z;
+                            const float dx2 = dx * dx;
+                            const float dz2 = dz * dz;
+                            const float d2 = dx2 + dz2;
+                            if (d2 < B.c2) {
+                                const float e = rb_expneg(d2 * B.h);
+                                const float add = B.a * e;
+                                val = val + add;
+                            }
+                        }
+                    }
+            }
+            val = fminf(fmaxf(val, vlo), vhi);
+        }
+        if (c) c[(long)slot * g.fstride + (long)j * g.ldz + i] = (val * val) * kc;
+        if (v_out) v_out[(long)slot * ncell + idx] = val;
+    }
+}
+
 __global__ __launch_bounds__(256) void residual_kernel(const float *obs, float *cal,
                                                        const int *shot0, long per_shot,
                                                        long n, double *partials)
@@ -1887,6 +2006,18 @@ hipError_t launch_rb(const float *vpad, const Grid &g, int nslots, const int *sh
     const long n = (long)g.nzz * g.nxx;
     hipLaunchKernelGGL(rb_kernel, dim3(grid1d(n, 256, 2048), nslots), dim3(256), 0, s, vpad, g,
                        nslots, shot0, iteration, seed, amp_abs, amp_rel, vlo, vhi, k, c, v_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_rbg(const float *vpad, const Grid &g, int nslots, const int *shot0,
+                      uint32_t iteration, uint64_t seed, float sigma, float vlo, float vhi,
+                      float k, float dh, const RbBlob *blob, const int32_t *bstart, int nbx,
+                      int nbz, float bin, float *c, float *v_out, hipStream_t s)
+{
+    const long n = (long)g.nzz * g.nxx;
+    hipLaunchKernelGGL(rbg_kernel, dim3(grid1d(n, 256, 2048), nslots), dim3(256), 0, s, vpad, g,
+                       nslots, shot0, iteration, seed, sigma, vlo, vhi, k, dh, blob, bstart, nbx,
+                       nbz, bin, c, v_out);
     return hipGetLastError();
 }
 

@@ -60,7 +60,20 @@ void Modeling::set_properties()
     vmax = *std::max_element(vp.begin(), vp.end());
 }
 
-void Modeling::create_engine(int rbc_nb, float varVp, float varVp_rel, uint64_t seed)
+// Kind of random boundary: `rbc_kind = gaussian` (the default: the
+// reference's Gaussian noise and Poisson-disk blobs of radius up to
+// *_rbc_ratio, seeded) or `uniform` (a uniform perturbation of amplitude
+// *_rbc_varVp + *_rbc_varVp_rel times the edge speed).
+int rbc_kind_of(const std::string &parameters)
+{
+    const std::string k = catch_parameter("rbc_kind", parameters, "gaussian");
+    if (k == "gaussian") return FWI_RBC_GAUSSIAN;
+    if (k == "uniform") return FWI_RBC_UNIFORM;
+    throw std::runtime_error("Error: rbc_kind must be gaussian or uniform, not '" + k + "'");
+}
+
+void Modeling::create_engine(int rbc_nb, float varVp, float varVp_rel, uint64_t seed, int kind,
+                             float ratio)
 {
     geometry.set_parameters(parameters);
     max_batch = param_int("gpu_max_batch", parameters, std::min(geometry.nsrc, 256));
@@ -76,6 +89,8 @@ void Modeling::create_engine(int rbc_nb, float varVp, float varVp_rel, uint64_t 
     cfg.rbc_varVp = varVp;
     cfg.rbc_varVp_rel = varVp_rel;
     cfg.rbc_seed = seed;
+    cfg.rbc_kind = kind;
+    cfg.rbc_ratio = ratio;
     cfg.max_batch = std::max(1, max_batch);
     check(fwi_create(&cfg, &ctx), "create");
     check(fwi_set_wavelet(ctx, wavelet.data(), tlag), "set_wavelet");
@@ -142,6 +157,8 @@ void Migration::set_parameters()
     set_main_parameters();
     rbc_varVp = param_float("mig_rbc_varVp", parameters);
     rbc_varVp_rel = param_float("mig_rbc_varVp_rel", parameters, 0.f);
+    rbc_ratio = param_float("mig_rbc_ratio", parameters, 0.f);
+    rbc_kind = rbc_kind_of(parameters);
     const float rbc_length = param_float("mig_rbc_length", parameters);
     rbc_nb = (int)(rbc_length / dh) + 1;
     const uint64_t seed = param_u64("rbc_seed", parameters, 20240u);
@@ -151,7 +168,7 @@ void Migration::set_parameters()
     input_prefix = catch_parameter("mig_input_prefix", parameters);
     output_folder = catch_parameter("mig_output_folder", parameters, "");
     abc_nb = 0;
-    create_engine(rbc_nb, rbc_varVp, rbc_varVp_rel, seed);
+    create_engine(rbc_nb, rbc_varVp, rbc_varVp_rel, seed, rbc_kind, rbc_ratio);
     const long per = (long)geometry.nrec * nt;
     std::vector<float> data(per);
     for (int s = 0; s < geometry.nsrc; ++s) {   // migration.cu:83-88, once per shot
@@ -219,6 +236,8 @@ void Inversion::set_parameters()
     abc_nb = (int)(abc_length / dh) + 1;
     rbc_varVp = param_float("inv_rbc_varVp", parameters);
     rbc_varVp_rel = param_float("inv_rbc_varVp_rel", parameters, 0.f);
+    rbc_ratio = param_float("inv_rbc_ratio", parameters, 0.f);
+    rbc_kind = rbc_kind_of(parameters);
     const float rbc_length = param_float("inv_rbc_length", parameters);
     rbc_nb = (int)(rbc_length / dh) + 1;
     const uint64_t seed = param_u64("rbc_seed", parameters, 20240u);
@@ -237,7 +256,7 @@ void Inversion::set_parameters()
     vmin0 = vmin;
     vmax0 = vmax;
     set_trial();
-    create_engine(rbc_nb, rbc_varVp, rbc_varVp_rel, seed);
+    create_engine(rbc_nb, rbc_varVp, rbc_varVp_rel, seed, rbc_kind, rbc_ratio);
     // observed data stay resident on the device (read once, not every pass)
     const long per = (long)geometry.nrec * nt;
     std::vector<float> data(per);

@@ -44,7 +44,8 @@ protected:
     void set_main_parameters();     // modeling.cu:17-30
     void set_wavelet();             // modeling.cu:32-90
     void set_properties();          // modeling.cu:152-188
-    void create_engine(int rbc_nb, float varVp, float varVp_rel, uint64_t seed);
+    void create_engine(int rbc_nb, float varVp, float varVp_rel, uint64_t seed,
+                       int kind = FWI_RBC_UNIFORM, float ratio = 0.f);
 };
 
 class Migration : public Modeling {
@@ -54,8 +55,8 @@ public:
     void export_seismic();          // migration.cu:90-115
 
 private:
-    float rbc_varVp = 0, rbc_varVp_rel = 0;
-    int rbc_nb = 0;
+    float rbc_varVp = 0, rbc_varVp_rel = 0, rbc_ratio = 0;
+    int rbc_nb = 0, rbc_kind = FWI_RBC_GAUSSIAN;
     std::string input_folder, input_prefix, output_folder;
 };
 
@@ -90,8 +91,8 @@ public:
 
 private:
     float zmask = 0;
-    float rbc_varVp = 0, rbc_varVp_rel = 0;
-    int rbc_nb = 0;
+    float rbc_varVp = 0, rbc_varVp_rel = 0, rbc_ratio = 0;
+    int rbc_nb = 0, rbc_kind = FWI_RBC_GAUSSIAN;
     double sum_res = 0;
     float vmin0 = 0, vmax0 = 0;     // extremes of the initial model (dm of the line search)
     std::string input_folder, input_prefix, output_folder, residuo_folder;

@@ -371,7 +371,7 @@ def _seed_samples(setup, seed):
 
 
 def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, log=None,
-          checkpoint=None):
+          checkpoint=None, init=None, state_out=None):
     """Adam on the GPU; returns (params, history of (epoch, train loss,
     held-out loss), held-out indices).  With flip, each training sample is
     mirrored along x with probability 1/2 (the wave equation is symmetric
@@ -383,13 +383,24 @@ def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, lo
     from the epoch after it: a run stopped and resumed gives the same
     parameters, bit for bit, as one that ran straight through.  A checkpoint
     of another training problem (sample count, image size, batch, seed,
-    hold-out, mirroring, learning rate) is refused."""
+    hold-out, mirroring, learning rate) is refused.
+
+    With `init` (a dict: params, and optionally the Adam moments m, v and the
+    step count steps of an earlier training), training starts from that state
+    instead of a fresh network: a training set too large for one session is
+    then learned in phases, one set of models per phase.  With `state_out`
+    (a file path), the final parameters and Adam state are written there in
+    the form `init` takes (load_state)."""
     n, _, h, w = x.shape
     rng = np.random.default_rng(seed)
     order = rng.permutation(n)
     nval = max(1, int(round(holdout * n)))
     val, tr = order[:nval], order[nval:]
     net = A.Net(batch, h, w, seed)
+    if init is not None:
+        net.set_params(init["params"])
+        if "m" in init:
+            net.set_optimizer(init["m"], init["v"], int(init["steps"]))
     hist = []
     first = 0
     problem = {"n": int(n), "h": int(h), "w": int(w), "batch": int(batch), "seed": int(seed),
@@ -435,8 +446,21 @@ def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, lo
             _save_npz(checkpoint, params=net.get_params(), m=m, v=v, order=tr,
                       meta=np.array(json.dumps(meta)))
     p = net.get_params()
+    if state_out:
+        m, v, steps = net.get_optimizer()
+        _save_npz(state_out, params=p, m=m, v=v, steps=np.array(steps))
     net.close()
     return p, hist, val
+
+
+def load_state(path):
+    """Training state for train(init=...): a state file written by train
+    (parameters and Adam state) or a bare parameter file (.bin, float32;
+    Adam starts afresh)."""
+    if path.endswith(".npz"):
+        with np.load(path) as z:
+            return {k: z[k] for k in ("params", "m", "v", "steps")}
+    return {"params": np.fromfile(path, np.float32)}
 
 
 def holdout_cosines(params, x, y, g, idx):
@@ -515,6 +539,9 @@ def main(argv=None):
     p.add_argument("--checkpoint", default="",
                    help="folder for resumable sample generation (one file per training "
                         "model) and training (one file, rewritten every epoch)")
+    p.add_argument("--continue-from", default="",
+                   help="train from this state (a train_state.npz of an earlier phase, or a "
+                        "parameter file) instead of a fresh network")
     p.add_argument("--merge", nargs="+", default=None, metavar="REPORT",
                    help="summarise the runs of several report.json files of the same "
                         "comparison (runs made in separate sessions) and exit")
@@ -548,12 +575,17 @@ def main(argv=None):
         t0 = time.perf_counter()
         params, th, val = train(x, y, epochs=a.epochs, lr=a.lr, log=log,
                                 checkpoint=os.path.join(ck, f"train_{a.train_size}.npz")
-                                if ck else None)
+                                if ck else None,
+                                init=load_state(a.continue_from) if a.continue_from else None,
+                                state_out=os.path.join(a.out, "train_state.npz"))
         t_train = time.perf_counter() - t0
         cos = holdout_cosines(params, x, y, g, val)
         log({"holdout_cosine_learned": cos[0], "holdout_cosine_plain": cos[1]})
     params.tofile(os.path.join(a.out, "preconditioner_params.bin"))
     train_report = {"samples": int(len(x)), "generation_s": t_gen, "training_s": t_train,
+                    "continued_from": a.continue_from or None,
+                    "train_seeds": None if a.params else
+                    [a.train_first_seed, a.train_first_seed + a.train_models - 1],
                     "holdout_cosine": {"learned": cos[0], "plain_direction": cos[1]},
                     "train_loss_first_last": [th[0][1], th[-1][1]],
                     "holdout_loss_first_last": [th[0][2], th[-1][2]]}

@@ -35,6 +35,16 @@ extern "C" {
 #define FWI_BOUNDARY_ABC 0  /* Cerjan sponge on the abc_nb-padded grid  */
 #define FWI_BOUNDARY_RBC 1  /* seeded random boundary on the rbc_nb grid */
 
+/* Kinds of random boundary (fwi_config.rbc_kind), both seeded by rbc_seed. */
+#define FWI_RBC_UNIFORM  0  /* uniform perturbation of amplitude rbc_varVp +
+                               rbc_varVp_rel * edge speed, ramped 0 -> 1 */
+#define FWI_RBC_GAUSSIAN 1  /* the reference's boundary: Gaussian noise of
+                               standard deviation rbc_varVp ramped 0 -> 1, plus
+                               Poisson-disk Gaussian blobs of radius up to
+                               rbc_ratio and amplitude up to rbc_varVp in the
+                               outer half of the pad
+                               (src/modeling/modeling.cu:431-531) */
+
 #define FWI_RESET_IMAGE    1
 #define FWI_RESET_GRADIENT 2
 #define FWI_RESET_ILLUM    4
@@ -53,6 +63,9 @@ typedef struct fwi_config {
     float    rbc_varVp_rel; /* extra amplitude as a fraction of the edge speed */
     uint64_t rbc_seed;      /* seed of the random boundary                     */
     int32_t  max_batch;     /* shots resident on the device per pass (>= 1)    */
+    int32_t  rbc_kind;      /* FWI_RBC_UNIFORM (0) or FWI_RBC_GAUSSIAN         */
+    float    rbc_ratio;     /* blob distance and largest radius [m] (GAUSSIAN;
+                               0: no blobs); rbc_varVp_rel is not used there   */
 } fwi_config;
 
 /* Message of the last failure on this thread ("" if none). */
@@ -170,6 +183,15 @@ int fwi_set_accumulator(fwi_ctx *ctx, int32_t what, const float *in);
 /* Test hook: the random-boundary velocity [nxx][nzz] (rbc grid) that a pass
  * of shot `shot` in FWI iteration `iteration` uses (migration: iteration 0). */
 int fwi_rb_velocity(fwi_ctx *ctx, int32_t shot, uint32_t iteration, float *vp_padded_out);
+
+/* Test hook, host only (no device needed): the blobs of the FWI_RBC_GAUSSIAN
+ * boundary of one shot and iteration under cfg, in sample order, five floats
+ * each (x, z [m] on the padded grid, 16 r^2, 0.5 / r^2, signed amplitude).
+ * Writes at most cap blobs; *count gets the number of blobs.  Replaces the
+ * host part of Modeling::set_random_boundary (src/modeling/modeling.cu:431-465)
+ * and poissonDiskSampling (src/admin/admin.cpp:115-206). */
+int fwi_rb_blobs(const fwi_config *cfg, int32_t shot, uint32_t iteration, float *out,
+                 int64_t cap, int64_t *count);
 
 /* fwi_rb_velocity without a device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP

@@ -69,6 +69,16 @@ float or_rb_uniform(uint64_t seed, uint64_t stream, uint64_t cell, int k);
 void or_rb_velocity(const float *vpad, int nzz, int nxx, int nb,
                     float amp_abs, float amp_rel, float vlo, float vhi,
                     uint64_t seed, uint64_t stream, float *out);
+/* The reference-style boundary (FWI_RBC_GAUSSIAN): blobs of one stream
+ * (5 floats each, sample order; returns the count, writes at most cap),
+ * the Irwin-Hall normal draw, e^-q, and the RB model. */
+long or_rb_blobs(int nzz, int nxx, int nb, float dh, float ratio, float amp, uint64_t seed,
+                 uint64_t stream, float *out, long cap);
+float or_rb_normal(uint64_t seed, uint64_t stream, uint64_t cell, int k);
+float or_rb_expneg(float q);
+void or_rb_velocity_gauss(const float *vpad, int nzz, int nxx, int nb, float dh, float varVp,
+                          float ratio, float vlo, float vhi, uint64_t seed, uint64_t stream,
+                          float *out);
 
 /* ---- propagators (oracle_core.inc, instantiated twice) ----------------- */
 /* Laplacian (d2/dx2 + d2/dz2) of P on the updated cells, 0 on the 4-cell rim. */

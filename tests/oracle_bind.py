@@ -47,6 +47,16 @@ def lib():
         L.or_rb_velocity.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float,
                                      C.c_float, C.c_float, C.c_float, C.c_uint64,
                                      C.c_uint64, _f32p]
+        L.or_rb_blobs.restype = C.c_long
+        L.or_rb_blobs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                  C.c_uint64, C.c_uint64, C.c_void_p, C.c_long]
+        L.or_rb_normal.restype = C.c_float
+        L.or_rb_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]
+        L.or_rb_expneg.restype = C.c_float
+        L.or_rb_expneg.argtypes = [C.c_float]
+        L.or_rb_velocity_gauss.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                           C.c_float, C.c_float, C.c_float, C.c_float,
+                                           C.c_uint64, C.c_uint64, _f32p]
         for sfx, fp in (("f64", _f64p), ("f32", _f32p)):
             fwd = getattr(L, "or_forward_" + sfx)
             fwd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
@@ -99,6 +109,24 @@ def rb_velocity(vpad, nb, amp_abs, amp_rel, vlo, vhi, seed, stream):
     out = np.zeros_like(vpad, dtype=np.float32)
     lib().or_rb_velocity(np.ascontiguousarray(vpad, np.float32), nzz, nxx, nb,
                          amp_abs, amp_rel, vlo, vhi, seed, stream, out)
+    return out
+
+
+def rb_blobs(nzz, nxx, nb, dh, ratio, amp, seed, stream):
+    """Blobs of the reference-style boundary of one stream: [n, 5] (x, z,
+    16 r^2, 0.5 / r^2, amplitude) in sample order."""
+    n = lib().or_rb_blobs(nzz, nxx, nb, dh, ratio, amp, seed, stream, None, 0)
+    out = np.zeros(5 * max(n, 1), np.float32)
+    lib().or_rb_blobs(nzz, nxx, nb, dh, ratio, amp, seed, stream,
+                      out.ctypes.data_as(C.c_void_p), n)
+    return out[:5 * n].reshape(n, 5)
+
+
+def rb_velocity_gauss(vpad, nb, dh, varVp, ratio, vlo, vhi, seed, stream):
+    nxx, nzz = vpad.shape
+    out = np.zeros_like(vpad, dtype=np.float32)
+    lib().or_rb_velocity_gauss(np.ascontiguousarray(vpad, np.float32), nzz, nxx, nb, dh, varVp,
+                               ratio, vlo, vhi, seed, stream, out)
     return out
 
 

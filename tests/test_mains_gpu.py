@@ -269,3 +269,40 @@ def test_inversion_trial_directions(tmp_path):
                       open(os.path.join(f["residuo"], "convergence_3_iterations.txt")).read())
     assert np.array_equal(outs["plain"][0], outs["network"][0])
     assert outs["plain"][1] == outs["network"][1]
+
+
+def test_reference_random_boundary_keys(tmp_path):
+    """A parameters.txt with the reference's keys only (no rbc_kind): the
+    executables take the reference's boundary, Gaussian noise of standard
+    deviation mig_rbc_varVp plus Poisson-disk blobs of radius up to
+    mig_rbc_ratio (seeded), and the RTM section equals the library's with
+    that boundary bit for bit."""
+    tmp = str(tmp_path)
+    v, vsmooth, zs, zr, folders, prefix, params = setup_files(tmp)
+    s = SURVEY
+    for name in ("mod", "mig"):
+        with open(params[name]) as f:
+            lines = [l for l in f if not l.startswith("rbc_kind")]
+        with open(params[name], "w") as f:
+            f.writelines(lines)
+    run("modeling", params["mod"])
+    nrec = len(XR)
+    files = [os.path.join(folders["data"], f"{prefix}{k + 1}.bin") for k in range(len(XS))]
+    obs = np.stack([np.fromfile(f, np.float32).reshape(nrec, NT) for f in files])
+    run("migration", params["mig"])
+    img = np.fromfile(os.path.join(folders["seismic"], f"RTM_section_{int(FMAX)}Hz_{s.nz}x"
+                                   f"{s.nx}_{int(s.dh)}m.bin"), np.float32).reshape(s.nx, s.nz)
+    w, tlag = F.ricker_wavelet(NT, DT, FMAX)
+    eng = F.Engine(s.nz, s.nx, s.dh, NT, DT, abc_nb=s.abc_nb, abc_factor=s.abc_factor,
+                   rbc_nb=s.nb, rbc_varVp=s.varVp, rbc_varVp_rel=0.0, rbc_seed=s.seed,
+                   max_batch=len(XS), rbc_kind=F.abi.RBC_GAUSSIAN, rbc_ratio=200.0)
+    eng.set_wavelet(w, tlag)
+    eng.set_geometry(XS, zs, XR, zr)
+    eng.set_model(vsmooth)
+    eng.set_observed(obs)
+    eng.reset()
+    eng.migration()
+    ref_img, _, _ = eng.get_image()
+    eng.close()
+    assert np.array_equal(img, ref_img)
+    assert np.isfinite(img).all() and np.abs(img).max() > 0
