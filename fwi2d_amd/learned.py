@@ -455,9 +455,11 @@ def train(x, y, epochs=40, batch=16, lr=1e-3, seed=0, holdout=0.1, flip=True, lo
 
 def load_state(path):
     """Training state for train(init=...): a state file written by train
-    (parameters and Adam state) or a bare parameter file (.bin, float32;
-    Adam starts afresh)."""
-    if path.endswith(".npz"):
+    (an npz archive: parameters and Adam state, whatever its name) or a bare
+    parameter file (float32; Adam starts afresh)."""
+    with open(path, "rb") as f:
+        is_zip = f.read(4) == b"PK\x03\x04"
+    if is_zip:
         with np.load(path) as z:
             return {k: z[k] for k in ("params", "m", "v", "steps")}
     return {"params": np.fromfile(path, np.float32)}
