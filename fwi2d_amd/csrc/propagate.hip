@@ -195,6 +195,32 @@ __device__ __forceinline__ float lap8r(const float (&q)[9][VZ], const float (&ex
     return fmaf(kC4, s1, fmaf(-kC3, s2, fmaf(kC2, s3, -kC1 * s4)));
 }
 
+// lap8r with the z-part of the nearest ring taken from the row differences
+// dz[m] = ext[m + 4] - ext[m + 3] (zdiff), which neighbouring rows of a lane
+// share: (ext[r+3] - p) + (ext[r+5] - p) is -dz[r] + dz[r+1], the same
+// rounded values summed in the same order, so the result is bit-identical
+// to lap8r with VZ + 1 subtractions per lane instead of 2 VZ.
+template <int VZ, int OFF>
+__device__ __forceinline__ float lap8z(const float (&q)[9][VZ], const float (&ext)[VZ + 8],
+                                       const float (&dz)[VZ + 1], int r)
+{
+    constexpr int P0 = (0 + OFF) % 9, P1 = (1 + OFF) % 9, P2 = (2 + OFF) % 9,
+                  P3 = (3 + OFF) % 9, P4 = (4 + OFF) % 9, P5 = (5 + OFF) % 9,
+                  P6 = (6 + OFF) % 9, P7 = (7 + OFF) % 9, P8 = (8 + OFF) % 9;
+    const float p = q[P4][r];
+    const float s1 = ((q[P3][r] - p) + (q[P5][r] - p)) + (dz[r + 1] - dz[r]);
+    const float s2 = fmaf(-4.f, p, (q[P2][r] + q[P6][r]) + (ext[r + 2] + ext[r + 6]));
+    const float s3 = fmaf(-4.f, p, (q[P1][r] + q[P7][r]) + (ext[r + 1] + ext[r + 7]));
+    const float s4 = fmaf(-4.f, p, (q[P0][r] + q[P8][r]) + (ext[r + 0] + ext[r + 8]));
+    return fmaf(kC4, s1, fmaf(-kC3, s2, fmaf(kC2, s3, -kC1 * s4)));
+}
+template <int VZ>
+__device__ __forceinline__ void zdiff(float (&dz)[VZ + 1], const float (&ext)[VZ + 8])
+{
+#pragma unroll
+    for (int m = 0; m <= VZ; ++m) dz[m] = ext[m + 4] - ext[m + 3];
+}
+
 // acc + a*b with the product rounded on its own (no FMA): the one-step and
 // two-step kernels then accumulate images and illumination identically.
 __device__ __forceinline__ float mul_nc(float a, float b)
@@ -830,7 +856,6 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
     const int x1 = min(x0 + xw, nxx - 4);
     // odd chunks march right to left: neighbouring chunks then read their
     // shared halo columns at the same time (column c of the march is PC(c))
-    // odd chunks march right to left (column c of the march is PC(c))
     const bool back = (id.xc & 1) != 0;
     const int mir = x0 + x1 - 1;
     auto PC = [&](int c) { return back ? mir - c : c; };
@@ -898,20 +923,23 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
                 bool in;
                 dxj = staged ? st.dx[pj - xa] : pad_factor(a.b1d, pj, nxx, g.nb, in);
             }
-            float ext[VZ + 8];
+            float ext[VZ + 8], dz[VZ + 1];
             zext<VZ>(ext, q1[C4], h);
             h = ldhaloc<VZ>(cur, PC(j + 1), nxx, ldz, zs, lane);
+            zdiff<VZ>(dz, ext);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
-                float val = leapfrog<ABC>(q1[C4][r], o1[r], cj[r], lap8r<VZ, C0>(q1, ext, r), d);
+                float val = leapfrog<ABC>(q1[C4][r], o1[r], cj[r], lap8z<VZ, C0>(q1, ext, dz, r), d);
                 q2[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o1[r];
             }
             add_source<VZ>(q2[C8], pj, six, zb, siz, w1);
         }
-        // ---- step 2: u_{t+2} at column jj = j - 4 (columns before x0 - 4 are not used)
-        {
+        // ---- step 2: u_{t+2} at column jj = j - 4, from x0 - 4 on (step 3
+        // reads no column before it; the queue slots of the skipped columns
+        // are unused)
+        if (j - 4 >= x0 - 4) {
             const int jj = j - 4, pjj = PC(jj);
             const bool jin = (pjj >= 4) && (pjj < nxx - 4);
             float dxj = 1.f;
@@ -921,16 +949,17 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
                 dxj = staged ? st.dx[min(max(pjj - xa, 0), x1 + 7 - xa)]
                              : pad_factor(a.b1d, pjj, nxx, g.nb, in);
             }
-            float ext[VZ + 8], c2v[VZ];
+            float ext[VZ + 8], c2v[VZ], dz[VZ + 1];
             zext_local<VZ>(ext, q2[C4]);
             vload<VZ>(c2v, &cring[C4][lane * VZ]);
+            zdiff<VZ>(dz, ext);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float o2 = q1[C0][r];             // u_t at column jj
                 const float c2 = c2v[r];
                 const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
-                float val = leapfrog<ABC>(q2[C4][r], o2, c2, lap8r<VZ, C0>(q2, ext, r), d);
+                float val = leapfrog<ABC>(q2[C4][r], o2, c2, lap8z<VZ, C0>(q2, ext, dz, r), d);
                 q3[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o2;
             }
             add_source<VZ>(q3[C8], pjj, six, zb, siz, w2);
@@ -947,15 +976,16 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
             }
             float c3[VZ];
             vload<VZ>(c3, &cring[C0][lane * VZ]);
-            float ext[VZ + 8];
+            float ext[VZ + 8], dz[VZ + 1];
             zext_local<VZ>(ext, q3[C4]);
+            zdiff<VZ>(dz, ext);
             float n3[VZ];
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float o3 = q2[C0][r];             // u_{t+1} at column jjj
                 const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
-                float val = leapfrog<ABC>(q3[C4][r], o3, c3[r], lap8r<VZ, C0>(q3, ext, r), d);
+                float val = leapfrog<ABC>(q3[C4][r], o3, c3[r], lap8z<VZ, C0>(q3, ext, dz, r), d);
                 n3[r] = ((i >= 4) & (i < g.nzz - 4)) ? val : o3;
             }
             add_source<VZ>(n3, pjjj, six, zb, siz, w3);
@@ -1379,14 +1409,17 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
             zext<VZ>(er, qr1[C4], hr);
             hs = ldhaloc<VZ>(scur, PC(j + 1), nxx, ldz, zs, lane);
             hr = ldhaloc<VZ>(rcur, PC(j + 1), nxx, ldz, zs, lane);
+            float dzs[VZ + 1], dzr[VZ + 1];
+            zdiff<VZ>(dzs, es);
+            zdiff<VZ>(dzr, er);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const bool inside = jin1 && (i >= 4) && (i < g.nzz - 4);
                 const float ps = qs1[C4][r], pr = qr1[C4][r];
-                const float lapS = lap8r<VZ, C0>(qs1, es, r);
+                const float lapS = lap8z<VZ, C0>(qs1, es, dzs, r);
                 const float sn = fmaf(cj[r], lapS, ps + (ps - os[r]));
-                const float rn = fmaf(cj[r], lap8r<VZ, C0>(qr1, er, r), pr + (pr - orr[r]));
+                const float rn = fmaf(cj[r], lap8z<VZ, C0>(qr1, er, dzr, r), pr + (pr - orr[r]));
                 qs2[C8][r] = inside ? sn : os[r];
                 qr2[C8][r] = inside ? rn : orr[r];
                 if (inside) {
@@ -1400,14 +1433,18 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 else inject_receivers<VZ>(a, qr2[C8], pj, zb, dat, a.nt - 2 - k);
             }
         }
-        // ---- step 2 at column jj = j - 4 (columns before x0 - 4 are not used)
+        // ---- step 2 at column jj = j - 4, from x0 - 4 on (step 3 reads no
+        // column before it; the queue slots of the skipped columns are unused)
         const int jj = j - 4, pjj = PC(jj);
         const bool jin2 = (pjj >= 4) && (pjj < nxx - 4);
-        {
+        if (jj >= x0 - 4) {
             float es[VZ + 8], er[VZ + 8], c2v[VZ];
             zext_local<VZ>(es, qs2[C4]);
             zext_local<VZ>(er, qr2[C4]);
             vload<VZ>(c2v, &cring[C4][lane * VZ]);
+            float dzs[VZ + 1], dzr[VZ + 1];
+            zdiff<VZ>(dzs, es);
+            zdiff<VZ>(dzr, er);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
@@ -1416,9 +1453,9 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 const float os2 = qs1[C0][r];           // S_cur at column jj
                 const float or2 = qr1[C0][r];           // R_cur at column jj
                 const float ps = qs2[C4][r], pr = qr2[C4][r];
-                const float lapS = lap8r<VZ, C0>(qs2, es, r);
+                const float lapS = lap8z<VZ, C0>(qs2, es, dzs, r);
                 const float sn = fmaf(c2, lapS, ps + (ps - os2));
-                const float rn = fmaf(c2, lap8r<VZ, C0>(qr2, er, r), pr + (pr - or2));
+                const float rn = fmaf(c2, lap8z<VZ, C0>(qr2, er, dzr, r), pr + (pr - or2));
                 qs3[C8][r] = inside ? sn : os2;
                 qr3[C8][r] = inside ? rn : or2;
                 if (inside) {
@@ -1427,7 +1464,7 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                                                               : mul_nc(pr, mul_nc(c2, lapS)));
                 }
             }
-            if (inj2 && jin2 && jj >= x0 - 4) {
+            if (inj2 && jin2) {
                 if constexpr (staged) inject_staged<VZ, 1>(st, a.idh2, qr3[C8], pjj - xa, zb);
                 else inject_receivers<VZ>(a, qr3[C8], pjj, zb, dat, a.nt - 3 - k);
             }
@@ -1446,6 +1483,9 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
             zext_local<VZ>(es, qs3[C4]);
             zext_local<VZ>(er, qr3[C4]);
             float ns3[VZ], nr3[VZ], fa[VZ], fi[VZ];
+            float dzs[VZ + 1], dzr[VZ + 1];
+            zdiff<VZ>(dzs, es);
+            zdiff<VZ>(dzr, er);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
@@ -1453,9 +1493,9 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
                 const float os3 = qs2[C0][r];           // step-1 S at column jjj
                 const float or3 = qr2[C0][r];           // step-1 R at column jjj
                 const float ps = qs3[C4][r], pr = qr3[C4][r];
-                const float lapS = lap8r<VZ, C0>(qs3, es, r);
+                const float lapS = lap8z<VZ, C0>(qs3, es, dzs, r);
                 const float sn = fmaf(c3[r], lapS, ps + (ps - os3));
-                const float rn = fmaf(c3[r], lap8r<VZ, C0>(qr3, er, r), pr + (pr - or3));
+                const float rn = fmaf(c3[r], lap8z<VZ, C0>(qr3, er, dzr, r), pr + (pr - or3));
                 ns3[r] = inside ? sn : os3;
                 nr3[r] = inside ? rn : or3;
                 fi[r] = pi[C0][r];
