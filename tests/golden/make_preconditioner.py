@@ -5,6 +5,7 @@ project's own kernels on plain-FWI samples.
     python tests/golden/make_preconditioner.py mid     (GPU, ~8 minutes)
     python tests/golden/make_preconditioner.py mid96   (GPU, ~15 minutes)
     python tests/golden/make_preconditioner.py config4 (GPU, ~16 minutes)
+    python tests/golden/make_preconditioner.py config4_256 (GPU, ~60 minutes)
 
 small -> preconditioner_48models.bin: 48 seeded training models of the
 config-4 family at 128 x 384 (10 m, 16 shots), 8 plain FWI iterations each
@@ -29,6 +30,16 @@ stated grid and acquisition (256 x 768 at 10 m, 96 shots, 768 receivers,
 on 64 of its 256 models (seeds 1-64: what one 20-minute GPU session
 generates), 40 epochs of Adam at lr 1e-3, batch 16, mirrored samples
 (`python -m fwi2d_amd.learned --train-size full --train-models 64 --runs ""`).
+
+config4_256 -> preconditioner_config4_256.bin: all 256 models of the
+training configuration, learned in four phases of 64 models (seeds 1-64,
+65-128, 129-192, 193-256), one GPU session each: each phase generates its
+512 samples and runs 40 epochs of Adam at batch 16 from the previous phase's
+network (`python -m fwi2d_amd.learned --train-size full --train-models 64
+--train-first-seed S --continue-from STATE --runs ""`).  Phase 1 is the
+config4 fixture; phase 2 started from its parameters with fresh Adam moments
+(that fixture kept none), phases 3 and 4 from the full state of the phase
+before.  5120 Adam steps in all, as 40 epochs over 2048 samples would take.
 """
 import os
 import sys
@@ -39,7 +50,28 @@ sys.path.insert(0, ROOT)
 from fwi2d_amd import learned as L  # noqa: E402
 
 
+def phased(name):
+    """config4_256: four phases of 64 models, as described above."""
+    import tempfile
+    here = os.path.dirname(os.path.abspath(__file__))
+    state = None
+    with tempfile.TemporaryDirectory() as tmp:
+        for phase, first in enumerate((1, 65, 129, 193)):
+            x, y, g, _ = L.make_samples(L.TRAIN_FULL, range(first, first + 64), log=print)
+            path = os.path.join(tmp, f"phase{phase + 1}.npz")
+            params, hist, val = L.train(x, y, epochs=40, lr=1e-3, flip=True, log=print,
+                                        init=state, state_out=path)
+            print(f"phase {phase + 1}: held-out cosine (learned, plain):",
+                  L.holdout_cosines(params, x, y, g, val))
+            # phase 2 starts from phase 1's parameters with fresh moments, the
+            # later phases from the full state of the phase before
+            state = {"params": params} if phase == 0 else L.load_state(path)
+    params.tofile(os.path.join(here, name))
+
+
 def main(size):
+    if size == "config4_256":
+        return phased("preconditioner_config4_256.bin")
     setup, flip, name, models = {
         "small": (L.TRAIN_SMALL, False, "preconditioner_48models.bin", 48),
         "mid": (L.TRAIN_MID, True, "preconditioner_mid48.bin", 48),
