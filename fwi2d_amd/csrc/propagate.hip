@@ -12,7 +12,9 @@
 //    from memory about once per step (plus an 8-column warm-up per chunk);
 //  * the +-4 z-neighbours come from the neighbouring lanes by DPP whole-wave
 //    shifts; the 4 rows beyond each end of the wave come from one prefetched
-//    float4 "halo" load per column;
+//    float4 "halo" load per column (forward kernels), or, in the three-step
+//    reverse kernel, from one float per lane loaded beside the column's main
+//    load (mostly the same cache lines) and kept with it in the queue;
 //  * one, two or three time steps per launch (temporal blocking): each step
 //    after the first is computed on 8 fewer rows and columns than the one
 //    before, from a register queue of the previous step's values; c, read
@@ -378,6 +380,36 @@ __device__ __forceinline__ float4 ldhaloc(const float *base, int col, int nxx, i
     col = min(max(col, 4), nxx - 5);
     const unsigned off = lane < 32 ? 0u : unsigned(64 * VZ + 4);
     return *reinterpret_cast<const float4 *>((base + (long)col * ldz + (z0 - 4)) + off);
+}
+
+// The eight halo rows of ldhaloc's column in one VGPR: lanes 0-3 hold rows
+// z0-4 .. z0-1 and lanes 60-63 rows z0+64*VZ .. z0+64*VZ+3 (the lanes in
+// between repeat them), the addresses ldhaloc's float4 covers.
+template <int VZ>
+__device__ __forceinline__ float ldhalo1(const float *base, int col, int nxx, int ldz, int z0,
+                                         int lane)
+{
+    col = min(max(col, 4), nxx - 5);
+    const unsigned off = (lane < 32 ? 0u : unsigned(64 * VZ + 4)) + unsigned(lane & 3);
+    return (base + (long)col * ldz + (z0 - 4))[off];
+}
+
+// Lane 4q + K's value in every lane of quad q (DPP quad_perm [K,K,K,K]): for
+// a halo of ldhalo1, lane 0 receives row K above the wave and lane 63 row K
+// below it, the two lanes whose edge values zext reads.
+template <int K>
+__device__ __forceinline__ float quad_bcast(float h)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(h), K * 0x55, 0xF, 0xF, false));
+}
+
+// zext from a halo of ldhalo1.
+template <int VZ>
+__device__ __forceinline__ void zext1(float (&ext)[VZ + 8], const float (&own)[VZ], float hv)
+{
+    zext<VZ>(ext, own,
+             make_float4(quad_bcast<0>(hv), quad_bcast<1>(hv), quad_bcast<2>(hv),
+                         quad_bcast<3>(hv)));
 }
 
 // Receivers in column j whose row is in this lane: seis[rec][tsamp] = d * v[row].
@@ -892,10 +924,14 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
         vzero<VZ>(q2[m]);
         vzero<VZ>(q3[m]);
     }
-    float on[VZ], cn[VZ], pn[VZ];
+    // halos (ldhalo1) of the u_t queue columns, in the queue's slots
+    float hq[9];
+#pragma unroll
+    for (int m = 4; m < 9; ++m) hq[m] = ldhalo1<VZ>(cur, PC(x0 - 12 + m), nxx, ldz, zs, lane);
+    hq[0] = hq[1] = hq[2] = hq[3] = 0.f;
+    float on[VZ], cn[VZ], pn[VZ], hn;
     ldcolc<VZ>(on, old, PC(x0 - 8), nxx, ldz, zl);
     ldcolc<VZ>(cn, cc, PC(x0 - 8), nxx, ldz, zl);
-    float4 h = ldhaloc<VZ>(cur, PC(x0 - 8), nxx, ldz, zs, lane);
     __shared__ float cring[9][64 * VZ];              // c of the last nine columns
     const int jend = x1 + 8;
     int j = x0 - 8;
@@ -912,6 +948,7 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
         for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
         vstore<VZ>(&cring[C8][lane * VZ], cj);
         ldcolc<VZ>(pn, cur, PC(j + 5), nxx, ldz, zl);
+        hn = ldhalo1<VZ>(cur, PC(j + 5), nxx, ldz, zs, lane);   // beside its main load
         ldcolc<VZ>(on, old, PC(j + 1), nxx, ldz, zl);
         ldcolc<VZ>(cn, cc, PC(j + 1), nxx, ldz, zl);
         // ---- step 1: u_{t+1} at column j
@@ -924,8 +961,7 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
                 dxj = staged ? st.dx[pj - xa] : pad_factor(a.b1d, pj, nxx, g.nb, in);
             }
             float ext[VZ + 8], dz[VZ + 1];
-            zext<VZ>(ext, q1[C4], h);
-            h = ldhaloc<VZ>(cur, PC(j + 1), nxx, ldz, zs, lane);
+            zext1<VZ>(ext, q1[C4], hq[C4]);
             zdiff<VZ>(dz, ext);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
@@ -1005,6 +1041,7 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
         // column j + 5 of u_t replaces column j - 4
 #pragma unroll
         for (int r = 0; r < VZ; ++r) q1[C0][r] = pn[r];
+        hq[C0] = hn;
     };
     for (;;) {
         column(Phase<0>{}); if (++j >= jend) break;
@@ -1365,13 +1402,21 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
         vzero<VZ>(qs3[m]); vzero<VZ>(qr3[m]);
         vzero<VZ>(pa[m]); vzero<VZ>(pi[m]);
     }
+    // halos (ldhalo1) of the S_cur / R_cur queue columns, in the queues' slots
+    float hqs[9], hqr[9];
+#pragma unroll
+    for (int m = 4; m < 9; ++m) {
+        hqs[m] = ldhalo1<VZ>(scur, PC(x0 - 12 + m), nxx, ldz, zs, lane);
+        hqr[m] = ldhalo1<VZ>(rcur, PC(x0 - 12 + m), nxx, ldz, zs, lane);
+    }
+    hqs[0] = hqs[1] = hqs[2] = hqs[3] = 0.f;
+    hqr[0] = hqr[1] = hqr[2] = hqr[3] = 0.f;
     float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ], an[VZ], in[VZ];
+    float hsn, hrn;
     ldcolc<VZ>(osn, sold, PC(x0 - 8), nxx, ldz, zl);
     ldcolc<VZ>(orn, rold, PC(x0 - 8), nxx, ldz, zl);
     ldcolc<VZ>(cn, cc, PC(x0 - 8), nxx, ldz, zl);
     vzero<VZ>(an); vzero<VZ>(in);
-    float4 hs = ldhaloc<VZ>(scur, PC(x0 - 8), nxx, ldz, zs, lane);
-    float4 hr = ldhaloc<VZ>(rcur, PC(x0 - 8), nxx, ldz, zs, lane);
     __shared__ float cring[9][64 * VZ];              // c of the last nine columns
     const int jend = x1 + 8;                         // window [x0 - 8, x1 + 8)
     int j = x0 - 8;
@@ -1393,6 +1438,10 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
         vstore<VZ>(&cring[C8][lane * VZ], cj);
         ldcolc<VZ>(psn, scur, PC(j + 5), nxx, ldz, zl);
         ldcolc<VZ>(prn, rcur, PC(j + 5), nxx, ldz, zl);
+        // the halo rows of column j + 5 beside its main load: mostly the
+        // same cache lines
+        hsn = ldhalo1<VZ>(scur, PC(j + 5), nxx, ldz, zs, lane);
+        hrn = ldhalo1<VZ>(rcur, PC(j + 5), nxx, ldz, zs, lane);
         ldcolc<VZ>(osn, sold, PC(j + 1), nxx, ldz, zl);
         ldcolc<VZ>(orn, rold, PC(j + 1), nxx, ldz, zl);
         ldcolc<VZ>(cn, cc, PC(j + 1), nxx, ldz, zl);
@@ -1405,10 +1454,8 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
         const bool jin1 = (pj >= 4) && (pj < nxx - 4);
         {
             float es[VZ + 8], er[VZ + 8];
-            zext<VZ>(es, qs1[C4], hs);
-            zext<VZ>(er, qr1[C4], hr);
-            hs = ldhaloc<VZ>(scur, PC(j + 1), nxx, ldz, zs, lane);
-            hr = ldhaloc<VZ>(rcur, PC(j + 1), nxx, ldz, zs, lane);
+            zext1<VZ>(es, qs1[C4], hqs[C4]);
+            zext1<VZ>(er, qr1[C4], hqr[C4]);
             float dzs[VZ + 1], dzr[VZ + 1];
             zdiff<VZ>(dzs, es);
             zdiff<VZ>(dzr, er);
@@ -1521,6 +1568,8 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
         // column j + 5 of S_cur, R_cur replaces column j - 4
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { qs1[C0][r] = psn[r]; qr1[C0][r] = prn[r]; }
+        hqs[C0] = hsn;
+        hqr[C0] = hrn;
     };
     for (;;) {
         column(Phase<0>{}); if (++j >= jend) break;

@@ -7,9 +7,11 @@ Checks, for every tile and lane of a launch:
 * the owned rows of the z segments and the owned columns of the x chunks
   cover the updated cells [4, nzz-4) x [4, nxx-4) exactly once;
 * every load and store (register-queue columns, clamped to the grid; the
-  float4 halo rows, whose column is clamped to [4, nxx-5]; image and
-  illumination of owned columns; stores of owned cells) falls inside its
-  allocation [0, slots * fstride + 1024 guard floats).
+  halo rows, whose column is clamped to [4, nxx-5]: a float4 per lane in
+  the forward kernel, one float per lane (lanes 0-3 above the wave, 60-63
+  below it) loaded beside each queue column in the reverse kernel; image
+  and illumination of owned columns; stores of owned cells) falls inside
+  its allocation [0, slots * fstride + 1024 guard floats).
 
 The halo clamp is the case this pins: the first z segment starts at row -4,
 so its halo rows -8 .. -5 lie in the previous column, which exists only for
@@ -59,6 +61,7 @@ def accesses(nzz, nxx, vz, xw, reverse, halo_lo=4):
     for zs, zb, own, x0, x1, back in tiles(nzz, nxx, vz, xw):
         zl = np.maximum(zb, 0)
         hz = np.where(lanes < 32, zs - 4, zs + 64 * vz)
+        hz1 = np.where(lanes < 32, zs - 4, zs + 64 * vz) + (lanes & 3)
         span = lambda c0, c1, rows, n: (c0 * ldz + rows.min(), c1 * ldz + rows.max() + n - 1)
         # the march's column range [a, b] as grid columns (mirrored for odd chunks)
         pc = lambda c: (x0 + x1 - 1 - c) if back else c
@@ -68,7 +71,11 @@ def accesses(nzz, nxx, vz, xw, reverse, halo_lo=4):
         c0, c1 = cols(x0 - 8, x1 + 8)
         out.append(("field",) + span(cl(c0), cl(c1), zl, vz))              # u_old
         out.append(("c",) + span(cl(c0), cl(c1), zl, vz))                 # coefficients
-        out.append(("field",) + span(ch(c0), ch(c1), hz, 4))               # float4 halo
+        if reverse:
+            c0, c1 = cols(x0 - 8, x1 + 12)                                 # halo of each queue column
+            out.append(("field",) + span(ch(c0), ch(c1), hz1, 1))          # one float per lane
+        else:
+            out.append(("field",) + span(ch(c0), ch(c1), hz, 4))           # float4 halo
         if reverse:
             out.append(("field",) + span(x0, x1 - 1, zl, vz))              # image, illum
         if own.any():
