@@ -205,6 +205,7 @@ fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g, int kind)
 bool staging_fits(const GridBuf &gb, const fwi::Tiling &t, int steps, bool tables)
 {
     const int h = fwi::window_halo(steps);
+    const int maxrec = steps == 4 ? fwi::kStageRec4 : fwi::kStageRec;
     if (t.xw + 2 * h > fwi::kStageCols) return false;
     if (!tables || gb.h_col_ptr.empty()) return true;
     const int nxx = gb.g.nxx;
@@ -212,7 +213,7 @@ bool staging_fits(const GridBuf &gb, const fwi::Tiling &t, int steps, bool table
         const int x0 = 4 + xc * t.xw, x1 = std::min(x0 + t.xw, nxx - 4);
         const int ca = std::min(std::max(x0 - h, 0), nxx);
         const int cb = std::min(std::max(x1 + h, 0), nxx);
-        if (gb.h_col_ptr[cb] - gb.h_col_ptr[ca] > fwi::kStageRec) return false;
+        if (gb.h_col_ptr[cb] - gb.h_col_ptr[ca] > maxrec) return false;
     }
     return true;
 }
@@ -316,13 +317,17 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
     const int kind = fa ? (fa->abc ? 2 : 0) : 1;    // tuning slot
     const int pkind = fa ? 0 : 1;                   // profile slot: forward / reverse
     const fwi::Tiling tl = tiling_of(c, g, kind);
-    const int per = c->nsteps[kind];
-    // the remainder of a three-step pass runs as a two- or one-step launch,
-    // whose waves own more rows each
-    fwi::Tiling tls[4] = {tl, tl, tl, tl};
-    const GridBuf &gbuf = c->grid[kind == 2 ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC];
+    // four steps per launch: random-boundary forward passes without
+    // receivers and imaging reverse passes (a pass that records, and a
+    // gradient pass, run three-step launches)
     const bool tables = fa ? (fa->seis != nullptr) : true;
-    for (int n = 1; n <= 3; ++n) {
+    const bool four = fa ? !tables : (ra->mode == 0);
+    const int per = (c->nsteps[kind] == 4 && !four) ? 3 : c->nsteps[kind];
+    // the remainder of a multi-step pass runs as shorter launches, whose
+    // waves own more rows each
+    fwi::Tiling tls[5] = {tl, tl, tl, tl, tl};
+    const GridBuf &gbuf = c->grid[kind == 2 ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC];
+    for (int n = 1; n <= 4; ++n) {
         tls[n].nzseg = fwi::zsegments(g.nzz, tl.vz, n);
         tls[n].staged = staging_fits(gbuf, tls[n], n, tables);
     }
@@ -332,7 +337,10 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
         Quad s = s0, r = r0;
         launches = 0;
         for (int t = 0; t < T;) {
-            const int n = (per == 3 && t + 3 <= T) ? 3 : (per >= 2 && t + 2 <= T) ? 2 : 1;
+            const int n = (per == 4 && t + 4 <= T)   ? 4
+                          : (per >= 3 && t + 3 <= T) ? 3
+                          : (per >= 2 && t + 2 <= T) ? 2
+                                                     : 1;
             int f1 = 0, f2 = 0, h1 = 0, h2 = 0;
             s.free_pair(&f1, &f2);
             r.free_pair(&h1, &h2);
@@ -1202,7 +1210,9 @@ int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_
     if (!c) return fail(FWI_ERR_ARG, "null context");
     if (kernel < -1 || kernel > 2) return fail(FWI_ERR_ARG, "kernel must be -1, 0, 1 or 2");
     if (vz > 0 && vz != 1 && vz != 2 && vz != 4) return fail(FWI_ERR_ARG, "vz must be 1, 2 or 4");
-    if (steps > 3) return fail(FWI_ERR_ARG, "steps per launch must be 1, 2 or 3");
+    if (steps > 4) return fail(FWI_ERR_ARG, "steps per launch must be 1, 2, 3 or 4");
+    if (steps == 4 && kernel != 0 && kernel != 1)
+        return fail(FWI_ERR_ARG, "four steps per launch: kernels 0 and 1 only");
     int ns[3], vs[3], ws[3];
     for (int k = 0; k < 3; ++k) {
         const bool sel = kernel == -1 || kernel == k;
@@ -1210,8 +1220,8 @@ int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_
         vs[k] = (sel && vz > 0) ? vz : c->vz[k];
         ws[k] = (sel && xw > 0) ? xw : c->xw[k];
     }
-    if (ns[1] == 3 && vs[1] == 4)
-        return fail(FWI_ERR_ARG, "the three-step reverse kernel takes 1 or 2 rows per lane");
+    if (ns[1] >= 3 && vs[1] == 4)
+        return fail(FWI_ERR_ARG, "the three- and four-step reverse kernels take 1 or 2 rows per lane");
     for (int k = 0; k < 3; ++k) {
         c->nsteps[k] = ns[k];
         c->vz[k] = vs[k];
@@ -1261,9 +1271,9 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
         const long cells = (long)nslots * gb.g.nzz * gb.g.nxx;
         const int ncols = gb.g.nxx - 8;
         int bst = c->nsteps[kind], bvz = c->vz[kind], bxw = c->xw[kind];
-        for (int st = 1; st <= 3; ++st)
+        for (int st = 1; st <= (kind != 2 ? 4 : 3); ++st)
             for (int a = 0; a < 3; ++a) {
-                if (kind == 1 && st == 3 && vzs[a] == 4) continue;
+                if (kind == 1 && st >= 3 && vzs[a] == 4) continue;
                 // fixed widths, narrow ones for small batches (more waves in
                 // flight), and widths that make the launch about 1-4 full
                 // rounds of the waves the device holds at once

@@ -22,7 +22,8 @@ struct Grid {
 
 // Wave tiling of one launch: each 64-lane wave computes 64*vz rows of one
 // x-chunk of xw columns of one shot slot and writes the inner
-// 64*vz - 8*(steps-1) of them (steps per launch: 1, 2 or 3).
+// 64*vz - 8*(steps-1) of them (steps per launch: 1, 2, 3, or 4 for the
+// random-boundary forward without receivers and the imaging reverse).
 struct Tiling {
     int vz;            // rows per lane: 1, 2 or 4
     int xw;            // columns per wave
@@ -35,6 +36,9 @@ struct Tiling {
 // a launch of s steps has the window [x0 - halo, x1 + halo), halo = 4(s-1).
 constexpr int kStageCols = 160;
 constexpr int kStageRec = 256;
+// the four-step reverse kernel's table (its LDS also holds 13 columns of c and
+// 9 of image partial sums per wave)
+constexpr int kStageRec4 = 176;
 inline int window_halo(int steps) { return 4 * (steps - 1); }
 
 // Receivers of one grid, sorted by column (CSR over columns).
@@ -46,7 +50,7 @@ struct RecTable {
 };
 
 // One forward launch at step t.  Single step: reads cur = u_t and
-// old = u_{t-1}, writes u_{t+1} over old (out1 == old).  s = 2 or 3 steps:
+// old = u_{t-1}, writes u_{t+1} over old (out1 == old).  s = 2, 3 or 4 steps:
 // reads cur, old; writes u_{t+s-1} to out1 and u_{t+s} to out2 (four
 // distinct buffers).
 struct FwdStep {
@@ -68,7 +72,7 @@ struct FwdStep {
 
 // One reverse launch at reverse step k.  S: cur = u_{T-1-k}, old = u_{T-k};
 // R: cur = R_k, old = R_{k-1}.  Single step writes over the olds; a launch
-// of s = 2 or 3 steps writes steps k+s-1 / k+s to the out buffers.
+// of s = 2, 3 or 4 steps writes steps k+s-1 / k+s to the out buffers.
 struct RevStep {
     Grid g;
     const float *scur, *sold;
@@ -87,7 +91,8 @@ struct RevStep {
     int mode;                  // 0 image, 1 gradient
 };
 
-// steps = 1 or 2 time steps per launch
+// steps = 1-4 time steps per launch (4: random boundary, no receivers; the
+// reverse: imaging mode, vz 1 or 2)
 hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
                            hipStream_t s);
 hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,

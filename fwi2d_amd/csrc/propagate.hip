@@ -12,15 +12,17 @@
 //    from memory about once per step (plus an 8-column warm-up per chunk);
 //  * the +-4 z-neighbours come from the neighbouring lanes by DPP whole-wave
 //    shifts; the 4 rows beyond each end of the wave come from one prefetched
-//    float4 "halo" load per column (forward kernels), or, in the three-step
-//    reverse kernel, from one float per lane loaded beside the column's main
-//    load (mostly the same cache lines) and kept with it in the queue;
-//  * one, two or three time steps per launch (temporal blocking): each step
+//    float4 "halo" load per column (one- and two-step kernels), or from one
+//    float per lane loaded beside the column's main load (mostly the same
+//    cache lines) and kept with it in the queue (three- and four-step);
+//  * one to four time steps per launch (temporal blocking): each step
 //    after the first is computed on 8 fewer rows and columns than the one
 //    before, from a register queue of the previous step's values; c, read
 //    for the first step, waits in an LDS ring for the later ones; a launch
-//    writes only the two fields the next one needs;
-//  * the three-step kernels unroll their column loop over the nine phases
+//    writes only the two fields the next one needs, so the HBM bytes of a
+//    launch do not grow with its steps (four steps: the random-boundary
+//    forward without receivers and the imaging reverse);
+//  * the three- and four-step kernels unroll their column loop over the nine phases
 //    of the queues, so the queues advance by renaming, not by moves, and
 //    their loads are branch-free (columns clamped into the grid);
 //  * source injection, the Cerjan sponge, receiver sampling (forward) and
@@ -498,7 +500,7 @@ __device__ __forceinline__ bool stage_table(const RecTable &rt, int xa, int nc, 
     const int ca = min(max(xa, 0), nxx), cb = min(max(xa + nc, 0), nxx);
     e0 = rt.col_ptr[ca];
     ne = rt.col_ptr[cb] - e0;
-    if (nc > kStageCols || ne > kStageRec) return false;
+    if (nc > kStageCols || ne > int(sizeof(st.iz) / sizeof(int))) return false;
     for (int c = lane; c <= nc; c += 64) st.cp[c] = rt.col_ptr[min(max(xa + c, 0), nxx)] - e0;
     for (int e = lane; e < ne; e += 64) st.iz[e] = rt.iz[e0 + e];
     return true;
@@ -550,6 +552,38 @@ __device__ __forceinline__ bool stage_rev(const RevStep &a, const float *dat, in
     return ok;
 }
 
+// The four-step reverse kernel's table: data rows of steps 1-4, at most
+// kStageRec4 receivers (its LDS also holds 13 columns of c and 9 of image
+// partial sums).
+struct RevStage4 {
+    int cp[kStageCols + 1];
+    int iz[kStageRec4];
+    float v[4][kStageRec4];
+};
+
+__device__ __forceinline__ bool stage_rev4(const RevStep &a, const float *dat, int xa, int nc,
+                                           int zlo, int zhi, int k, RevStage4 &st, int lane)
+{
+    int e0 = 0, ne = 0;
+    const bool ok = (k + 1 < a.nt) && stage_table(a.rec, xa, nc, a.g.nxx, st, lane, e0, ne);
+    if (ok) {
+        for (int e = lane; e < ne; e += 64) {
+            const int iz = st.iz[e];        // written by this lane
+            float v[4] = {0.f, 0.f, 0.f, 0.f};
+            if (iz >= zlo && iz < zhi) {
+                const float *row = dat + (long)a.rec.id[e0 + e] * a.nt;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (k + 1 + q < a.nt) v[q] = row[a.nt - 2 - k - q];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) st.v[q][e] = v[q];
+        }
+    }
+    __syncthreads();
+    return ok;
+}
+
 // seis[rec][tsamp] = d * v for the receivers of window column c in this
 // lane, d the sponge factor of the cell (dx of the column, dz - 1 of the
 // row) or 1.
@@ -595,8 +629,8 @@ __device__ __forceinline__ void sample3_staged(const FwdStage &st, int nt, const
 // Adds idh2 * data to the receivers of window column c in this lane, in
 // table order.  Branch-free: every entry takes one FMA per row, with data 0
 // for the rows that are not its own (an exact + 0).
-template <int VZ, int STEP>
-__device__ __forceinline__ void inject_staged(const RevStage &st, float idh2, float (&v)[VZ], int c,
+template <int VZ, int STEP, class Stage>
+__device__ __forceinline__ void inject_staged(const Stage &st, float idh2, float (&v)[VZ], int c,
                                               int zb)
 {
     const int m0 = st.cp[c], m1 = st.cp[c + 1];
@@ -1042,6 +1076,176 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
 #pragma unroll
         for (int r = 0; r < VZ; ++r) q1[C0][r] = pn[r];
         hq[C0] = hn;
+    };
+    for (;;) {
+        column(Phase<0>{}); if (++j >= jend) break;
+        column(Phase<1>{}); if (++j >= jend) break;
+        column(Phase<2>{}); if (++j >= jend) break;
+        column(Phase<3>{}); if (++j >= jend) break;
+        column(Phase<4>{}); if (++j >= jend) break;
+        column(Phase<5>{}); if (++j >= jend) break;
+        column(Phase<6>{}); if (++j >= jend) break;
+        column(Phase<7>{}); if (++j >= jend) break;
+        column(Phase<8>{}); if (++j >= jend) break;
+    }
+}
+
+// Four time steps per launch on the random-boundary grid without receivers
+// (the source pass of migration and of the gradient): the three-step march
+// with a fourth register queue.  Step s runs on the chunk +- 4(4 - s)
+// columns, on the rows that stay valid without the neighbouring waves' values
+// (64*VZ rows for step 1, 8 fewer for each later step; 64*VZ - 24 owned), so
+// a launch reads u_t, u_{t-1} and c once and writes u_{t+3} and u_{t+4}: the
+// HBM bytes of four steps are those of three in fwd3_kernel.  c of the last
+// thirteen columns waits in LDS for steps 2-4.  Same operations in the same
+// order as every other forward kernel, so the same bits.
+template <int VZ>
+__global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int t)
+{
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int ldz = g.ldz, nxx = g.nxx;
+    const int zs = id.zseg * (64 * VZ - 24) - 8;    // first computed row; owned from zs + 12
+    const int zb = zs + lane * VZ;
+    const int zl = zb < 0 ? 0 : zb;
+    const unsigned uzl = unsigned(zl);
+    const bool active = zb < ldz;
+    const bool own = active && (zb >= zs + 12) && (zb < zs + 64 * VZ - 12);
+    const int x0 = 4 + id.xc * xw;
+    const int x1 = min(x0 + xw, nxx - 4);
+    const bool back = (id.xc & 1) != 0;
+    const int mir = x0 + x1 - 1;
+    auto PC = [&](int c) { return back ? mir - c : c; };
+
+    const long soff = (long)id.slot * g.fstride;
+    const float *cur = a.cur + soff;
+    const float *old = a.old + soff;
+    float *out1 = a.out1 + soff;
+    float *out2 = a.out2 + soff;
+    const float *cc = a.c + (long)id.slot * a.cstride;
+    const int gshot = *a.shot0 + id.slot;
+    const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
+    const float w1 = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
+    const float w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
+    const float w3 = (t + 3 < a.nt) ? a.wav[t + 3] : 0.f;
+    const float w4 = (t + 4 < a.nt) ? a.wav[t + 4] : 0.f;
+
+    float q1[9][VZ];     // u_t
+    float q2[9][VZ];     // u_{t+1}
+    float q3[9][VZ];     // u_{t+2}
+    float q4[9][VZ];     // u_{t+3}
+    float hq[9];         // halos (ldhalo1) of the q1 columns
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        ldcolc<VZ>(q1[m], cur, PC(x0 - 16 + m), nxx, ldz, zl);
+        vzero<VZ>(q2[m]);
+        vzero<VZ>(q3[m]);
+        vzero<VZ>(q4[m]);
+    }
+#pragma unroll
+    for (int m = 4; m < 9; ++m) hq[m] = ldhalo1<VZ>(cur, PC(x0 - 16 + m), nxx, ldz, zs, lane);
+    hq[0] = hq[1] = hq[2] = hq[3] = 0.f;
+    float on[VZ], cn[VZ], pn[VZ], hn;
+    ldcolc<VZ>(on, old, PC(x0 - 12), nxx, ldz, zl);
+    ldcolc<VZ>(cn, cc, PC(x0 - 12), nxx, ldz, zl);
+    __shared__ float cring[13][64 * VZ];             // c of the last thirteen columns
+    const int jend = x1 + 12;                        // window [x0 - 12, x1 + 12)
+    int j = x0 - 12;
+    int cs = 0;                                      // cring slot of column j
+
+    // One column j; slot (m + P) % 9 of a queue holds its column m (q1:
+    // j-4+m; q2: j-8+m; q3: j-12+m; q4: j-16+m once the column's results
+    // are in).  Steps 1-3 keep the old (zero rim) value outside [4, nxx-4).
+    auto column = [&](auto phase) {
+        constexpr int P = decltype(phase)::value;
+        constexpr int C0 = P % 9, C4 = (P + 4) % 9, C8 = (P + 8) % 9;
+        float o1[VZ], cj[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { o1[r] = on[r]; cj[r] = cn[r]; }
+        vstore<VZ>(&cring[cs][lane * VZ], cj);
+        ldcolc<VZ>(pn, cur, PC(j + 5), nxx, ldz, zl);
+        hn = ldhalo1<VZ>(cur, PC(j + 5), nxx, ldz, zs, lane);
+        ldcolc<VZ>(on, old, PC(j + 1), nxx, ldz, zl);
+        ldcolc<VZ>(cn, cc, PC(j + 1), nxx, ldz, zl);
+        const int cs2 = cs >= 4 ? cs - 4 : cs + 9;       // slot of column j - 4
+        const int cs3 = cs >= 8 ? cs - 8 : cs + 5;       // j - 8
+        const int cs4 = cs == 12 ? 0 : cs + 1;           // j - 12
+        // ---- step 1: u_{t+1} at column j
+        {
+            const int pj = PC(j);
+            const bool jin = (pj >= 4) && (pj < nxx - 4);
+            float ext[VZ + 8], dz[VZ + 1];
+            zext1<VZ>(ext, q1[C4], hq[C4]);
+            zdiff<VZ>(dz, ext);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                float val = leapfrog<false>(q1[C4][r], o1[r], cj[r], lap8z<VZ, C0>(q1, ext, dz, r), 1.f);
+                q2[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o1[r];
+            }
+            add_source<VZ>(q2[C8], pj, six, zb, siz, w1);
+        }
+        // ---- step 2: u_{t+2} at column j - 4, from x0 - 8 on
+        if (j - 4 >= x0 - 8) {
+            const int jj = j - 4, pjj = PC(jj);
+            const bool jin = (pjj >= 4) && (pjj < nxx - 4);
+            float ext[VZ + 8], c2v[VZ], dz[VZ + 1];
+            zext_local<VZ>(ext, q2[C4]);
+            vload<VZ>(c2v, &cring[cs2][lane * VZ]);
+            zdiff<VZ>(dz, ext);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const float o2 = q1[C0][r];             // u_t at column jj
+                float val = leapfrog<false>(q2[C4][r], o2, c2v[r], lap8z<VZ, C0>(q2, ext, dz, r), 1.f);
+                q3[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o2;
+            }
+            add_source<VZ>(q3[C8], pjj, six, zb, siz, w2);
+        }
+        // ---- step 3: u_{t+3} at column j - 8, from x0 - 4 on
+        if (j - 8 >= x0 - 4) {
+            const int jjj = j - 8, pjjj = PC(jjj);
+            const bool jin = (pjjj >= 4) && (pjjj < nxx - 4);
+            float ext[VZ + 8], c3v[VZ], dz[VZ + 1];
+            zext_local<VZ>(ext, q3[C4]);
+            vload<VZ>(c3v, &cring[cs3][lane * VZ]);
+            zdiff<VZ>(dz, ext);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const float o3 = q2[C0][r];             // u_{t+1} at column jjj
+                float val = leapfrog<false>(q3[C4][r], o3, c3v[r], lap8z<VZ, C0>(q3, ext, dz, r), 1.f);
+                q4[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o3;
+            }
+            add_source<VZ>(q4[C8], pjjj, six, zb, siz, w3);
+            if (own && jjj >= x0 && jjj < x1) vstore<VZ>((out1 + (long)pjjj * ldz) + uzl, q4[C8]);
+        }
+        // ---- step 4: u_{t+4} at column j - 12 (an owned column)
+        const int j4 = j - 12;
+        if (j4 >= x0) {
+            const int pj4 = PC(j4);
+            float c4v[VZ], ext[VZ + 8], dz[VZ + 1], n4[VZ];
+            vload<VZ>(c4v, &cring[cs4][lane * VZ]);
+            zext_local<VZ>(ext, q4[C4]);
+            zdiff<VZ>(dz, ext);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const float o4 = q3[C0][r];             // u_{t+2} at column j4
+                float val = leapfrog<false>(q4[C4][r], o4, c4v[r], lap8z<VZ, C0>(q4, ext, dz, r), 1.f);
+                n4[r] = ((i >= 4) & (i < g.nzz - 4)) ? val : o4;
+            }
+            add_source<VZ>(n4, pj4, six, zb, siz, w4);
+            if (own) vstore<VZ>((out2 + (long)pj4 * ldz) + uzl, n4);
+        }
+        // column j + 5 of u_t replaces column j - 4
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) q1[C0][r] = pn[r];
+        hq[C0] = hn;
+        cs = (cs == 12) ? 0 : cs + 1;
     };
     for (;;) {
         column(Phase<0>{}); if (++j >= jend) break;
@@ -1584,6 +1788,268 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
     }
 }
 
+// Four reverse steps per launch, imaging mode: the three-step march with a
+// fourth pair of register queues (steps on the chunk +- 12, 8, 4, 0 columns
+// and 64*VZ, -8, -16, -24 rows), so the S, R, c, image and illumination
+// bytes of four steps are those of three.  c of the last thirteen columns
+// waits in LDS.  The image and illumination products of a column are taken
+// two steps at a time from values still in the queues -- steps 1 and 2 when
+// step 2 reaches the column, steps 3 and 4 when step 4 does -- and added to
+// the column's sums in step order, with the partial sums in LDS in between,
+// so the result has the bits of every other launch shape.
+template <int VZ, bool ST>
+__global__ __launch_bounds__(64) void rev4_kernel(RevStep a, int nzseg, int nxchunk, int xw,
+                                                  int nslots, int k)
+{
+    TileId id;
+    if (!tile_of(nzseg, nxchunk, nslots, id)) return;
+    const Grid g = a.g;
+    const int lane = threadIdx.x;
+    const int ldz = g.ldz, nxx = g.nxx;
+    const int zs = id.zseg * (64 * VZ - 24) - 8;    // first computed row; owned from zs + 12
+    const int zb = zs + lane * VZ;
+    const int zl = zb < 0 ? 0 : zb;
+    const unsigned uzl = unsigned(zl);
+    const bool active = zb < ldz;
+    const bool own = active && (zb >= zs + 12) && (zb < zs + 64 * VZ - 12);
+    const int x0 = 4 + id.xc * xw;
+    const int x1 = min(x0 + xw, nxx - 4);
+    const bool back = (id.xc & 1) != 0;
+    const int mir = x0 + x1 - 1;
+    auto PC = [&](int c) { return back ? mir - c : c; };
+
+    const long soff = (long)id.slot * g.fstride;
+    const float *scur = a.scur + soff;
+    const float *sold = a.sold + soff;
+    float *sout1 = a.sout1 + soff;
+    float *sout2 = a.sout2 + soff;
+    const float *rcur = a.rcur + soff;
+    const float *rold = a.rold + soff;
+    float *rout1 = a.rout1 + soff;
+    float *rout2 = a.rout2 + soff;
+    const float *cc = a.c + (long)id.slot * a.cstride;
+    float *acc = a.acc + soff;
+    float *ill = a.ill + soff;
+    const bool inj1 = (k + 1 < a.nt), inj2 = (k + 2 < a.nt);
+    const bool inj3 = (k + 3 < a.nt), inj4 = (k + 4 < a.nt);
+    const int drow = a.data_by_shot ? (*a.shot0 + id.slot) : id.slot;
+    const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
+    const int xa = x0 - 12;                          // window [x0 - 12, x1 + 12)
+    __shared__ RevStage4 st;
+    constexpr bool staged = ST;
+    if constexpr (ST) (void)stage_rev4(a, dat, xa, x1 + 12 - xa, zs, zs + 64 * VZ, k, st, lane);
+
+    float qs1[9][VZ], qr1[9][VZ];    // S_cur, R_cur
+    float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R
+    float qs3[9][VZ], qr3[9][VZ];    // step-2 S, R
+    float qs4[9][VZ], qr4[9][VZ];    // step-3 S, R
+    float hqs[9], hqr[9];            // halos (ldhalo1) of the q1 columns
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        ldcolc<VZ>(qs1[m], scur, PC(x0 - 16 + m), nxx, ldz, zl);
+        ldcolc<VZ>(qr1[m], rcur, PC(x0 - 16 + m), nxx, ldz, zl);
+        vzero<VZ>(qs2[m]); vzero<VZ>(qr2[m]);
+        vzero<VZ>(qs3[m]); vzero<VZ>(qr3[m]);
+        vzero<VZ>(qs4[m]); vzero<VZ>(qr4[m]);
+    }
+#pragma unroll
+    for (int m = 4; m < 9; ++m) {
+        hqs[m] = ldhalo1<VZ>(scur, PC(x0 - 16 + m), nxx, ldz, zs, lane);
+        hqr[m] = ldhalo1<VZ>(rcur, PC(x0 - 16 + m), nxx, ldz, zs, lane);
+    }
+    hqs[0] = hqs[1] = hqs[2] = hqs[3] = 0.f;
+    hqr[0] = hqr[1] = hqr[2] = hqr[3] = 0.f;
+    float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ], an[VZ], in[VZ];
+    float hsn, hrn;
+    ldcolc<VZ>(osn, sold, PC(x0 - 12), nxx, ldz, zl);
+    ldcolc<VZ>(orn, rold, PC(x0 - 12), nxx, ldz, zl);
+    ldcolc<VZ>(cn, cc, PC(x0 - 12), nxx, ldz, zl);
+    vzero<VZ>(an); vzero<VZ>(in);
+    __shared__ float cring[13][64 * VZ];             // c of the last thirteen columns
+    // image / illumination partial sums (steps 1-2) of the columns j-12 .. j-4
+    // in slot (m + P) % 9: the lane's VZ image rows, then its VZ illumination rows
+    __shared__ __align__(16) float pring[9][64 * 2 * VZ];
+    const int jend = x1 + 12;
+    int j = x0 - 12;
+    int cs = 0;                                      // cring slot of column j
+
+    auto column = [&](auto phase) {
+        constexpr int P = decltype(phase)::value;
+        constexpr int C0 = P % 9, C4 = (P + 4) % 9, C8 = (P + 8) % 9;
+        float os[VZ], orr[VZ], cj[VZ], a2[VZ], i2[VZ];
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) {
+            os[r] = osn[r]; orr[r] = orn[r]; cj[r] = cn[r];
+            a2[r] = an[r]; i2[r] = in[r];
+        }
+        vstore<VZ>(&cring[cs][lane * VZ], cj);
+        ldcolc<VZ>(psn, scur, PC(j + 5), nxx, ldz, zl);
+        ldcolc<VZ>(prn, rcur, PC(j + 5), nxx, ldz, zl);
+        hsn = ldhalo1<VZ>(scur, PC(j + 5), nxx, ldz, zs, lane);
+        hrn = ldhalo1<VZ>(rcur, PC(j + 5), nxx, ldz, zs, lane);
+        ldcolc<VZ>(osn, sold, PC(j + 1), nxx, ldz, zl);
+        ldcolc<VZ>(orn, rold, PC(j + 1), nxx, ldz, zl);
+        ldcolc<VZ>(cn, cc, PC(j + 1), nxx, ldz, zl);
+        if (j - 3 >= x0 && j - 3 < x1) {             // sums of the column step 2 reaches next
+            vload<VZ>(an, (acc + (long)PC(j - 3) * ldz) + uzl);
+            vload<VZ>(in, (ill + (long)PC(j - 3) * ldz) + uzl);
+        }
+        const int cs2 = cs >= 4 ? cs - 4 : cs + 9;       // cring slot of column j - 4
+        const int cs3 = cs >= 8 ? cs - 8 : cs + 5;       // j - 8
+        const int cs4 = cs == 12 ? 0 : cs + 1;           // j - 12
+        // ---- step 1 at column j
+        const int pj = PC(j);
+        const bool jin1 = (pj >= 4) && (pj < nxx - 4);
+        {
+            float es[VZ + 8], er[VZ + 8], dzs[VZ + 1], dzr[VZ + 1];
+            zext1<VZ>(es, qs1[C4], hqs[C4]);
+            zext1<VZ>(er, qr1[C4], hqr[C4]);
+            zdiff<VZ>(dzs, es);
+            zdiff<VZ>(dzr, er);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = jin1 && (i >= 4) && (i < g.nzz - 4);
+                const float ps = qs1[C4][r], pr = qr1[C4][r];
+                const float sn = fmaf(cj[r], lap8z<VZ, C0>(qs1, es, dzs, r), ps + (ps - os[r]));
+                const float rn = fmaf(cj[r], lap8z<VZ, C0>(qr1, er, dzr, r), pr + (pr - orr[r]));
+                qs2[C8][r] = inside ? sn : os[r];
+                qr2[C8][r] = inside ? rn : orr[r];
+            }
+            if (inj1 && jin1) {
+                if constexpr (staged) inject_staged<VZ, 0>(st, a.idh2, qr2[C8], pj - xa, zb);
+                else inject_receivers<VZ>(a, qr2[C8], pj, zb, dat, a.nt - 2 - k);
+            }
+        }
+        // ---- step 2 at column jj = j - 4, from x0 - 8 on; the image and
+        // illumination of steps 1 and 2 of the column
+        const int jj = j - 4, pjj = PC(jj);
+        const bool jin2 = (pjj >= 4) && (pjj < nxx - 4);
+        if (jj >= x0 - 8) {
+            float es[VZ + 8], er[VZ + 8], c2v[VZ], dzs[VZ + 1], dzr[VZ + 1], pp[2 * VZ];
+            zext_local<VZ>(es, qs2[C4]);
+            zext_local<VZ>(er, qr2[C4]);
+            vload<VZ>(c2v, &cring[cs2][lane * VZ]);
+            zdiff<VZ>(dzs, es);
+            zdiff<VZ>(dzr, er);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = jin2 && (i >= 4) && (i < g.nzz - 4);
+                const float os2 = qs1[C0][r], or2 = qr1[C0][r];   // S_cur, R_cur at jj
+                const float ps = qs2[C4][r], pr = qr2[C4][r];     // step-1 S, R at jj
+                const float sn = fmaf(c2v[r], lap8z<VZ, C0>(qs2, es, dzs, r), ps + (ps - os2));
+                const float rn = fmaf(c2v[r], lap8z<VZ, C0>(qr2, er, dzr, r), pr + (pr - or2));
+                qs3[C8][r] = inside ? sn : os2;
+                qr3[C8][r] = inside ? rn : or2;
+                pp[r] = a2[r];
+                pp[VZ + r] = i2[r];
+                if (inside) {        // step 1: S_1 R_0, then step 2: S_2 R_1
+                    pp[VZ + r] = add_nc(pp[VZ + r], mul_nc(ps, ps));
+                    pp[r] = add_nc(pp[r], mul_nc(ps, or2));
+                    pp[VZ + r] = add_nc(pp[VZ + r], mul_nc(sn, sn));
+                    pp[r] = add_nc(pp[r], mul_nc(sn, pr));
+                }
+            }
+            vstore<2 * VZ>(&pring[C8][lane * 2 * VZ], pp);
+            if (inj2 && jin2) {
+                if constexpr (staged) inject_staged<VZ, 1>(st, a.idh2, qr3[C8], pjj - xa, zb);
+                else inject_receivers<VZ>(a, qr3[C8], pjj, zb, dat, a.nt - 3 - k);
+            }
+        }
+        // ---- step 3 at column jjj = j - 8, from x0 - 4 on
+        const int jjj = j - 8, pjjj = PC(jjj);
+        const bool jin3 = (pjjj >= 4) && (pjjj < nxx - 4);
+        if (jjj >= x0 - 4) {
+            float es[VZ + 8], er[VZ + 8], c3v[VZ], dzs[VZ + 1], dzr[VZ + 1];
+            zext_local<VZ>(es, qs3[C4]);
+            zext_local<VZ>(er, qr3[C4]);
+            vload<VZ>(c3v, &cring[cs3][lane * VZ]);
+            zdiff<VZ>(dzs, es);
+            zdiff<VZ>(dzr, er);
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = jin3 && (i >= 4) && (i < g.nzz - 4);
+                const float os3 = qs2[C0][r], or3 = qr2[C0][r];   // step-1 S, R at jjj
+                const float ps = qs3[C4][r], pr = qr3[C4][r];
+                const float sn = fmaf(c3v[r], lap8z<VZ, C0>(qs3, es, dzs, r), ps + (ps - os3));
+                const float rn = fmaf(c3v[r], lap8z<VZ, C0>(qr3, er, dzr, r), pr + (pr - or3));
+                qs4[C8][r] = inside ? sn : os3;
+                qr4[C8][r] = inside ? rn : or3;
+            }
+            if (inj3 && jin3) {
+                if constexpr (staged) inject_staged<VZ, 2>(st, a.idh2, qr4[C8], pjjj - xa, zb);
+                else inject_receivers<VZ>(a, qr4[C8], pjjj, zb, dat, a.nt - 4 - k);
+            }
+            if (own && jjj >= x0 && jjj < x1) {
+                vstore<VZ>((sout1 + (long)pjjj * ldz) + uzl, qs4[C8]);
+                vstore<VZ>((rout1 + (long)pjjj * ldz) + uzl, qr4[C8]);
+            }
+        }
+        // ---- step 4 at column j4 = j - 12 (an owned column); the image and
+        // illumination of steps 3 and 4, added to the column's partial sums
+        const int j4 = j - 12;
+        if (j4 >= x0) {
+            const int pj4 = PC(j4);
+            float c4v[VZ], pp[2 * VZ], es[VZ + 8], er[VZ + 8], dzs[VZ + 1], dzr[VZ + 1];
+            vload<VZ>(c4v, &cring[cs4][lane * VZ]);
+            vload<2 * VZ>(pp, &pring[C0][lane * 2 * VZ]);
+            zext_local<VZ>(es, qs4[C4]);
+            zext_local<VZ>(er, qr4[C4]);
+            zdiff<VZ>(dzs, es);
+            zdiff<VZ>(dzr, er);
+            float ns4[VZ], nr4[VZ], fa[VZ], fi[VZ];
+#pragma unroll
+            for (int r = 0; r < VZ; ++r) {
+                const int i = zb + r;
+                const bool inside = (i >= 4) && (i < g.nzz - 4);
+                const float os4 = qs3[C0][r], or4 = qr3[C0][r];   // step-2 S, R at j4
+                const float ps = qs4[C4][r], pr = qr4[C4][r];     // step-3 S, R at j4
+                const float sn = fmaf(c4v[r], lap8z<VZ, C0>(qs4, es, dzs, r), ps + (ps - os4));
+                const float rn = fmaf(c4v[r], lap8z<VZ, C0>(qr4, er, dzr, r), pr + (pr - or4));
+                ns4[r] = inside ? sn : os4;
+                nr4[r] = inside ? rn : or4;
+                fa[r] = pp[r];
+                fi[r] = pp[VZ + r];
+                if (inside) {        // step 3: S_3 R_2, then step 4: S_4 R_3
+                    fi[r] = add_nc(fi[r], mul_nc(ps, ps));
+                    fa[r] = add_nc(fa[r], mul_nc(ps, or4));
+                    fi[r] = add_nc(fi[r], mul_nc(sn, sn));
+                    fa[r] = add_nc(fa[r], mul_nc(sn, pr));
+                }
+            }
+            if (inj4) {
+                if constexpr (staged) inject_staged<VZ, 3>(st, a.idh2, nr4, pj4 - xa, zb);
+                else inject_receivers<VZ>(a, nr4, pj4, zb, dat, a.nt - 5 - k);
+            }
+            if (own) {
+                const long off = (long)pj4 * ldz;
+                vstore<VZ>((sout2 + off) + uzl, ns4);
+                vstore<VZ>((rout2 + off) + uzl, nr4);
+                vstore<VZ>((acc + off) + uzl, fa);
+                vstore<VZ>((ill + off) + uzl, fi);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < VZ; ++r) { qs1[C0][r] = psn[r]; qr1[C0][r] = prn[r]; }
+        hqs[C0] = hsn;
+        hqr[C0] = hrn;
+        cs = (cs == 12) ? 0 : cs + 1;
+    };
+    for (;;) {
+        column(Phase<0>{}); if (++j >= jend) break;
+        column(Phase<1>{}); if (++j >= jend) break;
+        column(Phase<2>{}); if (++j >= jend) break;
+        column(Phase<3>{}); if (++j >= jend) break;
+        column(Phase<4>{}); if (++j >= jend) break;
+        column(Phase<5>{}); if (++j >= jend) break;
+        column(Phase<6>{}); if (++j >= jend) break;
+        column(Phase<7>{}); if (++j >= jend) break;
+        column(Phase<8>{}); if (++j >= jend) break;
+    }
+}
+
 // ---------------------------------------------------------------- helpers
 
 __global__ void fwd_init_kernel(FwdStep a, float *field, int nslots)
@@ -1915,7 +2381,12 @@ template <int VZ, bool ABC, bool ST>
 void fwd_launch(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps, int nblocks,
                 hipStream_t s)
 {
-    if (steps == 3)
+    if (steps == 4) {
+        // random boundary, no receivers (the host checks)
+        if constexpr (!ABC)
+            hipLaunchKernelGGL((fwd4_kernel<VZ>), dim3(nblocks), dim3(64), 0, s, a, tl.nzseg,
+                               tl.nxchunk, tl.xw, nslots, t);
+    } else if (steps == 3)
         hipLaunchKernelGGL((fwd3_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
                            tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
     else if (steps == 2)
@@ -1946,7 +2417,13 @@ template <int VZ, int MODE, bool ST>
 hipError_t rev_launch(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
                       int nblocks, hipStream_t s)
 {
-    if (steps == 3) {
+    if (steps == 4) {
+        // imaging mode, one or two rows per lane (the host checks)
+        if constexpr (VZ == 4 || MODE != 0) return hipErrorInvalidValue;
+        else
+            hipLaunchKernelGGL((rev4_kernel<VZ, ST>), dim3(nblocks), dim3(64), 0, s, a,
+                               tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
+    } else if (steps == 3) {
         // six register queues per lane: four rows per lane do not fit
         if constexpr (VZ == 4) return hipErrorInvalidValue;
         else
@@ -2023,15 +2500,18 @@ template <int VZ>
 int resident_vz(int kind, int steps)
 {
     if (kind == 0)
-        return steps == 3   ? resident_of(fwd3_kernel<VZ, false, true>)
+        return steps == 4   ? resident_of(fwd4_kernel<VZ>)
+               : steps == 3 ? resident_of(fwd3_kernel<VZ, false, true>)
                : steps == 2 ? resident_of(fwd2_kernel<VZ, false, true>)
                             : resident_of(fwd1_kernel<VZ, false, true>);
     if (kind == 2)
         return steps == 3   ? resident_of(fwd3_kernel<VZ, true, true>)
                : steps == 2 ? resident_of(fwd2_kernel<VZ, true, true>)
                             : resident_of(fwd1_kernel<VZ, true, true>);
-    if constexpr (VZ != 4)
+    if constexpr (VZ != 4) {
+        if (steps == 4) return resident_of(rev4_kernel<VZ, true>);
         if (steps == 3) return resident_of(rev3_kernel<VZ, 0, true>);
+    }
     return steps == 2 ? resident_of(rev2_kernel<VZ, 0, true>)
                       : resident_of(rev1_kernel<VZ, 0, true>);
 }

@@ -121,12 +121,24 @@ SHAPES = [(1, 1, 8, 0), (1, 2, 17, 1), (1, 4, 64, 1), (1, 4, 1000, 0), (1, 1, 3,
 # the largest that is staged in LDS
 SHAPES3 = [(3, 1, 8, 0), (3, 2, 17, 1), (3, 4, 64, 1), (3, 4, 1000, 0), (3, 1, 3, 1),
            (3, 2, 5, 0), (3, 2, 144, 1), (3, 1, 145, 0)]
+# four steps per launch: the random-boundary forward kernel of migration and
+# gradient passes and the imaging reverse kernel (the others take three)
+SHAPES4 = [(4, 1, 8, 0), (4, 2, 17, 1), (4, 4, 64, 1), (4, 4, 1000, 0), (4, 2, 130, 1),
+           (4, 1, 3, 0)]
 
 
 def set_shape(eng, st, vz, xw, g):
     """One launch shape for every kernel; the three-step reverse kernel has no
-    four-rows-per-lane form and then takes two steps."""
-    if st == 3 and vz == 4:
+    four-rows-per-lane form and then takes two steps; four steps per launch
+    set the random-boundary forward kernel and, with one or two rows per lane,
+    the reverse kernel (whose gradient passes then take three), and three the
+    others."""
+    if st == 4:
+        set_shape(eng, 3, vz, xw, g)
+        eng.set_tuning(vz, xw, g, kernel=0, steps=4)
+        if vz != 4:
+            eng.set_tuning(vz, xw, g, kernel=1, steps=4)
+    elif st == 3 and vz == 4:
         eng.set_tuning(vz, xw, g, kernel=1, steps=2)
         for k in (0, 2):
             eng.set_tuning(vz, xw, g, kernel=k, steps=3)
@@ -145,11 +157,11 @@ def test_launch_shapes_bit_identical():
         assert np.array_equal(out, base), (st, vz, xw, g)
 
 
-@pytest.mark.parametrize("nt", [998, 999, 1000])
+@pytest.mark.parametrize("nt", [998, 999, 1000, 1001])
 def test_three_step_remainders_bit_identical(nt):
-    # passes of T = nt + tlag steps with T mod 3 = 0, 1, 2 end in a 3-, 1- or
-    # 2-step launch; traces, image, illumination, misfit and gradient equal
-    # the one-step kernels'
+    # passes of T = nt + tlag steps with T mod 3 = 0, 1, 2 (and T mod 4 = 0-3
+    # for the four-step forward) end in shorter launches; traces, image,
+    # illumination, misfit and gradient equal the one-step kernels'
     s = overthrust_survey(nt=nt, nsrc=2, abc_nb=51, abc_factor=0.004, rbc_nb=41,
                           varVp=700.0, seed=4, max_batch=2)
     s.eng.set_tuning(2, 64, 0, steps=1)
@@ -169,7 +181,8 @@ def test_three_step_remainders_bit_identical(nt):
         return out
 
     base = run()
-    for st, vz, xw, g in [(3, 2, 64, 1), (3, 1, 16, 0), (3, 4, 96, 1)]:
+    for st, vz, xw, g in [(3, 2, 64, 1), (3, 1, 16, 0), (3, 4, 96, 1), (4, 2, 64, 1),
+                          (4, 4, 96, 0)]:
         set_shape(s.eng, st, vz, xw, g)
         for k, (a, b) in enumerate(zip(run(), base)):
             assert np.array_equal(a, b), (nt, vz, xw, g, k)
@@ -182,7 +195,7 @@ def test_reverse_launch_shapes_bit_identical():
     s.eng.set_observed(obs)
     s.eng.set_model((s.v * 0.98).astype(np.float32))
     results = []
-    for st, vz, xw, g in [(1, 2, 32, 0)] + SHAPES[5:] + SHAPES3:
+    for st, vz, xw, g in [(1, 2, 32, 0)] + SHAPES[5:] + SHAPES3 + SHAPES4:
         set_shape(s.eng, st, vz, xw, g)
         s.eng.reset()
         s.eng.migration()
@@ -420,8 +433,14 @@ def test_errors_are_reported():
     assert eng.get_tuning(1)[:2] == (3, 1)      # a rejected call changes nothing
     eng.set_tuning(4, kernel=2)
     assert eng.get_tuning(2)[:2] == (3, 4)
-    with pytest.raises(F.FwiError):
-        eng.set_tuning(kernel=0, steps=4)
+    for kernel, steps in ((0, 5), (2, 4), (-1, 4)):
+        with pytest.raises(F.FwiError):   # four steps: forward and reverse kernels only
+            eng.set_tuning(kernel=kernel, steps=steps)
+    with pytest.raises(F.FwiError):       # no four-step reverse kernel with 4 rows per lane
+        eng.set_tuning(4, kernel=1, steps=4)
+    eng.set_tuning(kernel=0, steps=4)
+    eng.set_tuning(kernel=1, steps=4)
+    assert eng.get_tuning(0)[0] == 4 and eng.get_tuning(1)[:2] == (4, 1)
 
 
 def test_rejected_geometry_keeps_the_previous_one():
