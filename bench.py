@@ -307,7 +307,7 @@ def kernel_key(kind, steps, vz):
     """Name fragment of the random-boundary forward (kind 0) or imaging
     reverse (kind 1) kernel instance (either staging build)."""
     if kind == 0:
-        return f"fwd4_kernel<{vz}>" if steps == 4 else f"fwd{steps}_kernel<{vz}, false,"
+        return f"fwd{steps}_kernel<{vz}, false,"
     return f"rev4_kernel<{vz}," if steps == 4 else f"rev{steps}_kernel<{vz}, 0,"
 
 
@@ -519,8 +519,7 @@ def main():
     fwd_roof = roof(fwd_ms, fwd_n, fwd_cells, BYTES_FWD, copy_gbs, 0, stf)
     rev_roof["kernel"] = (f"rev4_kernel<{vzr}, staged> (fused reverse + imaging)" if str_ == 4 else
                           f"rev{str_}_kernel<{vzr}, 0, staged> (fused reverse + imaging)")
-    fwd_roof["kernel"] = (f"fwd4_kernel<{vzf}> (random-boundary forward)" if stf == 4 else
-                          f"fwd{stf}_kernel<{vzf}, false, staged> (random-boundary forward)")
+    fwd_roof["kernel"] = f"fwd{stf}_kernel<{vzf}, false, staged> (random-boundary forward)"
     out = {
         "metric": "Gcell-updates/s", "value": round(value, 2), "unit": "Gcell-updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

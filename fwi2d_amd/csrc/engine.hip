@@ -317,12 +317,8 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
     const int kind = fa ? (fa->abc ? 2 : 0) : 1;    // tuning slot
     const int pkind = fa ? 0 : 1;                   // profile slot: forward / reverse
     const fwi::Tiling tl = tiling_of(c, g, kind);
-    // four steps per launch: random-boundary forward passes without
-    // receivers and imaging reverse passes (a pass that records, and a
-    // gradient pass, run three-step launches)
     const bool tables = fa ? (fa->seis != nullptr) : true;
-    const bool four = fa ? !tables : (ra->mode == 0);
-    const int per = (c->nsteps[kind] == 4 && !four) ? 3 : c->nsteps[kind];
+    const int per = c->nsteps[kind];
     // the remainder of a multi-step pass runs as shorter launches, whose
     // waves own more rows each
     fwi::Tiling tls[5] = {tl, tl, tl, tl, tl};
@@ -1211,8 +1207,6 @@ int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_
     if (kernel < -1 || kernel > 2) return fail(FWI_ERR_ARG, "kernel must be -1, 0, 1 or 2");
     if (vz > 0 && vz != 1 && vz != 2 && vz != 4) return fail(FWI_ERR_ARG, "vz must be 1, 2 or 4");
     if (steps > 4) return fail(FWI_ERR_ARG, "steps per launch must be 1, 2, 3 or 4");
-    if (steps == 4 && kernel != 0 && kernel != 1)
-        return fail(FWI_ERR_ARG, "four steps per launch: kernels 0 and 1 only");
     int ns[3], vs[3], ws[3];
     for (int k = 0; k < 3; ++k) {
         const bool sel = kernel == -1 || kernel == k;
@@ -1271,7 +1265,7 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
         const long cells = (long)nslots * gb.g.nzz * gb.g.nxx;
         const int ncols = gb.g.nxx - 8;
         int bst = c->nsteps[kind], bvz = c->vz[kind], bxw = c->xw[kind];
-        for (int st = 1; st <= (kind != 2 ? 4 : 3); ++st)
+        for (int st = 1; st <= 4; ++st)
             for (int a = 0; a < 3; ++a) {
                 if (kind == 1 && st >= 3 && vzs[a] == 4) continue;
                 // fixed widths, narrow ones for small batches (more waves in

@@ -22,8 +22,7 @@ struct Grid {
 
 // Wave tiling of one launch: each 64-lane wave computes 64*vz rows of one
 // x-chunk of xw columns of one shot slot and writes the inner
-// 64*vz - 8*(steps-1) of them (steps per launch: 1, 2, 3, or 4 for the
-// random-boundary forward without receivers and the imaging reverse).
+// 64*vz - 8*(steps-1) of them (steps per launch: 1-4).
 struct Tiling {
     int vz;            // rows per lane: 1, 2 or 4
     int xw;            // columns per wave
@@ -91,8 +90,8 @@ struct RevStep {
     int mode;                  // 0 image, 1 gradient
 };
 
-// steps = 1-4 time steps per launch (4: random boundary, no receivers; the
-// reverse: imaging mode, vz 1 or 2)
+// steps = 1-4 time steps per launch (the three- and four-step reverse
+// kernels: vz 1 or 2)
 hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
                            hipStream_t s);
 hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,

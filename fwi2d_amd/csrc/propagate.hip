@@ -20,11 +20,10 @@
 //    before, from a register queue of the previous step's values; c, read
 //    for the first step, waits in an LDS ring for the later ones; a launch
 //    writes only the two fields the next one needs, so the HBM bytes of a
-//    launch do not grow with its steps (four steps: the random-boundary
-//    forward without receivers and the imaging reverse);
-//  * the three- and four-step kernels unroll their column loop over the nine phases
-//    of the queues, so the queues advance by renaming, not by moves, and
-//    their loads are branch-free (columns clamped into the grid);
+//    launch do not grow with its steps;
+//  * the three- and four-step kernels unroll their column loop over the
+//    nine phases of the queues, so the queues advance by renaming, not by
+//    moves, and their loads are branch-free (columns clamped into the grid);
 //  * source injection, the Cerjan sponge, receiver sampling (forward) and
 //    data injection, imaging / gradient and illumination (reverse) are fused
 //    into the same pass; the receiver tables and data of a wave's columns
@@ -1090,16 +1089,59 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
     }
 }
 
-// Four time steps per launch on the random-boundary grid without receivers
-// (the source pass of migration and of the gradient): the three-step march
-// with a fourth register queue.  Step s runs on the chunk +- 4(4 - s)
-// columns, on the rows that stay valid without the neighbouring waves' values
-// (64*VZ rows for step 1, 8 fewer for each later step; 64*VZ - 24 owned), so
-// a launch reads u_t, u_{t-1} and c once and writes u_{t+3} and u_{t+4}: the
-// HBM bytes of four steps are those of three in fwd3_kernel.  c of the last
-// thirteen columns waits in LDS for steps 2-4.  Same operations in the same
-// order as every other forward kernel, so the same bits.
-template <int VZ>
+// Two samples (steps s and s+1 of a launch) of the receivers of window
+// column c in this lane: seis[rec][tsamp + k] = d * v_k for the flagged k.
+template <int VZ, bool ABC>
+__device__ __forceinline__ void sample2_staged(const FwdStage &st, int nt, const float (&v1)[VZ],
+                                               const float (&v2)[VZ], float dx,
+                                               const float (&zm1)[VZ], int c, int zb,
+                                               float *seis, int tsamp, bool r1, bool r2)
+{
+    const int m0 = st.cp[c], m1 = st.cp[c + 1];
+    for (int m = m0; m < m1; ++m) {
+        const int rr = st.iz[m] - zb;
+#pragma unroll
+        for (int r = 0; r < VZ; ++r)
+            if (rr == r) {
+                const float d = ABC ? sponge(dx, zm1[r]) : 1.f;
+                const long base = (long)st.id[m] * nt + tsamp;
+                if (r1) seis[base] = v1[r] * d;
+                if (r2) seis[base + 1] = v2[r] * d;
+            }
+    }
+}
+
+// sample2_staged from the global receiver table.
+template <int VZ, bool ABC>
+__device__ __forceinline__ void sample2_receivers(const FwdStep &a, const float (&v1)[VZ],
+                                                  const float (&v2)[VZ], int j, int zb,
+                                                  float *seis, int tsamp, bool r1, bool r2)
+{
+    const int k0 = a.rec.col_ptr[j], k1 = a.rec.col_ptr[j + 1];
+    for (int k = k0; k < k1; ++k) {
+        const int rr = a.rec.iz[k] - zb;
+#pragma unroll
+        for (int r = 0; r < VZ; ++r)
+            if (rr == r) {
+                const float d = ABC ? damper(a.b1d, zb + r, j, a.g) : 1.f;
+                const long base = (long)a.rec.id[k] * a.nt + tsamp;
+                if (r1) seis[base] = v1[r] * d;
+                if (r2) seis[base + 1] = v2[r] * d;
+            }
+    }
+}
+
+// Four time steps per launch: the three-step march with a fourth register
+// queue.  Step s runs on the chunk +- 4(4 - s) columns, on the rows that
+// stay valid without the neighbouring waves' values (64*VZ rows for step 1,
+// 8 fewer for each later step; 64*VZ - 24 owned), so a launch reads u_t,
+// u_{t-1} and c once and writes u_{t+3} and u_{t+4}: the HBM bytes of four
+// steps are those of three in fwd3_kernel.  c of the last thirteen columns
+// waits in LDS for steps 2-4.  A receiver's samples of steps 1 and 2 are
+// written when step 2 reaches its column (u_{t+1} is then still in the
+// second queue), those of steps 3 and 4 when step 4 does.  Same operations
+// in the same order as every other forward kernel, so the same bits.
+template <int VZ, bool ABC, bool ST>
 __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int t)
 {
@@ -1132,6 +1174,30 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
     const float w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
     const float w3 = (t + 3 < a.nt) ? a.wav[t + 3] : 0.f;
     const float w4 = (t + 4 < a.nt) ? a.wav[t + 4] : 0.f;
+    const bool rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
+    const bool rec3 = (a.seis != nullptr) && (t + 2 >= a.tlag) && (t + 2 <= a.T - 2);
+    const bool rec4 = (a.seis != nullptr) && (t + 3 >= a.tlag) && (t + 3 <= a.T - 2);
+    float *seis = (rec1 || rec2 || rec3 || rec4) ? a.seis + (long)id.slot * a.rec.nrec * a.nt
+                                                 : nullptr;
+    const int xa = x0 - 12;                          // window [x0 - 12, x1 + 12)
+    __shared__ FwdStage st;
+    constexpr bool staged = ST;     // tables fit the LDS (the host checks)
+    if constexpr (ST)
+        (void)stage_fwd(a, rec1 || rec2 || rec3 || rec4, ABC, xa, x1 + 12 - xa, st, lane);
+
+    float zm1[VZ];                                   // dz - 1 of the lane's rows
+    if constexpr (ABC) row_factors<VZ>(a.b1d, zb, g, zm1);
+    // sponge x factor of grid column pc (1 without the sponge)
+    auto xfac = [&](int pc) {
+        if constexpr (ABC) {
+            bool in;
+            return staged ? st.dx[min(max(pc - xa, 0), x1 + 11 - xa)]
+                          : pad_factor(a.b1d, pc, nxx, g.nb, in);
+        } else {
+            return 1.f;
+        }
+    };
 
     float q1[9][VZ];     // u_t
     float q2[9][VZ];     // u_{t+1}
@@ -1152,7 +1218,7 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
     ldcolc<VZ>(on, old, PC(x0 - 12), nxx, ldz, zl);
     ldcolc<VZ>(cn, cc, PC(x0 - 12), nxx, ldz, zl);
     __shared__ float cring[13][64 * VZ];             // c of the last thirteen columns
-    const int jend = x1 + 12;                        // window [x0 - 12, x1 + 12)
+    const int jend = x1 + 12;
     int j = x0 - 12;
     int cs = 0;                                      // cring slot of column j
 
@@ -1177,13 +1243,15 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
         {
             const int pj = PC(j);
             const bool jin = (pj >= 4) && (pj < nxx - 4);
+            const float dxj = xfac(pj);
             float ext[VZ + 8], dz[VZ + 1];
             zext1<VZ>(ext, q1[C4], hq[C4]);
             zdiff<VZ>(dz, ext);
 #pragma unroll
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
-                float val = leapfrog<false>(q1[C4][r], o1[r], cj[r], lap8z<VZ, C0>(q1, ext, dz, r), 1.f);
+                const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
+                float val = leapfrog<ABC>(q1[C4][r], o1[r], cj[r], lap8z<VZ, C0>(q1, ext, dz, r), d);
                 q2[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o1[r];
             }
             add_source<VZ>(q2[C8], pj, six, zb, siz, w1);
@@ -1192,6 +1260,7 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
         if (j - 4 >= x0 - 8) {
             const int jj = j - 4, pjj = PC(jj);
             const bool jin = (pjj >= 4) && (pjj < nxx - 4);
+            const float dxj = xfac(pjj);
             float ext[VZ + 8], c2v[VZ], dz[VZ + 1];
             zext_local<VZ>(ext, q2[C4]);
             vload<VZ>(c2v, &cring[cs2][lane * VZ]);
@@ -1200,15 +1269,26 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float o2 = q1[C0][r];             // u_t at column jj
-                float val = leapfrog<false>(q2[C4][r], o2, c2v[r], lap8z<VZ, C0>(q2, ext, dz, r), 1.f);
+                const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
+                float val = leapfrog<ABC>(q2[C4][r], o2, c2v[r], lap8z<VZ, C0>(q2, ext, dz, r), d);
                 q3[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o2;
             }
             add_source<VZ>(q3[C8], pjj, six, zb, siz, w2);
+            // samples of steps 1 and 2: u_{t+1} (q2) and u_{t+2} of an owned column
+            if (own && jj >= x0 && jj < x1 && (rec1 || rec2)) {
+                if constexpr (staged)
+                    sample2_staged<VZ, ABC>(st, a.nt, q2[C4], q3[C8], dxj, zm1, pjj - xa, zb,
+                                            seis, t + 1 - a.tlag, rec1, rec2);
+                else
+                    sample2_receivers<VZ, ABC>(a, q2[C4], q3[C8], pjj, zb, seis,
+                                               t + 1 - a.tlag, rec1, rec2);
+            }
         }
         // ---- step 3: u_{t+3} at column j - 8, from x0 - 4 on
         if (j - 8 >= x0 - 4) {
             const int jjj = j - 8, pjjj = PC(jjj);
             const bool jin = (pjjj >= 4) && (pjjj < nxx - 4);
+            const float dxj = xfac(pjjj);
             float ext[VZ + 8], c3v[VZ], dz[VZ + 1];
             zext_local<VZ>(ext, q3[C4]);
             vload<VZ>(c3v, &cring[cs3][lane * VZ]);
@@ -1217,7 +1297,8 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float o3 = q2[C0][r];             // u_{t+1} at column jjj
-                float val = leapfrog<false>(q3[C4][r], o3, c3v[r], lap8z<VZ, C0>(q3, ext, dz, r), 1.f);
+                const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
+                float val = leapfrog<ABC>(q3[C4][r], o3, c3v[r], lap8z<VZ, C0>(q3, ext, dz, r), d);
                 q4[C8][r] = (jin & (i >= 4) & (i < g.nzz - 4)) ? val : o3;
             }
             add_source<VZ>(q4[C8], pjjj, six, zb, siz, w3);
@@ -1227,6 +1308,7 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
         const int j4 = j - 12;
         if (j4 >= x0) {
             const int pj4 = PC(j4);
+            const float dxj = xfac(pj4);
             float c4v[VZ], ext[VZ + 8], dz[VZ + 1], n4[VZ];
             vload<VZ>(c4v, &cring[cs4][lane * VZ]);
             zext_local<VZ>(ext, q4[C4]);
@@ -1235,11 +1317,23 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
             for (int r = 0; r < VZ; ++r) {
                 const int i = zb + r;
                 const float o4 = q3[C0][r];             // u_{t+2} at column j4
-                float val = leapfrog<false>(q4[C4][r], o4, c4v[r], lap8z<VZ, C0>(q4, ext, dz, r), 1.f);
+                const float d = ABC ? sponge(dxj, zm1[r]) : 1.f;
+                float val = leapfrog<ABC>(q4[C4][r], o4, c4v[r], lap8z<VZ, C0>(q4, ext, dz, r), d);
                 n4[r] = ((i >= 4) & (i < g.nzz - 4)) ? val : o4;
             }
             add_source<VZ>(n4, pj4, six, zb, siz, w4);
-            if (own) vstore<VZ>((out2 + (long)pj4 * ldz) + uzl, n4);
+            if (own) {
+                vstore<VZ>((out2 + (long)pj4 * ldz) + uzl, n4);
+                // samples of steps 3 and 4: u_{t+3} (q4) and u_{t+4}
+                if (rec3 || rec4) {
+                    if constexpr (staged)
+                        sample2_staged<VZ, ABC>(st, a.nt, q4[C4], n4, dxj, zm1, pj4 - xa, zb,
+                                                seis, t + 3 - a.tlag, rec3, rec4);
+                    else
+                        sample2_receivers<VZ, ABC>(a, q4[C4], n4, pj4, zb, seis,
+                                                   t + 3 - a.tlag, rec3, rec4);
+                }
+            }
         }
         // column j + 5 of u_t replaces column j - 4
 #pragma unroll
@@ -1788,17 +1882,18 @@ __global__ __launch_bounds__(64) void rev3_kernel(RevStep a, int nzseg, int nxch
     }
 }
 
-// Four reverse steps per launch, imaging mode: the three-step march with a
-// fourth pair of register queues (steps on the chunk +- 12, 8, 4, 0 columns
-// and 64*VZ, -8, -16, -24 rows), so the S, R, c, image and illumination
-// bytes of four steps are those of three.  c of the last thirteen columns
-// waits in LDS.  The image and illumination products of a column are taken
+// Four reverse steps per launch: the three-step march with a fourth pair of
+// register queues (steps on the chunk +- 12, 8, 4, 0 columns and 64*VZ,
+// -8, -16, -24 rows), so the S, R, c, image and illumination bytes of four
+// steps are those of three.  c of the last thirteen columns waits in LDS.
+// The image (or gradient) and illumination products of a column are taken
 // two steps at a time from values still in the queues -- steps 1 and 2 when
 // step 2 reaches the column, steps 3 and 4 when step 4 does -- and added to
 // the column's sums in step order, with the partial sums in LDS in between,
-// so the result has the bits of every other launch shape.
-template <int VZ, bool ST>
-__global__ __launch_bounds__(64) void rev4_kernel(RevStep a, int nzseg, int nxchunk, int xw,
+// so the result has the bits of every other launch shape.  The gradient's
+// products need c lap S of steps 1 and 3, which wait in register rings.
+template <int VZ, int MODE, bool ST>
+__global__ __launch_bounds__(64, 2) void rev4_kernel(RevStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int k)
 {
     TileId id;
@@ -1869,6 +1964,9 @@ __global__ __launch_bounds__(64) void rev4_kernel(RevStep a, int nzseg, int nxch
     // image / illumination partial sums (steps 1-2) of the columns j-12 .. j-4
     // in slot (m + P) % 9: the lane's VZ image rows, then its VZ illumination rows
     __shared__ __align__(16) float pring[9][64 * 2 * VZ];
+    // gradient mode: c lap S_0 of columns j-8 .. j (g1q, the q2 slots) and
+    // c lap S_2 of columns j-16 .. j-8 (g3q, the q4 slots)
+    float g1q[9][MODE == 1 ? VZ : 1], g3q[9][MODE == 1 ? VZ : 1];
     const int jend = x1 + 12;
     int j = x0 - 12;
     int cs = 0;                                      // cring slot of column j
@@ -1911,10 +2009,12 @@ __global__ __launch_bounds__(64) void rev4_kernel(RevStep a, int nzseg, int nxch
                 const int i = zb + r;
                 const bool inside = jin1 && (i >= 4) && (i < g.nzz - 4);
                 const float ps = qs1[C4][r], pr = qr1[C4][r];
-                const float sn = fmaf(cj[r], lap8z<VZ, C0>(qs1, es, dzs, r), ps + (ps - os[r]));
+                const float lapS = lap8z<VZ, C0>(qs1, es, dzs, r);
+                const float sn = fmaf(cj[r], lapS, ps + (ps - os[r]));
                 const float rn = fmaf(cj[r], lap8z<VZ, C0>(qr1, er, dzr, r), pr + (pr - orr[r]));
                 qs2[C8][r] = inside ? sn : os[r];
                 qr2[C8][r] = inside ? rn : orr[r];
+                if constexpr (MODE == 1) g1q[C8][r] = mul_nc(cj[r], lapS);
             }
             if (inj1 && jin1) {
                 if constexpr (staged) inject_staged<VZ, 0>(st, a.idh2, qr2[C8], pj - xa, zb);
@@ -1938,17 +2038,20 @@ __global__ __launch_bounds__(64) void rev4_kernel(RevStep a, int nzseg, int nxch
                 const bool inside = jin2 && (i >= 4) && (i < g.nzz - 4);
                 const float os2 = qs1[C0][r], or2 = qr1[C0][r];   // S_cur, R_cur at jj
                 const float ps = qs2[C4][r], pr = qr2[C4][r];     // step-1 S, R at jj
-                const float sn = fmaf(c2v[r], lap8z<VZ, C0>(qs2, es, dzs, r), ps + (ps - os2));
+                const float lapS = lap8z<VZ, C0>(qs2, es, dzs, r);
+                const float sn = fmaf(c2v[r], lapS, ps + (ps - os2));
                 const float rn = fmaf(c2v[r], lap8z<VZ, C0>(qr2, er, dzr, r), pr + (pr - or2));
                 qs3[C8][r] = inside ? sn : os2;
                 qr3[C8][r] = inside ? rn : or2;
                 pp[r] = a2[r];
                 pp[VZ + r] = i2[r];
-                if (inside) {        // step 1: S_1 R_0, then step 2: S_2 R_1
+                if (inside) {        // step 1 (S_1 R_0 or R_0 c lap S_0), then step 2
                     pp[VZ + r] = add_nc(pp[VZ + r], mul_nc(ps, ps));
-                    pp[r] = add_nc(pp[r], mul_nc(ps, or2));
+                    if constexpr (MODE == 0) pp[r] = add_nc(pp[r], mul_nc(ps, or2));
+                    else pp[r] = add_nc(pp[r], mul_nc(or2, g1q[C4][r]));
                     pp[VZ + r] = add_nc(pp[VZ + r], mul_nc(sn, sn));
-                    pp[r] = add_nc(pp[r], mul_nc(sn, pr));
+                    if constexpr (MODE == 0) pp[r] = add_nc(pp[r], mul_nc(sn, pr));
+                    else pp[r] = add_nc(pp[r], mul_nc(pr, mul_nc(c2v[r], lapS)));
                 }
             }
             vstore<2 * VZ>(&pring[C8][lane * 2 * VZ], pp);
@@ -1973,10 +2076,12 @@ __global__ __launch_bounds__(64) void rev4_kernel(RevStep a, int nzseg, int nxch
                 const bool inside = jin3 && (i >= 4) && (i < g.nzz - 4);
                 const float os3 = qs2[C0][r], or3 = qr2[C0][r];   // step-1 S, R at jjj
                 const float ps = qs3[C4][r], pr = qr3[C4][r];
-                const float sn = fmaf(c3v[r], lap8z<VZ, C0>(qs3, es, dzs, r), ps + (ps - os3));
+                const float lapS = lap8z<VZ, C0>(qs3, es, dzs, r);
+                const float sn = fmaf(c3v[r], lapS, ps + (ps - os3));
                 const float rn = fmaf(c3v[r], lap8z<VZ, C0>(qr3, er, dzr, r), pr + (pr - or3));
                 qs4[C8][r] = inside ? sn : os3;
                 qr4[C8][r] = inside ? rn : or3;
+                if constexpr (MODE == 1) g3q[C8][r] = mul_nc(c3v[r], lapS);
             }
             if (inj3 && jin3) {
                 if constexpr (staged) inject_staged<VZ, 2>(st, a.idh2, qr4[C8], pjjj - xa, zb);
@@ -2006,17 +2111,20 @@ __global__ __launch_bounds__(64) void rev4_kernel(RevStep a, int nzseg, int nxch
                 const bool inside = (i >= 4) && (i < g.nzz - 4);
                 const float os4 = qs3[C0][r], or4 = qr3[C0][r];   // step-2 S, R at j4
                 const float ps = qs4[C4][r], pr = qr4[C4][r];     // step-3 S, R at j4
-                const float sn = fmaf(c4v[r], lap8z<VZ, C0>(qs4, es, dzs, r), ps + (ps - os4));
+                const float lapS = lap8z<VZ, C0>(qs4, es, dzs, r);
+                const float sn = fmaf(c4v[r], lapS, ps + (ps - os4));
                 const float rn = fmaf(c4v[r], lap8z<VZ, C0>(qr4, er, dzr, r), pr + (pr - or4));
                 ns4[r] = inside ? sn : os4;
                 nr4[r] = inside ? rn : or4;
                 fa[r] = pp[r];
                 fi[r] = pp[VZ + r];
-                if (inside) {        // step 3: S_3 R_2, then step 4: S_4 R_3
+                if (inside) {        // step 3 (S_3 R_2 or R_2 c lap S_2), then step 4
                     fi[r] = add_nc(fi[r], mul_nc(ps, ps));
-                    fa[r] = add_nc(fa[r], mul_nc(ps, or4));
+                    if constexpr (MODE == 0) fa[r] = add_nc(fa[r], mul_nc(ps, or4));
+                    else fa[r] = add_nc(fa[r], mul_nc(or4, g3q[C4][r]));
                     fi[r] = add_nc(fi[r], mul_nc(sn, sn));
-                    fa[r] = add_nc(fa[r], mul_nc(sn, pr));
+                    if constexpr (MODE == 0) fa[r] = add_nc(fa[r], mul_nc(sn, pr));
+                    else fa[r] = add_nc(fa[r], mul_nc(pr, mul_nc(c4v[r], lapS)));
                 }
             }
             if (inj4) {
@@ -2381,12 +2489,10 @@ template <int VZ, bool ABC, bool ST>
 void fwd_launch(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps, int nblocks,
                 hipStream_t s)
 {
-    if (steps == 4) {
-        // random boundary, no receivers (the host checks)
-        if constexpr (!ABC)
-            hipLaunchKernelGGL((fwd4_kernel<VZ>), dim3(nblocks), dim3(64), 0, s, a, tl.nzseg,
-                               tl.nxchunk, tl.xw, nslots, t);
-    } else if (steps == 3)
+    if (steps == 4)
+        hipLaunchKernelGGL((fwd4_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
+                           tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
+    else if (steps == 3)
         hipLaunchKernelGGL((fwd3_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
                            tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
     else if (steps == 2)
@@ -2418,10 +2524,10 @@ hipError_t rev_launch(const RevStep &a, const Tiling &tl, int nslots, int k, int
                       int nblocks, hipStream_t s)
 {
     if (steps == 4) {
-        // imaging mode, one or two rows per lane (the host checks)
-        if constexpr (VZ == 4 || MODE != 0) return hipErrorInvalidValue;
+        // one or two rows per lane (the host checks)
+        if constexpr (VZ == 4) return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((rev4_kernel<VZ, ST>), dim3(nblocks), dim3(64), 0, s, a,
+            hipLaunchKernelGGL((rev4_kernel<VZ, MODE, ST>), dim3(nblocks), dim3(64), 0, s, a,
                                tl.nzseg, tl.nxchunk, tl.xw, nslots, k);
     } else if (steps == 3) {
         // six register queues per lane: four rows per lane do not fit
@@ -2500,16 +2606,17 @@ template <int VZ>
 int resident_vz(int kind, int steps)
 {
     if (kind == 0)
-        return steps == 4   ? resident_of(fwd4_kernel<VZ>)
+        return steps == 4   ? resident_of(fwd4_kernel<VZ, false, true>)
                : steps == 3 ? resident_of(fwd3_kernel<VZ, false, true>)
                : steps == 2 ? resident_of(fwd2_kernel<VZ, false, true>)
                             : resident_of(fwd1_kernel<VZ, false, true>);
     if (kind == 2)
-        return steps == 3   ? resident_of(fwd3_kernel<VZ, true, true>)
+        return steps == 4   ? resident_of(fwd4_kernel<VZ, true, true>)
+               : steps == 3 ? resident_of(fwd3_kernel<VZ, true, true>)
                : steps == 2 ? resident_of(fwd2_kernel<VZ, true, true>)
                             : resident_of(fwd1_kernel<VZ, true, true>);
     if constexpr (VZ != 4) {
-        if (steps == 4) return resident_of(rev4_kernel<VZ, true>);
+        if (steps == 4) return resident_of(rev4_kernel<VZ, 0, true>);
         if (steps == 3) return resident_of(rev3_kernel<VZ, 0, true>);
     }
     return steps == 2 ? resident_of(rev2_kernel<VZ, 0, true>)

@@ -218,10 +218,9 @@ int fwi_synchronize(fwi_ctx *ctx);
 
 /* Launch tuning of the step kernels (kernel 0 random-boundary forward, 1
  * reverse, 2 sponge-boundary forward, -1 all):
- * time steps per launch (1, 2 or 3, and 4 for kernel 0: temporal blocking;
- * a pass's remainder runs as shorter launches, and a kernel-0 pass that
- * records seismograms runs three-step launches), cells per lane along z (1,
- * 2 or 4; the three-step reverse kernel takes 1 or 2), columns per wave, and hipGraph
+ * time steps per launch (1-4: temporal blocking; a pass's remainder runs as
+ * shorter launches), cells per lane along z (1, 2 or 4; the three- and
+ * four-step reverse kernels take 1 or 2), columns per wave, and hipGraph
  * capture of the time loops (1 on, 0 off).  0 / negative keeps the current
  * value.  Every shape gives bit-identical results. */
 int fwi_set_tuning(fwi_ctx *ctx, int32_t kernel, int32_t steps, int32_t vz, int32_t xw,
@@ -237,8 +236,7 @@ int fwi_get_tuning(fwi_ctx *ctx, int32_t kernel, int32_t *steps, int32_t *vz, in
  * reverse kernel (0 without a random-boundary grid).  Until it runs, or
  * fwi_set_tuning sets them, every kernel uses three steps per launch, 81
  * columns per wave and 4 rows per lane (forward) or 2 (reverse).
- * Candidates: 1, 2 or 3 steps per launch (and 4 for the random-boundary
- * forward) x 1, 2 or 4 rows per lane x chunk
+ * Candidates: 1-4 steps per launch x 1, 2 or 4 rows per lane x chunk
  * widths 16-128 columns, 4 and 8 for small batches, and the widths that make
  * a launch 1-4 full rounds of the waves the device holds at once.  No
  * reference counterpart (the reference launches one fixed shape). */

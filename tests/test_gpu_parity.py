@@ -121,27 +121,19 @@ SHAPES = [(1, 1, 8, 0), (1, 2, 17, 1), (1, 4, 64, 1), (1, 4, 1000, 0), (1, 1, 3,
 # the largest that is staged in LDS
 SHAPES3 = [(3, 1, 8, 0), (3, 2, 17, 1), (3, 4, 64, 1), (3, 4, 1000, 0), (3, 1, 3, 1),
            (3, 2, 5, 0), (3, 2, 144, 1), (3, 1, 145, 0)]
-# four steps per launch: the random-boundary forward kernel of migration and
-# gradient passes and the imaging reverse kernel (the others take three)
+# four steps per launch: the window of a 136-column chunk (160 columns) is
+# the largest that is staged in LDS
 SHAPES4 = [(4, 1, 8, 0), (4, 2, 17, 1), (4, 4, 64, 1), (4, 4, 1000, 0), (4, 2, 130, 1),
-           (4, 1, 3, 0)]
+           (4, 1, 3, 0), (4, 2, 136, 1), (4, 1, 137, 0)]
 
 
 def set_shape(eng, st, vz, xw, g):
-    """One launch shape for every kernel; the three-step reverse kernel has no
-    four-rows-per-lane form and then takes two steps; four steps per launch
-    set the random-boundary forward kernel and, with one or two rows per lane,
-    the reverse kernel (whose gradient passes then take three), and three the
-    others."""
-    if st == 4:
-        set_shape(eng, 3, vz, xw, g)
-        eng.set_tuning(vz, xw, g, kernel=0, steps=4)
-        if vz != 4:
-            eng.set_tuning(vz, xw, g, kernel=1, steps=4)
-    elif st == 3 and vz == 4:
+    """One launch shape for every kernel; the three- and four-step reverse
+    kernels have no four-rows-per-lane form and then take two steps."""
+    if st >= 3 and vz == 4:
         eng.set_tuning(vz, xw, g, kernel=1, steps=2)
         for k in (0, 2):
-            eng.set_tuning(vz, xw, g, kernel=k, steps=3)
+            eng.set_tuning(vz, xw, g, kernel=k, steps=st)
     else:
         eng.set_tuning(vz, xw, g, steps=st)
 
@@ -151,7 +143,7 @@ def test_launch_shapes_bit_identical():
     s = overthrust_survey(abc_nb=51, abc_factor=0.004, max_batch=3)
     s.eng.set_tuning(steps=1, vz=4, xw=32, use_graph=0)
     base = s.eng.modeling(F.ABC)
-    for st, vz, xw, g in SHAPES + SHAPES3:
+    for st, vz, xw, g in SHAPES + SHAPES3 + SHAPES4:
         set_shape(s.eng, st, vz, xw, g)
         out = s.eng.modeling(F.ABC)
         assert np.array_equal(out, base), (st, vz, xw, g)
@@ -433,14 +425,13 @@ def test_errors_are_reported():
     assert eng.get_tuning(1)[:2] == (3, 1)      # a rejected call changes nothing
     eng.set_tuning(4, kernel=2)
     assert eng.get_tuning(2)[:2] == (3, 4)
-    for kernel, steps in ((0, 5), (2, 4), (-1, 4)):
-        with pytest.raises(F.FwiError):   # four steps: forward and reverse kernels only
-            eng.set_tuning(kernel=kernel, steps=steps)
+    for kernel in (0, 1, 2):
+        with pytest.raises(F.FwiError):   # at most four steps per launch
+            eng.set_tuning(kernel=kernel, steps=5)
     with pytest.raises(F.FwiError):       # no four-step reverse kernel with 4 rows per lane
         eng.set_tuning(4, kernel=1, steps=4)
-    eng.set_tuning(kernel=0, steps=4)
-    eng.set_tuning(kernel=1, steps=4)
-    assert eng.get_tuning(0)[0] == 4 and eng.get_tuning(1)[:2] == (4, 1)
+    eng.set_tuning(kernel=-1, steps=4)
+    assert [eng.get_tuning(k)[:2] for k in (0, 1, 2)] == [(4, 1), (4, 1), (4, 4)]
 
 
 def test_rejected_geometry_keeps_the_previous_one():
@@ -514,8 +505,9 @@ def test_full_size_invariants():
     random pad: 64 x 601 x 1301 cells, nt 4001; 100-cell sponge grid for the
     observed data), through properties that need no oracle:
     * a shot modeled alone equals the same shot in the 64-shot batch;
-    * one-step and three-step launch shapes give the same raw image and
-      illumination (both passes of the RTM);
+    * one-, three- and four-step launch shapes (the bench's shapes among
+      them) give the same raw image and illumination (both passes of the
+      RTM);
     * the RTM is linear in the data: doubled data give a doubled image and
       the same illumination (which depends on the source wavefield only);
     * modeling is linear in the wavelet: a doubled wavelet gives doubled
@@ -557,6 +549,13 @@ def test_full_size_invariants():
     _, raw1, ill1 = eng.get_image()
     assert np.abs(raw3).max() > 0
     assert np.array_equal(raw3, raw1) and np.array_equal(ill3, ill1)
+    for fvz, fxw, rxw in ((2, 130, 81), (4, 130, 130)):
+        eng.set_tuning(vz=fvz, xw=fxw, kernel=0, steps=4)
+        eng.set_tuning(vz=2, xw=rxw, kernel=1, steps=4)
+        eng.reset()
+        eng.migration()
+        _, raw4, ill4 = eng.get_image()
+        assert np.array_equal(raw4, raw1) and np.array_equal(ill4, ill1), (fvz, fxw, rxw)
 
     eng.set_tuning(vz=4, xw=81, kernel=0, steps=3)
     eng.set_tuning(vz=2, xw=81, kernel=1, steps=3)
