@@ -1938,7 +1938,11 @@ __global__ __launch_bounds__(64, 2) void rev4_kernel(RevStep a, int nzseg, int n
     float qs2[9][VZ], qr2[9][VZ];    // step-1 S, R
     float qs3[9][VZ], qr3[9][VZ];    // step-2 S, R
     float qs4[9][VZ], qr4[9][VZ];    // step-3 S, R
-    float hqs[9], hqr[9];            // halos (ldhalo1) of the q1 columns
+    // halos (ldhalo1) of the q1 columns; the gradient build, short of
+    // registers, loads a float4 halo one column ahead instead
+    constexpr int NH = MODE == 0 ? 9 : 1;
+    float hqs[NH], hqr[NH];
+    float4 h4s, h4r;
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
         ldcolc<VZ>(qs1[m], scur, PC(x0 - 16 + m), nxx, ldz, zl);
@@ -1947,13 +1951,18 @@ __global__ __launch_bounds__(64, 2) void rev4_kernel(RevStep a, int nzseg, int n
         vzero<VZ>(qs3[m]); vzero<VZ>(qr3[m]);
         vzero<VZ>(qs4[m]); vzero<VZ>(qr4[m]);
     }
+    if constexpr (MODE == 0) {
 #pragma unroll
-    for (int m = 4; m < 9; ++m) {
-        hqs[m] = ldhalo1<VZ>(scur, PC(x0 - 16 + m), nxx, ldz, zs, lane);
-        hqr[m] = ldhalo1<VZ>(rcur, PC(x0 - 16 + m), nxx, ldz, zs, lane);
+        for (int m = 4; m < 9; ++m) {
+            hqs[m] = ldhalo1<VZ>(scur, PC(x0 - 16 + m), nxx, ldz, zs, lane);
+            hqr[m] = ldhalo1<VZ>(rcur, PC(x0 - 16 + m), nxx, ldz, zs, lane);
+        }
+        hqs[0] = hqs[1] = hqs[2] = hqs[3] = 0.f;
+        hqr[0] = hqr[1] = hqr[2] = hqr[3] = 0.f;
+    } else {
+        h4s = ldhaloc<VZ>(scur, PC(x0 - 12), nxx, ldz, zs, lane);
+        h4r = ldhaloc<VZ>(rcur, PC(x0 - 12), nxx, ldz, zs, lane);
     }
-    hqs[0] = hqs[1] = hqs[2] = hqs[3] = 0.f;
-    hqr[0] = hqr[1] = hqr[2] = hqr[3] = 0.f;
     float osn[VZ], orn[VZ], cn[VZ], psn[VZ], prn[VZ], an[VZ], in[VZ];
     float hsn, hrn;
     ldcolc<VZ>(osn, sold, PC(x0 - 12), nxx, ldz, zl);
@@ -1983,8 +1992,10 @@ __global__ __launch_bounds__(64, 2) void rev4_kernel(RevStep a, int nzseg, int n
         vstore<VZ>(&cring[cs][lane * VZ], cj);
         ldcolc<VZ>(psn, scur, PC(j + 5), nxx, ldz, zl);
         ldcolc<VZ>(prn, rcur, PC(j + 5), nxx, ldz, zl);
-        hsn = ldhalo1<VZ>(scur, PC(j + 5), nxx, ldz, zs, lane);
-        hrn = ldhalo1<VZ>(rcur, PC(j + 5), nxx, ldz, zs, lane);
+        if constexpr (MODE == 0) {
+            hsn = ldhalo1<VZ>(scur, PC(j + 5), nxx, ldz, zs, lane);
+            hrn = ldhalo1<VZ>(rcur, PC(j + 5), nxx, ldz, zs, lane);
+        }
         ldcolc<VZ>(osn, sold, PC(j + 1), nxx, ldz, zl);
         ldcolc<VZ>(orn, rold, PC(j + 1), nxx, ldz, zl);
         ldcolc<VZ>(cn, cc, PC(j + 1), nxx, ldz, zl);
@@ -2000,8 +2011,15 @@ __global__ __launch_bounds__(64, 2) void rev4_kernel(RevStep a, int nzseg, int n
         const bool jin1 = (pj >= 4) && (pj < nxx - 4);
         {
             float es[VZ + 8], er[VZ + 8], dzs[VZ + 1], dzr[VZ + 1];
-            zext1<VZ>(es, qs1[C4], hqs[C4]);
-            zext1<VZ>(er, qr1[C4], hqr[C4]);
+            if constexpr (MODE == 0) {
+                zext1<VZ>(es, qs1[C4], hqs[C4]);
+                zext1<VZ>(er, qr1[C4], hqr[C4]);
+            } else {
+                zext<VZ>(es, qs1[C4], h4s);
+                zext<VZ>(er, qr1[C4], h4r);
+                h4s = ldhaloc<VZ>(scur, PC(j + 1), nxx, ldz, zs, lane);
+                h4r = ldhaloc<VZ>(rcur, PC(j + 1), nxx, ldz, zs, lane);
+            }
             zdiff<VZ>(dzs, es);
             zdiff<VZ>(dzr, er);
 #pragma unroll
@@ -2141,8 +2159,10 @@ __global__ __launch_bounds__(64, 2) void rev4_kernel(RevStep a, int nzseg, int n
         }
 #pragma unroll
         for (int r = 0; r < VZ; ++r) { qs1[C0][r] = psn[r]; qr1[C0][r] = prn[r]; }
-        hqs[C0] = hsn;
-        hqr[C0] = hrn;
+        if constexpr (MODE == 0) {
+            hqs[C0 % NH] = hsn;
+            hqr[C0 % NH] = hrn;
+        }
         cs = (cs == 12) ? 0 : cs + 1;
     };
     for (;;) {

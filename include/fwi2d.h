@@ -231,11 +231,12 @@ int fwi_get_tuning(fwi_ctx *ctx, int32_t kernel, int32_t *steps, int32_t *vz, in
 /* Autotuner: times every candidate launch shape of the forward and reverse
  * kernels on the random-boundary grid and of the sponge-boundary forward
  * kernel (receivers sampled) on its grid, for each grid that is configured,
- * with nslots resident shots (`reps` launches each), and keeps the fastest;
+ * with nslots resident shots (`reps` launches each), times the three
+ * fastest again with 3 x reps launches, and keeps the fastest of those;
  * returns the time per step (us) of the random-boundary forward and of the
  * reverse kernel (0 without a random-boundary grid).  Until it runs, or
- * fwi_set_tuning sets them, every kernel uses three steps per launch, 81
- * columns per wave and 4 rows per lane (forward) or 2 (reverse).
+ * fwi_set_tuning sets them, every kernel uses four steps per launch, 81
+ * columns per wave and 2 rows per lane.
  * Candidates: 1-4 steps per launch x 1, 2 or 4 rows per lane x chunk
  * widths 16-128 columns, 4 and 8 for small batches, and the widths that make
  * a launch 1-4 full rounds of the waves the device holds at once.  No
