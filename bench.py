@@ -308,7 +308,7 @@ def kernel_key(kind, steps, vz):
     reverse (kind 1) kernel instance (either staging build)."""
     if kind == 0:
         return f"fwd{steps}_kernel<{vz}, false,"
-    return f"rev4_kernel<{vz}," if steps == 4 else f"rev{steps}_kernel<{vz}, 0,"
+    return f"rev{steps}_kernel<{vz}, 0,"
 
 
 def counter_pass(prof, counter, child_args, workdir):
@@ -517,8 +517,7 @@ def main():
     sts, vzs, xws, _ = eng.get_tuning(2)
     rev_roof = roof(rev_ms, rev_n, rev_cells, BYTES_REV, copy_gbs, 1, str_)
     fwd_roof = roof(fwd_ms, fwd_n, fwd_cells, BYTES_FWD, copy_gbs, 0, stf)
-    rev_roof["kernel"] = (f"rev4_kernel<{vzr}, staged> (fused reverse + imaging)" if str_ == 4 else
-                          f"rev{str_}_kernel<{vzr}, 0, staged> (fused reverse + imaging)")
+    rev_roof["kernel"] = f"rev{str_}_kernel<{vzr}, 0, staged> (fused reverse + imaging)"
     fwd_roof["kernel"] = f"fwd{stf}_kernel<{vzf}, false, staged> (random-boundary forward)"
     out = {
         "metric": "Gcell-updates/s", "value": round(value, 2), "unit": "Gcell-updates/s",

@@ -69,3 +69,23 @@ def test_roofline_record_is_a_fraction_of_the_hbm_peak():
     assert abs(r["frac"] - 3.140e9 / 709.5e-6 / 8e12) < 1e-4 and r["frac"] <= 1.0
     assert abs(r["achieved"] - r["frac"] * 8000.0) < 0.1
     assert abs(r["traffic_over_compulsory"] - 3.140e9 / (cells * 52.0)) < 1e-4
+
+
+def test_counter_kernel_names_match_the_built_kernels():
+    """The counter passes find their kernels by name fragment (kernel_key):
+    every launch shape the autotuner can choose for the random-boundary
+    forward and the imaging reverse names a kernel the library holds."""
+    import shutil
+    import pytest
+    sys.path.insert(0, ROOT)
+    import bench
+    lib = os.path.join(ROOT, "fwi2d_amd", "lib", "libfwi2d.so")
+    nm = shutil.which("nm")
+    if not os.path.exists(lib) or nm is None:
+        pytest.skip("library not built or nm missing")
+    names = subprocess.run([nm, "-C", lib], capture_output=True, text=True).stdout
+    for steps in (1, 2, 3, 4):
+        for vz in (1, 2, 4):
+            assert bench.kernel_key(0, steps, vz) in names, (0, steps, vz)
+            if not (steps >= 3 and vz == 4):
+                assert bench.kernel_key(1, steps, vz) in names, (1, steps, vz)
