@@ -200,12 +200,14 @@ fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g, int kind)
 
 // Whether every wave of a launch of `steps` steps with tiling t can stage its
 // window in LDS: at most kStageCols columns and, when the launch uses the
-// receiver tables, at most kStageRec receivers (the windows and counts the
-// kernels' stage_table computes).
-bool staging_fits(const GridBuf &gb, const fwi::Tiling &t, int steps, bool tables)
+// receiver tables, at most kStageRec receivers (kStageRec4 in the four-step
+// reverse kernel) -- the windows and counts the kernels' stage_table
+// computes.
+bool staging_fits(const GridBuf &gb, const fwi::Tiling &t, int steps, bool tables,
+                  bool reverse)
 {
     const int h = fwi::window_halo(steps);
-    const int maxrec = steps == 4 ? fwi::kStageRec4 : fwi::kStageRec;
+    const int maxrec = (reverse && steps == 4) ? fwi::kStageRec4 : fwi::kStageRec;
     if (t.xw + 2 * h > fwi::kStageCols) return false;
     if (!tables || gb.h_col_ptr.empty()) return true;
     const int nxx = gb.g.nxx;
@@ -325,7 +327,7 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
     const GridBuf &gbuf = c->grid[kind == 2 ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC];
     for (int n = 1; n <= 4; ++n) {
         tls[n].nzseg = fwi::zsegments(g.nzz, tl.vz, n);
-        tls[n].staged = staging_fits(gbuf, tls[n], n, tables);
+        tls[n].staged = staging_fits(gbuf, tls[n], n, tables, fa == nullptr);
     }
     Quad s0 = sq, r0 = rq ? *rq : sq;
     int launches = 0;
@@ -1270,7 +1272,7 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
             c->vz[kind] = vz;
             c->xw[kind] = xwv;
             fwi::Tiling tl = tiling_of(c, gb.g, kind);
-            tl.staged = staging_fits(gb, tl, st, kind != 0);
+            tl.staged = staging_fits(gb, tl, st, kind != 0, kind == 1);
             fwi::FwdStep fa = kind == 2 ? fwd_args(c, FWI_BOUNDARY_ABC, true)
                                         : fwd_args(c, FWI_BOUNDARY_RBC, false);
             fa.cur = c->d_f[1]; fa.old = c->d_f[0];
