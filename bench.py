@@ -305,8 +305,11 @@ def net_workload(a):
 
 def kernel_key(kind, steps, vz):
     """Name fragment of the random-boundary forward (kind 0) or imaging
-    reverse (kind 1) kernel instance (either staging build)."""
+    reverse (kind 1) kernel instance (either staging build); vz 16 is the
+    forward kernel on 64 x 64-cell LDS tiles."""
     if kind == 0:
+        if vz == 16:
+            return f"fwdt_kernel<{steps}, false,"
         return f"fwd{steps}_kernel<{vz}, false,"
     return f"rev{steps}_kernel<{vz}, 0,"
 

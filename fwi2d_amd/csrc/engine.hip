@@ -193,7 +193,7 @@ fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g, int kind)
     t.vz = c->vz[kind];
     t.xw = c->xw[kind];
     t.nzseg = fwi::zsegments(g.nzz, t.vz, c->nsteps[kind]);
-    t.nxchunk = (g.nxx - 8 + t.xw - 1) / t.xw;
+    t.nxchunk = fwi::xchunks(g.nxx, t.vz, t.xw, c->nsteps[kind]);
     t.staged = true;
     return t;
 }
@@ -206,6 +206,7 @@ fwi::Tiling tiling_of(const fwi_ctx *c, const fwi::Grid &g, int kind)
 bool staging_fits(const GridBuf &gb, const fwi::Tiling &t, int steps, bool tables,
                   bool reverse)
 {
+    if (t.vz == fwi::kTileVz) return true;         // no per-wave tables
     const int h = fwi::window_halo(steps);
     const int maxrec = (reverse && steps == 4) ? fwi::kStageRec4 : fwi::kStageRec;
     if (t.xw + 2 * h > fwi::kStageCols) return false;
@@ -327,6 +328,7 @@ int run_steps(fwi_ctx *c, int pass, int nslots, const fwi::FwdStep *fa, const fw
     const GridBuf &gbuf = c->grid[kind == 2 ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC];
     for (int n = 1; n <= 4; ++n) {
         tls[n].nzseg = fwi::zsegments(g.nzz, tl.vz, n);
+        tls[n].nxchunk = fwi::xchunks(g.nxx, tl.vz, tl.xw, n);
         tls[n].staged = staging_fits(gbuf, tls[n], n, tables, fa == nullptr);
     }
     Quad s0 = sq, r0 = rq ? *rq : sq;
@@ -1207,7 +1209,8 @@ int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_
 {
     if (!c) return fail(FWI_ERR_ARG, "null context");
     if (kernel < -1 || kernel > 2) return fail(FWI_ERR_ARG, "kernel must be -1, 0, 1 or 2");
-    if (vz > 0 && vz != 1 && vz != 2 && vz != 4) return fail(FWI_ERR_ARG, "vz must be 1, 2 or 4");
+    if (vz > 0 && vz != 1 && vz != 2 && vz != 4 && vz != fwi::kTileVz)
+        return fail(FWI_ERR_ARG, "vz must be 1, 2, 4 or %d (forward LDS tiles)", fwi::kTileVz);
     if (steps > 4) return fail(FWI_ERR_ARG, "steps per launch must be 1, 2, 3 or 4");
     int ns[3], vs[3], ws[3];
     for (int k = 0; k < 3; ++k) {
@@ -1218,6 +1221,12 @@ int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_
     }
     if (ns[1] >= 3 && vs[1] == 4)
         return fail(FWI_ERR_ARG, "the three- and four-step reverse kernels take 1 or 2 rows per lane");
+    if (vs[1] == fwi::kTileVz)
+        return fail(FWI_ERR_ARG, "vz %d (LDS tiles) is a forward kernel shape", fwi::kTileVz);
+    for (int k = 0; k < 3; ++k)
+        if (vs[k] == fwi::kTileVz && ws[k] != 4 && ws[k] != 8 && ws[k] != 16)
+            return fail(FWI_ERR_ARG, "with vz %d, xw is the tile columns per wave: 4, 8 or 16",
+                        fwi::kTileVz);
     for (int k = 0; k < 3; ++k) {
         c->nsteps[k] = ns[k];
         c->vz[k] = vs[k];
@@ -1305,7 +1314,14 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
             int st, vz, xw;
         };
         std::vector<Cand> cands;
-        for (int st = 1; st <= 4; ++st)
+        for (int st = 1; st <= 4; ++st) {
+            // the forward kernels on LDS tiles (one shape per step count)
+            if (kind != 1)
+                for (int cpw : {16, 8, 4}) {        // tile columns per wave
+                    double us = 0.0;
+                    TRY(time_shape(st, fwi::kTileVz, cpw, reps, &us));
+                    cands.push_back({us, st, fwi::kTileVz, cpw});
+                }
             for (int a = 0; a < 3; ++a) {
                 if (kind == 1 && st >= 3 && vzs[a] == 4) continue;
                 // fixed widths, narrow ones for small batches (more waves in
@@ -1329,6 +1345,7 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
                     cands.push_back({us, st, vzs[a], xwv});
                 }
             }
+        }
         // the three fastest again, with three times the launches: shapes
         // within a few per cent of each other are told apart by the longer run
         std::sort(cands.begin(), cands.end(),

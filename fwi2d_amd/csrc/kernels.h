@@ -23,8 +23,13 @@ struct Grid {
 // Wave tiling of one launch: each 64-lane wave computes 64*vz rows of one
 // x-chunk of xw columns of one shot slot and writes the inner
 // 64*vz - 8*(steps-1) of them (steps per launch: 1-4).
+// vz = kTileVz selects the forward kernel on 64 x 64-cell LDS tiles instead
+// (nzseg x nxchunk tiles of (64 - 8 steps)^2 own cells; xw = the tile columns
+// each wave updates, 16, 8 or 4: 4, 8 or 16 waves per tile).
+constexpr int kTileVz = 16;
+constexpr int kTile = 64;          // tile edge in cells
 struct Tiling {
-    int vz;            // rows per lane: 1, 2 or 4
+    int vz;            // rows per lane: 1, 2 or 4; or kTileVz
     int xw;            // columns per wave
     int nzseg, nxchunk;
     bool staged;       // every wave's window fits the LDS tables below
@@ -97,7 +102,9 @@ hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t
 hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
                            hipStream_t s);
 // z segments of a launch shape (a launch of s steps writes 64*vz - 8(s-1) rows per wave)
+// and its x chunks; for kTileVz the tile rows and columns
 int zsegments(int nzz, int vz, int steps);
+int xchunks(int nxx, int vz, int xw, int steps);
 // Waves of the random-boundary forward (kind 0), imaging reverse (kind 1) or
 // sponge forward (kind 2) step kernel that the current device holds at once
 // (0 if unknown).

@@ -1354,6 +1354,155 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
     }
 }
 
+// ---------------------------------------------------------------- 2-D tiles
+//
+// Forward steps on 64 x 64-cell tiles held in LDS, one workgroup of NW
+// waves per tile (each thread: one row of every NW-th column): for batches of one or a few shots, where the column-marching
+// waves above are too few to fill the device and each wave's march over its
+// chunk plus halo sets the launch time.  A launch of NS steps loads u_t,
+// u_{t-1} and c of the tile (out-of-grid cells as zero) and runs the steps
+// in LDS, each on a region 4 cells narrower on every side than the last; the
+// inner (64 - 8 NS)^2 cells are the tile's own.  Every cell is updated with
+// the arithmetic of the column kernels (lap8_cell is lap8r term for term,
+// the same leapfrog, rim select, source and sample formulas), so the fields
+// and seismograms are bit-identical to theirs.
+
+// lap8r of cell (x, z) of an LDS tile, term for term: the x neighbours from
+// LDS, the z neighbours shifted in from the lanes above and below (the
+// thread of lane z holds row z of the column; every lane of the wave must
+// run this, and only rows 4 .. 59 get all eight).
+__device__ __forceinline__ float lap8_cell(const float (*u)[kTile], int x, int z, float p)
+{
+    const float a1 = wave_shr1z(p), a2 = wave_shr1z(a1), a3 = wave_shr1z(a2), a4 = wave_shr1z(a3);
+    const float b1 = wave_shl1z(p), b2 = wave_shl1z(b1), b3 = wave_shl1z(b2), b4 = wave_shl1z(b3);
+    const float s1 = ((u[x - 1][z] - p) + (u[x + 1][z] - p)) + ((a1 - p) + (b1 - p));
+    const float s2 = fmaf(-4.f, p, (u[x - 2][z] + u[x + 2][z]) + (a2 + b2));
+    const float s3 = fmaf(-4.f, p, (u[x - 3][z] + u[x + 3][z]) + (a3 + b3));
+    const float s4 = fmaf(-4.f, p, (u[x - 4][z] + u[x + 4][z]) + (a4 + b4));
+    return fmaf(kC4, s1, fmaf(-kC3, s2, fmaf(kC2, s3, -kC1 * s4)));
+}
+
+// Tiles of a launch of `steps` steps: owned (64 - 8 steps)^2 cells each over
+// the updated rows and columns 4 .. n-5.
+__host__ __device__ __forceinline__ int tile_count(int n, int steps)
+{
+    const int own = kTile - 8 * steps;
+    const int k = (n - 8 + own - 1) / own;
+    return k > 1 ? k : 1;
+}
+
+template <int NS, bool ABC, int NW>
+__global__ __launch_bounds__(64 * NW) void fwdt_kernel(FwdStep a, int ntz, int ntx, int nslots,
+                                                       int t)
+{
+    constexpr int H = 4 * NS, OWN = kTile - 2 * H, CPT = kTile / NW;   // columns per thread
+    const int tile = xcd_tile(ntz * ntx * nslots);
+    if (tile < 0) return;                            // the whole workgroup
+    const int tz = tile % ntz, tx = (tile / ntz) % ntx, slot = tile / (ntz * ntx);
+    const Grid g = a.g;
+    const int ldz = g.ldz, nxx = g.nxx, nzz = g.nzz;
+    const int z0 = 4 + tz * OWN - H;                 // grid row of tile row 0
+    const int x0 = 4 + tx * OWN - H;                 // grid column of tile column 0
+    const int z = threadIdx.x & 63;                  // tile row of this thread
+    const int xg = threadIdx.x >> 6;                 // its columns: xg, xg + NW, ...
+    const int gz = z0 + z;
+    const bool zok = (gz >= 0) & (gz < nzz);
+    const long soff = (long)slot * g.fstride;
+    const float *cur = a.cur + soff;
+    const float *old = a.old + soff;
+    const float *cc = a.c + (long)slot * a.cstride;
+    const int gshot = *a.shot0 + slot;
+    const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
+
+    __shared__ float u[2][kTile][kTile];             // [buffer][column][row]
+    __shared__ float cs[kTile][kTile];
+    __shared__ float sdx[kTile], szm[kTile];         // sponge: dx of a column, dz - 1 of a row
+    if constexpr (ABC) {
+        bool in;
+        if (threadIdx.x < kTile) sdx[z] = pad_factor(a.b1d, x0 + z, nxx, g.nb, in);
+        else if (threadIdx.x < 2 * kTile) szm[z] = pad_factor(a.b1d, gz, nzz, g.nb, in) - 1.f;
+    }
+    {
+        // all loads issued before the first LDS store, so that their
+        // latencies overlap
+        float lu[CPT], lo[CPT], lc[CPT];
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int x = xg + NW * m, gx = x0 + x;
+            const bool ok = zok & (gx >= 0) & (gx < nxx);
+            const long idx = (long)gx * ldz + gz;
+            lu[m] = ok ? cur[idx] : 0.f;
+            lc[m] = ok ? cc[idx] : 0.f;
+            // u_{t-1} only where step 1 reads it (with one step: the own
+            // cells, which no other workgroup reads while the result
+            // overwrites them)
+            const bool need = (x >= 4) & (x < kTile - 4) & (z >= 4) & (z < kTile - 4);
+            lo[m] = (ok & need) ? old[idx] : 0.f;
+        }
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int x = xg + NW * m;
+            u[0][x][z] = lu[m];
+            u[1][x][z] = lo[m];
+            cs[x][z] = lc[m];
+        }
+    }
+    __syncthreads();
+
+    const int zlo = 4 + tz * OWN, xlo = 4 + tx * OWN;    // own rows / columns in the grid
+    const int zhi = min(zlo + OWN, nzz), xhi = min(xlo + OWN, nxx - 4);
+#pragma unroll
+    for (int k = 1; k <= NS; ++k) {
+        const float (*cu)[kTile] = u[(k + 1) & 1];       // u_{t+k-1}
+        float (*nu)[kTile] = u[k & 1];                   // u_{t+k-2} -> u_{t+k}
+        const float w = (t + k < a.nt) ? a.wav[t + k] : 0.f;
+        const bool zin = (z >= 4 * k) & (z < kTile - 4 * k);
+        const float dzm = ABC ? szm[z] : 0.f;
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int x = xg + NW * m;                   // the same for the whole wave
+            if ((x < 4 * k) | (x >= kTile - 4 * k)) continue;
+            const float p = cu[x][z];
+            const float lap = lap8_cell(cu, x, z, p);    // all 64 lanes
+            if (zin) {
+                const int gx = x0 + x;
+                const float o = nu[x][z];
+                const float d = ABC ? sponge(sdx[x], dzm) : 1.f;
+                const float val = leapfrog<ABC>(p, o, cs[x][z], lap, d);
+                float v = ((gx >= 4) & (gx < nxx - 4) & (gz >= 4) & (gz < nzz - 4)) ? val : o;
+                if ((gx == six) & (gz == siz)) v += w;
+                nu[x][z] = v;
+            }
+        }
+        __syncthreads();
+        // the samples of u_{t+k} at the receivers of the own cells
+        const bool rec = (a.seis != nullptr) && (t + k - 1 >= a.tlag) && (t + k - 1 <= a.T - 2);
+        if (rec && xlo < xhi) {
+            float *seis = a.seis + (long)slot * a.rec.nrec * a.nt;
+            const int e0 = a.rec.col_ptr[xlo], e1 = a.rec.col_ptr[xhi];
+            for (int e = e0 + (int)threadIdx.x; e < e1; e += 64 * NW) {
+                const int iz = a.rec.iz[e];
+                if (iz < zlo || iz >= zhi) continue;
+                int ix = xlo;
+                while (a.rec.col_ptr[ix + 1] <= e) ++ix;
+                const float v = nu[ix - x0][iz - z0];
+                const float d = ABC ? sponge(sdx[ix - x0], szm[iz - z0]) : 1.f;
+                seis[(long)a.rec.id[e] * a.nt + (t + k - a.tlag)] = v * d;
+            }
+        }
+        // u_{t+NS-1} to out1 and u_{t+NS} to out2 (one step: u_{t+1} to out1)
+        float *out = (k == NS) ? (NS == 1 ? a.out1 : a.out2) : (k == NS - 1 ? a.out1 : nullptr);
+        if (out && (gz >= zlo) & (gz < zhi)) {
+            out += soff;
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                const int x = xg + NW * m, gx = x0 + x;
+                if ((gx >= xlo) & (gx < xhi)) out[(long)gx * ldz + gz] = nu[x][z];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- reverse
 
 // One reverse step k: S_new = leapfrog(S_cur, S_old) over S_old, R likewise,
@@ -2539,6 +2688,47 @@ hipError_t fwd_dispatch(const FwdStep &a, const Tiling &tl, int nslots, int t, i
     return hipGetLastError();
 }
 
+template <int NS, int NW>
+void fwdt_launch(const FwdStep &a, const Tiling &tl, int nslots, int t, int nblocks,
+                 hipStream_t s)
+{
+    if (a.abc)
+        hipLaunchKernelGGL((fwdt_kernel<NS, true, NW>), dim3(nblocks), dim3(64 * NW), 0, s, a,
+                           tl.nzseg, tl.nxchunk, nslots, t);
+    else
+        hipLaunchKernelGGL((fwdt_kernel<NS, false, NW>), dim3(nblocks), dim3(64 * NW), 0, s, a,
+                           tl.nzseg, tl.nxchunk, nslots, t);
+}
+
+template <int NW>
+hipError_t fwdt_steps(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
+                      int nblocks, hipStream_t s)
+{
+    switch (steps) {
+    case 1: fwdt_launch<1, NW>(a, tl, nslots, t, nblocks, s); break;
+    case 2: fwdt_launch<2, NW>(a, tl, nslots, t, nblocks, s); break;
+    case 3: fwdt_launch<3, NW>(a, tl, nslots, t, nblocks, s); break;
+    case 4: fwdt_launch<4, NW>(a, tl, nslots, t, nblocks, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// xw is the columns of the tile each wave updates: 16, 8 or 4 (4, 8 or 16
+// waves per tile)
+hipError_t fwdt_dispatch(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
+                         hipStream_t s)
+{
+    const int ntiles = tl.nzseg * tl.nxchunk * nslots;
+    const int nblocks = ((ntiles + 7) >> 3) << 3;
+    switch (tl.xw) {
+    case 16: return fwdt_steps<4>(a, tl, nslots, t, steps, nblocks, s);
+    case 8: return fwdt_steps<8>(a, tl, nslots, t, steps, nblocks, s);
+    case 4: return fwdt_steps<16>(a, tl, nslots, t, steps, nblocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <int VZ, int MODE, bool ST>
 hipError_t rev_launch(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
                       int nblocks, hipStream_t s)
@@ -2582,10 +2772,17 @@ hipError_t rev_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, i
 
 int zsegments(int nzz, int vz, int steps)
 {
+    if (vz == kTileVz) return tile_count(nzz, steps);
     // waves tile the updated rows 4 .. nzz-5; the rim rows stay zero
     const int rows = 64 * vz - 8 * (steps - 1);
     const int n = (nzz - 8 + rows - 1) / rows;
     return n > 1 ? n : 1;
+}
+
+int xchunks(int nxx, int vz, int xw, int steps)
+{
+    if (vz == kTileVz) return tile_count(nxx, steps);
+    return (nxx - 8 + xw - 1) / xw;
 }
 
 hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps,
@@ -2595,6 +2792,7 @@ hipError_t launch_fwd_step(const FwdStep &a, const Tiling &tl, int nslots, int t
     case 1: return fwd_dispatch<1>(a, tl, nslots, t, steps, s);
     case 2: return fwd_dispatch<2>(a, tl, nslots, t, steps, s);
     case 4: return fwd_dispatch<4>(a, tl, nslots, t, steps, s);
+    case kTileVz: return fwdt_dispatch(a, tl, nslots, t, steps, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -85,6 +85,7 @@ def test_counter_kernel_names_match_the_built_kernels():
         pytest.skip("library not built or nm missing")
     names = subprocess.run([nm, "-C", lib], capture_output=True, text=True).stdout
     for steps in (1, 2, 3, 4):
+        assert bench.kernel_key(0, steps, 16) in names, (0, steps, 16)
         for vz in (1, 2, 4):
             assert bench.kernel_key(0, steps, vz) in names, (0, steps, vz)
             if not (steps >= 3 and vz == 4):

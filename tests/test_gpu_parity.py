@@ -125,12 +125,21 @@ SHAPES3 = [(3, 1, 8, 0), (3, 2, 17, 1), (3, 4, 64, 1), (3, 4, 1000, 0), (3, 1, 3
 # the largest that is staged in LDS
 SHAPES4 = [(4, 1, 8, 0), (4, 2, 17, 1), (4, 4, 64, 1), (4, 4, 1000, 0), (4, 2, 130, 1),
            (4, 1, 3, 0), (4, 2, 136, 1), (4, 1, 137, 0)]
+# the forward kernels on 64 x 64-cell LDS tiles (vz 16), one to four steps;
+# the reverse kernel then takes the steps with two rows per lane
+TILE = 16
+SHAPEST = [(1, TILE, 16, 0), (2, TILE, 8, 1), (3, TILE, 4, 0), (4, TILE, 16, 1), (4, TILE, 4, 0)]
 
 
 def set_shape(eng, st, vz, xw, g):
     """One launch shape for every kernel; the three- and four-step reverse
-    kernels have no four-rows-per-lane form and then take two steps."""
-    if st >= 3 and vz == 4:
+    kernels have no four-rows-per-lane form and then take two steps; with
+    the forward LDS tiles the reverse kernel takes two rows per lane."""
+    if vz == TILE:
+        eng.set_tuning(2, 32, g, kernel=1, steps=st)
+        for k in (0, 2):
+            eng.set_tuning(vz, xw, g, kernel=k, steps=st)
+    elif st >= 3 and vz == 4:
         eng.set_tuning(vz, xw, g, kernel=1, steps=2)
         for k in (0, 2):
             eng.set_tuning(vz, xw, g, kernel=k, steps=st)
@@ -143,7 +152,7 @@ def test_launch_shapes_bit_identical():
     s = overthrust_survey(abc_nb=51, abc_factor=0.004, max_batch=3)
     s.eng.set_tuning(steps=1, vz=4, xw=32, use_graph=0)
     base = s.eng.modeling(F.ABC)
-    for st, vz, xw, g in SHAPES + SHAPES3 + SHAPES4:
+    for st, vz, xw, g in SHAPES + SHAPES3 + SHAPES4 + SHAPEST:
         set_shape(s.eng, st, vz, xw, g)
         out = s.eng.modeling(F.ABC)
         assert np.array_equal(out, base), (st, vz, xw, g)
@@ -174,7 +183,7 @@ def test_three_step_remainders_bit_identical(nt):
 
     base = run()
     for st, vz, xw, g in [(3, 2, 64, 1), (3, 1, 16, 0), (3, 4, 96, 1), (4, 2, 64, 1),
-                          (4, 4, 96, 0)]:
+                          (4, 4, 96, 0), (4, TILE, 8, 1), (3, TILE, 16, 0)]:
         set_shape(s.eng, st, vz, xw, g)
         for k, (a, b) in enumerate(zip(run(), base)):
             assert np.array_equal(a, b), (nt, vz, xw, g, k)
@@ -400,6 +409,11 @@ def test_config1_forward_full_size():
         err = rel_l2(seis[0], ref)
         print(f"boundary {bnd} relL2 {err:.3e}")
         assert err <= SEIS_TOL
+        # the LDS-tile forward kernel (the single-shot shape) gives the same bits
+        for st, cpw in ((4, 8), (3, 16), (4, 4)):
+            for k in (0, 2):
+                s.eng.set_tuning(TILE, cpw, 1, kernel=k, steps=st)
+            assert np.array_equal(s.eng.modeling(bnd), seis), (bnd, st, cpw)
 
 
 def test_errors_are_reported():
@@ -432,6 +446,15 @@ def test_errors_are_reported():
         eng.set_tuning(4, kernel=1, steps=4)
     eng.set_tuning(kernel=-1, steps=4)
     assert [eng.get_tuning(k)[:2] for k in (0, 1, 2)] == [(4, 1), (4, 1), (4, 4)]
+    for kernel in (-1, 1):            # the LDS tiles are a forward kernel shape
+        with pytest.raises(F.FwiError):
+            eng.set_tuning(TILE, kernel=kernel)
+    with pytest.raises(F.FwiError):
+        eng.set_tuning(8, kernel=0)
+    with pytest.raises(F.FwiError):       # tile columns per wave: 4, 8 or 16
+        eng.set_tuning(TILE, 64, kernel=0)
+    eng.set_tuning(TILE, 8, kernel=0)
+    assert eng.get_tuning(0)[:3] == (4, TILE, 8) and eng.get_tuning(1)[:2] == (4, 1)
 
 
 def test_rejected_geometry_keeps_the_previous_one():

@@ -15,7 +15,11 @@ Checks, for every tile and lane of a launch:
 
 The halo clamp is the case this pins: the first z segment starts at row -4,
 so its halo rows -8 .. -5 lie in the previous column, which exists only for
-columns >= 1 (clamped to >= 4 here)."""
+columns >= 1 (clamped to >= 4 here).
+
+The forward kernel on 64 x 64-cell LDS tiles (`fwdt_kernel`) is modelled
+at the end: its stores cover the updated cells once, and every LDS read of
+every step is a cell the load phase filled or the previous step wrote."""
 import itertools
 
 import numpy as np
@@ -126,3 +130,72 @@ def test_the_model_catches_an_unclamped_halo():
     lows = [lo for buf, lo, hi in accesses(163, 283, 2, 81, False, halo_lo=0)]
     assert min(lows) == -8
     assert min(lo for buf, lo, hi in accesses(163, 283, 2, 81, False)) >= 0
+
+
+# ---- forward kernel on 64 x 64-cell LDS tiles (fwdt_kernel)
+
+TILE = 64
+
+
+def tile_count(n, steps):
+    own = TILE - 8 * steps
+    return max(1, (n - 8 + own - 1) // own)
+
+
+def tile_flow(nzz, nxx, steps):
+    """Per tile: the LDS cells each step reads (current field at the stencil's
+    17 points, older field and c at the cell), the cells it writes, and the
+    grid cells the launch stores, in the kernel's tile coordinates."""
+    h, own = 4 * steps, TILE - 8 * steps
+    for tz in range(tile_count(nzz, steps)):
+        for tx in range(tile_count(nxx, steps)):
+            z0, x0 = 4 + tz * own - h, 4 + tx * own - h
+            zlo, xlo = 4 + tz * own, 4 + tx * own
+            zhi, xhi = min(zlo + own, nzz), min(xlo + own, nxx - 4)
+            yield z0, x0, (zlo, zhi, xlo, xhi)
+
+
+@pytest.mark.parametrize("nzz,nxx", [(321, 601), (81, 93), (601, 1301), (17, 19), (72, 72)])
+@pytest.mark.parametrize("steps", [1, 2, 3, 4])
+def test_tile_kernel_stores_cover_the_grid_once(nzz, nxx, steps):
+    ldz, fs = layout(nzz, nxx)
+    hits = np.zeros((nxx, ldz), np.int32)
+    for z0, x0, (zlo, zhi, xlo, xhi) in tile_flow(nzz, nxx, steps):
+        h = 4 * steps
+        # the stored cells lie in the last step's region [h, 64 - h) of the tile
+        assert zlo - z0 >= h and zhi - z0 <= TILE - h and xlo - x0 >= h and xhi - x0 <= TILE - h
+        hits[xlo:xhi, zlo:zhi] += 1
+        assert 0 <= xlo and xhi <= nxx - 4 and 4 <= zlo and zhi <= nzz
+    assert (hits[4:nxx - 4, 4:nzz - 4] == 1).all()
+    assert hits.max() == 1
+    assert hits[:4].sum() == 0 and hits[nxx - 4:].sum() == 0 and hits[:, :4].sum() == 0
+    assert hits[:, nzz:].sum() == 0
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 4])
+def test_tile_kernel_reads_only_what_it_loaded_or_wrote(steps):
+    """Each step k updates tile cells [4k, 64 - 4k)^2 from the current field
+    at the stencil points and the older field at the cell; every such read is
+    a cell the load phase filled (u_t, c everywhere; u_{t-1} on [4, 60)^2) or
+    the previous step wrote, and stays inside the 64 x 64 tile."""
+    n = TILE
+    filled = [np.ones((n, n), bool), np.zeros((n, n), bool)]     # u[0] = u_t, u[1] = u_{t-1}
+    filled[1][4:n - 4, 4:n - 4] = True
+    offs = [(d, 0) for d in range(-4, 5)] + [(0, d) for d in range(-4, 5) if d]
+    for k in range(1, steps + 1):
+        cur, new = filled[(k + 1) & 1], filled[k & 1]
+        lo, hi = 4 * k, n - 4 * k
+        for x in range(lo, hi):
+            for z in range(lo, hi):
+                for dx, dz in offs:
+                    assert 0 <= x + dx < n and 0 <= z + dz < n
+                    assert cur[x + dx, z + dz], (k, x, z, dx, dz)
+                assert new[x, z], (k, x, z)          # the older field at the cell
+        region = np.zeros((n, n), bool)
+        region[lo:hi, lo:hi] = True
+        filled[k & 1] = new | region
+        # one step: u_{t-1} is read only on the tile's own cells, which its
+        # result overwrites in place in global memory
+        if steps == 1:
+            h = 4
+            assert lo == h and hi == n - h
