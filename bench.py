@@ -226,7 +226,14 @@ def other_workload(a):
         eng.set_observed(eng.modeling(F.ABC))
         eng.set_model(init_v)
         azz, axx, _ = eng.grid_shape(F.ABC)
-    if not a.no_autotune:
+    if a.fwd_tuning or a.rev_tuning:
+        # pinned shapes (profiling runs); the forward one also for the sponge kernel
+        for kernels, spec in (((0, 2), a.fwd_tuning), ((1,), a.rev_tuning)):
+            if spec:
+                st, vz, xw = (int(x) for x in spec.split(","))
+                for kernel in kernels:
+                    eng.set_tuning(vz, xw, kernel=kernel, steps=st)
+    elif not a.no_autotune:
         eng.autotune(s.nsrc, reps=8)
 
     def step():

@@ -1417,6 +1417,12 @@ __global__ __launch_bounds__(64 * NW) void fwdt_kernel(FwdStep a, int ntz, int n
     __shared__ float u[2][kTile][kTile];             // [buffer][column][row]
     __shared__ float cs[kTile][kTile];
     __shared__ float sdx[kTile], szm[kTile];         // sponge: dx of a column, dz - 1 of a row
+    __shared__ int scp[kTile + 1];                   // receiver table offsets of the own columns
+    const int zlo = 4 + tz * OWN, xlo = 4 + tx * OWN;    // own rows / columns in the grid
+    const int zhi = min(zlo + OWN, nzz), xhi = min(xlo + OWN, nxx - 4);
+    const bool rec_any = (a.seis != nullptr) && (t + NS - 1 >= a.tlag) && (t <= a.T - 2);
+    if (rec_any && (int)threadIdx.x <= OWN)
+        scp[threadIdx.x] = a.rec.col_ptr[min(xlo + (int)threadIdx.x, max(xhi, xlo))];
     if constexpr (ABC) {
         bool in;
         if (threadIdx.x < kTile) sdx[z] = pad_factor(a.b1d, x0 + z, nxx, g.nb, in);
@@ -1449,8 +1455,6 @@ __global__ __launch_bounds__(64 * NW) void fwdt_kernel(FwdStep a, int ntz, int n
     }
     __syncthreads();
 
-    const int zlo = 4 + tz * OWN, xlo = 4 + tx * OWN;    // own rows / columns in the grid
-    const int zhi = min(zlo + OWN, nzz), xhi = min(xlo + OWN, nxx - 4);
 #pragma unroll
     for (int k = 1; k <= NS; ++k) {
         const float (*cu)[kTile] = u[(k + 1) & 1];       // u_{t+k-1}
@@ -1477,14 +1481,12 @@ __global__ __launch_bounds__(64 * NW) void fwdt_kernel(FwdStep a, int ntz, int n
         __syncthreads();
         // the samples of u_{t+k} at the receivers of the own cells
         const bool rec = (a.seis != nullptr) && (t + k - 1 >= a.tlag) && (t + k - 1 <= a.T - 2);
-        if (rec && xlo < xhi) {
+        if (rec && (int)threadIdx.x < xhi - xlo) {       // one own column per thread
             float *seis = a.seis + (long)slot * a.rec.nrec * a.nt;
-            const int e0 = a.rec.col_ptr[xlo], e1 = a.rec.col_ptr[xhi];
-            for (int e = e0 + (int)threadIdx.x; e < e1; e += 64 * NW) {
+            const int c = threadIdx.x, ix = xlo + c;
+            for (int e = scp[c]; e < scp[c + 1]; ++e) {
                 const int iz = a.rec.iz[e];
                 if (iz < zlo || iz >= zhi) continue;
-                int ix = xlo;
-                while (a.rec.col_ptr[ix + 1] <= e) ++ix;
                 const float v = nu[ix - x0][iz - z0];
                 const float d = ABC ? sponge(sdx[ix - x0], szm[iz - z0]) : 1.f;
                 seis[(long)a.rec.id[e] * a.nt + (t + k - a.tlag)] = v * d;
