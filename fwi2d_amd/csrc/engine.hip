@@ -1221,8 +1221,6 @@ int fwi_set_tuning(fwi_ctx *c, int32_t kernel, int32_t steps, int32_t vz, int32_
     }
     if (ns[1] >= 3 && vs[1] == 4)
         return fail(FWI_ERR_ARG, "the three- and four-step reverse kernels take 1 or 2 rows per lane");
-    if (vs[1] == fwi::kTileVz)
-        return fail(FWI_ERR_ARG, "vz %d (LDS tiles) is a forward kernel shape", fwi::kTileVz);
     for (int k = 0; k < 3; ++k)
         if (vs[k] == fwi::kTileVz && ws[k] != 4 && ws[k] != 8 && ws[k] != 16)
             return fail(FWI_ERR_ARG, "with vz %d, xw is the tile columns per wave: 4, 8 or 16",
@@ -1316,12 +1314,11 @@ int fwi_autotune(fwi_ctx *c, int32_t nslots, int32_t reps, double *fwd_us, doubl
         std::vector<Cand> cands;
         for (int st = 1; st <= 4; ++st) {
             // the forward kernels on LDS tiles (one shape per step count)
-            if (kind != 1)
-                for (int cpw : {16, 8, 4}) {        // tile columns per wave
-                    double us = 0.0;
-                    TRY(time_shape(st, fwi::kTileVz, cpw, reps, &us));
-                    cands.push_back({us, st, fwi::kTileVz, cpw});
-                }
+            for (int cpw : {16, 8, 4}) {            // LDS tiles: tile columns per wave
+                double us = 0.0;
+                TRY(time_shape(st, fwi::kTileVz, cpw, reps, &us));
+                cands.push_back({us, st, fwi::kTileVz, cpw});
+            }
             for (int a = 0; a < 3; ++a) {
                 if (kind == 1 && st >= 3 && vzs[a] == 4) continue;
                 // fixed widths, narrow ones for small batches (more waves in

@@ -2329,6 +2329,166 @@ __global__ __launch_bounds__(64, 2) void rev4_kernel(RevStep a, int nzseg, int n
     }
 }
 
+// Reverse steps on 64 x 64-cell LDS tiles (the reverse counterpart of
+// fwdt_kernel, for one or a few shots): S and R of the tile in LDS, c and
+// the own cells' image (or gradient) and illumination sums in registers.
+// Step k updates S and R on the tile's inner (64 - 8k)^2 cells with the
+// column kernels' arithmetic, adds the step's products to the own cells'
+// sums in step order (image S_k R_{k-1} or gradient R_{k-1} c lap S_{k-1},
+// illumination S_k^2), and then adds the step's data at the receivers of the
+// region, one column per thread in receiver-table order (the data of the
+// window's first kRevTileRec receivers waits in LDS, the rest is read when
+// needed).  Bit-identical to the column kernels.
+constexpr int kRevTileRec = 256;
+
+template <int NS, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW) void revt_kernel(RevStep a, int ntz, int ntx, int nslots,
+                                                       int k)
+{
+    constexpr int H = 4 * NS, OWN = kTile - 2 * H, CPT = kTile / NW;
+    const int tile = xcd_tile(ntz * ntx * nslots);
+    if (tile < 0) return;                            // the whole workgroup
+    const int tz = tile % ntz, tx = (tile / ntz) % ntx, slot = tile / (ntz * ntx);
+    const Grid g = a.g;
+    const int ldz = g.ldz, nxx = g.nxx, nzz = g.nzz;
+    const int z0 = 4 + tz * OWN - H, x0 = 4 + tx * OWN - H;
+    const int z = threadIdx.x & 63, xg = threadIdx.x >> 6;
+    const int gz = z0 + z;
+    const bool zok = (gz >= 0) & (gz < nzz);
+    const long soff = (long)slot * g.fstride;
+    const int zlo = 4 + tz * OWN, xlo = 4 + tx * OWN;
+    const int zhi = min(zlo + OWN, nzz), xhi = min(xlo + OWN, nxx - 4);
+    const bool zown = (gz >= zlo) & (gz < zhi);
+
+    __shared__ float su[2][kTile][kTile], ru[2][kTile][kTile];   // [buffer][column][row]
+    __shared__ int scp[kTile - 7];       // receiver offsets of the columns x0+4 .. x0+60
+    __shared__ int riz[kRevTileRec];
+    __shared__ float rv[NS][kRevTileRec];
+    float cr[CPT], ga[CPT], gi[CPT];     // c of the thread's cells; own cells' sums
+    {
+        float l0[CPT], l1[CPT], l2[CPT], l3[CPT];
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int x = xg + NW * m, gx = x0 + x;
+            const bool ok = zok & (gx >= 0) & (gx < nxx);
+            const long idx = soff + (long)gx * ldz + gz;
+            const bool need = (x >= 4) & (x < kTile - 4) & (z >= 4) & (z < kTile - 4);
+            const bool own = zown & (gx >= xlo) & (gx < xhi);
+            l0[m] = ok ? a.scur[idx] : 0.f;
+            l1[m] = ok ? a.rcur[idx] : 0.f;
+            l2[m] = (ok & need) ? a.sold[idx] : 0.f;
+            l3[m] = (ok & need) ? a.rold[idx] : 0.f;
+            cr[m] = ok ? a.c[(long)slot * a.cstride + (long)gx * ldz + gz] : 0.f;
+            ga[m] = own ? a.acc[idx] : 0.f;
+            gi[m] = own ? a.ill[idx] : 0.f;
+        }
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int x = xg + NW * m;
+            su[0][x][z] = l0[m];
+            ru[0][x][z] = l1[m];
+            su[1][x][z] = l2[m];
+            ru[1][x][z] = l3[m];
+        }
+    }
+    // receivers of the window columns, one column per thread: offsets and,
+    // for the first kRevTileRec, their rows and the launch's data samples
+    const bool inj_any = (k + 1 < a.nt);
+    const int drow = a.data_by_shot ? (*a.shot0 + slot) : slot;
+    const float *dat = a.data + (long)drow * a.rec.nrec * a.nt;
+    const int c = threadIdx.x;                       // window column x0 + 4 + c
+    const bool colt = inj_any & (c < kTile - 8);
+    int e0 = 0;
+    if (inj_any) e0 = a.rec.col_ptr[min(max(x0 + 4, 0), nxx)];
+    if (colt) {
+        const int gx = x0 + 4 + c;
+        scp[c] = a.rec.col_ptr[min(max(gx, 0), nxx)] - e0;
+        if (c == kTile - 9) scp[c + 1] = a.rec.col_ptr[min(max(gx + 1, 0), nxx)] - e0;
+    }
+    __syncthreads();
+    if (colt) {
+        for (int e = scp[c]; e < min(scp[c + 1], kRevTileRec); ++e) {
+            riz[e] = a.rec.iz[e0 + e];
+            const float *row = dat + (long)a.rec.id[e0 + e] * a.nt;
+#pragma unroll
+            for (int q = 0; q < NS; ++q) rv[q][e] = (k + 1 + q < a.nt) ? row[a.nt - 2 - k - q] : 0.f;
+        }
+    }
+
+#pragma unroll
+    for (int s = 1; s <= NS; ++s) {
+        const float (*cS)[kTile] = su[(s + 1) & 1];      // S_{s-1}
+        const float (*cR)[kTile] = ru[(s + 1) & 1];      // R_{s-1}
+        float (*nS)[kTile] = su[s & 1];                  // S_{s-2} -> S_s
+        float (*nR)[kTile] = ru[s & 1];
+        const bool zin = (z >= 4 * s) & (z < kTile - 4 * s);
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int x = xg + NW * m;                   // the same for the whole wave
+            if ((x < 4 * s) | (x >= kTile - 4 * s)) continue;
+            const float ps = cS[x][z], pr = cR[x][z];
+            const float lapS = lap8_cell(cS, x, z, ps);  // all 64 lanes
+            const float lapR = lap8_cell(cR, x, z, pr);
+            if (zin) {
+                const int gx = x0 + x;
+                const float os = nS[x][z], orr = nR[x][z];
+                const float sn = fmaf(cr[m], lapS, ps + (ps - os));
+                const float rn = fmaf(cr[m], lapR, pr + (pr - orr));
+                const bool inside = (gx >= 4) & (gx < nxx - 4) & (gz >= 4) & (gz < nzz - 4);
+                const float S = inside ? sn : os;
+                nS[x][z] = S;
+                nR[x][z] = inside ? rn : orr;
+                if (inside & zown & (gx >= xlo) & (gx < xhi)) {
+                    gi[m] = add_nc(gi[m], mul_nc(S, S));
+                    if constexpr (MODE == 0) ga[m] = add_nc(ga[m], mul_nc(S, pr));
+                    else ga[m] = add_nc(ga[m], mul_nc(pr, mul_nc(cr[m], lapS)));
+                }
+            }
+        }
+        if (k + s < a.nt) {                              // the step's data at the receivers
+            __syncthreads();
+            const int lo = 4 * s, hi = kTile - 4 * s;
+            if (colt && (c + 4 >= lo) && (c + 4 < hi)) {
+                const int x = c + 4, gx = x0 + x;
+                if ((gx >= 4) & (gx < nxx - 4))
+                    for (int e = scp[c]; e < scp[c + 1]; ++e) {
+                        const bool st = e < kRevTileRec;
+                        const int zz = (st ? riz[e] : a.rec.iz[e0 + e]) - z0;
+                        if (zz < lo || zz >= hi) continue;
+                        const float v = st ? rv[s - 1][e]
+                                           : dat[(long)a.rec.id[e0 + e] * a.nt + a.nt - 1 - s - k];
+                        nR[x][zz] = fmaf(a.idh2, v, nR[x][zz]);
+                    }
+            }
+        }
+        __syncthreads();
+        float *os_ = (s == NS) ? (NS == 1 ? a.sout1 : a.sout2) : (s == NS - 1 ? a.sout1 : nullptr);
+        float *or_ = (s == NS) ? (NS == 1 ? a.rout1 : a.rout2) : (s == NS - 1 ? a.rout1 : nullptr);
+        if (os_ && zown) {
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                const int x = xg + NW * m, gx = x0 + x;
+                if ((gx >= xlo) & (gx < xhi)) {
+                    const long idx = soff + (long)gx * ldz + gz;
+                    os_[idx] = nS[x][z];
+                    or_[idx] = nR[x][z];
+                }
+            }
+        }
+    }
+    if (zown) {
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int gx = x0 + xg + NW * m;
+            if ((gx >= xlo) & (gx < xhi)) {
+                const long idx = soff + (long)gx * ldz + gz;
+                a.acc[idx] = ga[m];
+                a.ill[idx] = gi[m];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- helpers
 
 __global__ void fwd_init_kernel(FwdStep a, float *field, int nslots)
@@ -2770,6 +2930,45 @@ hipError_t rev_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, i
                      : rev_launch<VZ, 1, false>(a, tl, nslots, k, steps, nblocks, s);
 }
 
+template <int NS, int NW>
+void revt_launch(const RevStep &a, const Tiling &tl, int nslots, int k, int nblocks,
+                 hipStream_t s)
+{
+    if (a.mode == 0)
+        hipLaunchKernelGGL((revt_kernel<NS, 0, NW>), dim3(nblocks), dim3(64 * NW), 0, s, a,
+                           tl.nzseg, tl.nxchunk, nslots, k);
+    else
+        hipLaunchKernelGGL((revt_kernel<NS, 1, NW>), dim3(nblocks), dim3(64 * NW), 0, s, a,
+                           tl.nzseg, tl.nxchunk, nslots, k);
+}
+
+template <int NW>
+hipError_t revt_steps(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
+                      int nblocks, hipStream_t s)
+{
+    switch (steps) {
+    case 1: revt_launch<1, NW>(a, tl, nslots, k, nblocks, s); break;
+    case 2: revt_launch<2, NW>(a, tl, nslots, k, nblocks, s); break;
+    case 3: revt_launch<3, NW>(a, tl, nslots, k, nblocks, s); break;
+    case 4: revt_launch<4, NW>(a, tl, nslots, k, nblocks, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t revt_dispatch(const RevStep &a, const Tiling &tl, int nslots, int k, int steps,
+                         hipStream_t s)
+{
+    const int ntiles = tl.nzseg * tl.nxchunk * nslots;
+    const int nblocks = ((ntiles + 7) >> 3) << 3;
+    switch (tl.xw) {
+    case 16: return revt_steps<4>(a, tl, nslots, k, steps, nblocks, s);
+    case 8: return revt_steps<8>(a, tl, nslots, k, steps, nblocks, s);
+    case 4: return revt_steps<16>(a, tl, nslots, k, steps, nblocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
 
 int zsegments(int nzz, int vz, int steps)
@@ -2806,6 +3005,7 @@ hipError_t launch_rev_step(const RevStep &a, const Tiling &tl, int nslots, int k
     case 1: return rev_dispatch<1>(a, tl, nslots, k, steps, s);
     case 2: return rev_dispatch<2>(a, tl, nslots, k, steps, s);
     case 4: return rev_dispatch<4>(a, tl, nslots, k, steps, s);
+    case kTileVz: return revt_dispatch(a, tl, nslots, k, steps, s);
     default: return hipErrorInvalidValue;
     }
 }

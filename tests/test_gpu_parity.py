@@ -125,21 +125,16 @@ SHAPES3 = [(3, 1, 8, 0), (3, 2, 17, 1), (3, 4, 64, 1), (3, 4, 1000, 0), (3, 1, 3
 # the largest that is staged in LDS
 SHAPES4 = [(4, 1, 8, 0), (4, 2, 17, 1), (4, 4, 64, 1), (4, 4, 1000, 0), (4, 2, 130, 1),
            (4, 1, 3, 0), (4, 2, 136, 1), (4, 1, 137, 0)]
-# the forward kernels on 64 x 64-cell LDS tiles (vz 16), one to four steps;
-# the reverse kernel then takes the steps with two rows per lane
+# the forward and reverse kernels on 64 x 64-cell LDS tiles (vz 16), one to
+# four steps, xw the tile columns per wave
 TILE = 16
 SHAPEST = [(1, TILE, 16, 0), (2, TILE, 8, 1), (3, TILE, 4, 0), (4, TILE, 16, 1), (4, TILE, 4, 0)]
 
 
 def set_shape(eng, st, vz, xw, g):
     """One launch shape for every kernel; the three- and four-step reverse
-    kernels have no four-rows-per-lane form and then take two steps; with
-    the forward LDS tiles the reverse kernel takes two rows per lane."""
-    if vz == TILE:
-        eng.set_tuning(2, 32, g, kernel=1, steps=st)
-        for k in (0, 2):
-            eng.set_tuning(vz, xw, g, kernel=k, steps=st)
-    elif st >= 3 and vz == 4:
+    kernels have no four-rows-per-lane form and then take two steps."""
+    if st >= 3 and vz == 4:
         eng.set_tuning(vz, xw, g, kernel=1, steps=2)
         for k in (0, 2):
             eng.set_tuning(vz, xw, g, kernel=k, steps=st)
@@ -196,7 +191,7 @@ def test_reverse_launch_shapes_bit_identical():
     s.eng.set_observed(obs)
     s.eng.set_model((s.v * 0.98).astype(np.float32))
     results = []
-    for st, vz, xw, g in [(1, 2, 32, 0)] + SHAPES[5:] + SHAPES3 + SHAPES4:
+    for st, vz, xw, g in [(1, 2, 32, 0)] + SHAPES[5:] + SHAPES3 + SHAPES4 + SHAPEST:
         set_shape(s.eng, st, vz, xw, g)
         s.eng.reset()
         s.eng.migration()
@@ -332,11 +327,17 @@ def test_dense_receiver_column_matches_oracle():
     for st in (1, 2, 3):                    # windows wider than the table: global path
         set_shape(s.eng, st, 2, 200, 0)
         assert np.array_equal(s.eng.modeling(F.ABC), seis), st
+    for st in (4, 1):                       # LDS tiles (own columns' receivers)
+        set_shape(s.eng, st, TILE, 8, 0)
+        assert np.array_equal(s.eng.modeling(F.ABC), seis), (st, TILE)
     obs = np.stack(obs).astype(np.float32)
     s.eng.set_observed(obs)
     rref, rraw, rill = migration_oracle(s, obs)
     first = None
-    for st, vz, xw in [(2, 2, 64), (1, 1, 8), (2, 4, 128), (1, 2, 200), (3, 2, 64), (3, 4, 200)]:
+    # the LDS tiles: the tiles holding the dense column stage 256 of its
+    # receivers' data and read the rest from global memory
+    for st, vz, xw in [(2, 2, 64), (1, 1, 8), (2, 4, 128), (1, 2, 200), (3, 2, 64), (3, 4, 200),
+                       (4, TILE, 8), (1, TILE, 4), (3, TILE, 16)]:
         set_shape(s.eng, st, vz, xw, 0)
         s.eng.reset()
         s.eng.migration()
@@ -446,15 +447,15 @@ def test_errors_are_reported():
         eng.set_tuning(4, kernel=1, steps=4)
     eng.set_tuning(kernel=-1, steps=4)
     assert [eng.get_tuning(k)[:2] for k in (0, 1, 2)] == [(4, 1), (4, 1), (4, 4)]
-    for kernel in (-1, 1):            # the LDS tiles are a forward kernel shape
-        with pytest.raises(F.FwiError):
-            eng.set_tuning(TILE, kernel=kernel)
     with pytest.raises(F.FwiError):
         eng.set_tuning(8, kernel=0)
-    with pytest.raises(F.FwiError):       # tile columns per wave: 4, 8 or 16
-        eng.set_tuning(TILE, 64, kernel=0)
+    for kernel in (-1, 0, 1):             # tile columns per wave: 4, 8 or 16
+        with pytest.raises(F.FwiError):
+            eng.set_tuning(TILE, 64, kernel=kernel)
     eng.set_tuning(TILE, 8, kernel=0)
     assert eng.get_tuning(0)[:3] == (4, TILE, 8) and eng.get_tuning(1)[:2] == (4, 1)
+    eng.set_tuning(TILE, 4, kernel=1)
+    assert eng.get_tuning(1)[:3] == (4, TILE, 4)
 
 
 def test_rejected_geometry_keeps_the_previous_one():
