@@ -23,7 +23,7 @@ struct Grid {
 // Wave tiling of one launch: each 64-lane wave computes 64*vz rows of one
 // x-chunk of xw columns of one shot slot and writes the inner
 // 64*vz - 8*(steps-1) of them (steps per launch: 1-4).
-// vz = kTileVz selects the forward kernel on 64 x 64-cell LDS tiles instead
+// vz = kTileVz selects the kernels on 64 x 64-cell LDS tiles instead
 // (nzseg x nxchunk tiles of (64 - 8 steps)^2 own cells; xw = the tile columns
 // each wave updates, 16, 8 or 4: 4, 8 or 16 waves per tile).
 constexpr int kTileVz = 16;

@@ -220,7 +220,7 @@ int fwi_synchronize(fwi_ctx *ctx);
  * reverse, 2 sponge-boundary forward, -1 all):
  * time steps per launch (1-4: temporal blocking; a pass's remainder runs as
  * shorter launches), cells per lane along z (1, 2 or 4; the three- and
- * four-step reverse kernels take 1 or 2; 16 selects the forward kernels on
+ * four-step reverse kernels take 1 or 2; 16 selects the kernels on
  * 64 x 64-cell tiles held in LDS, the shape for one or a few shots),
  * columns per wave (with the LDS tiles: the tile columns each wave
  * updates, 4, 8 or 16), and hipGraph capture of the time loops (1 on, 0
@@ -242,8 +242,8 @@ int fwi_get_tuning(fwi_ctx *ctx, int32_t kernel, int32_t *steps, int32_t *vz, in
  * columns per wave and 2 rows per lane.
  * Candidates: 1-4 steps per launch x 1, 2 or 4 rows per lane x chunk
  * widths 16-128 columns, 4 and 8 for small batches, and the widths that make
- * a launch 1-4 full rounds of the waves the device holds at once, and for
- * the forward kernels the 64 x 64-cell LDS tiles with 1-4 steps.  No
+ * a launch 1-4 full rounds of the waves the device holds at once, and the
+ * 64 x 64-cell LDS tiles with 1-4 steps and 4, 8 or 16 waves per tile.  No
  * reference counterpart (the reference launches one fixed shape). */
 int fwi_autotune(fwi_ctx *ctx, int32_t nslots, int32_t reps, double *fwd_us, double *rev_us);
 
