@@ -30,11 +30,16 @@
 //    are staged in LDS once per launch (a second build of each kernel reads
 //    them from global memory, for launches whose windows do not fit);
 //  * blocks are renumbered so that each XCD receives a contiguous run of
-//    tiles, and in the three-step kernels odd x-chunks march right to left:
-//    two neighbouring chunks then reach the columns they share at the same
-//    time, which keeps the second read of those halo columns in the caches
-//    (the stencil is symmetric and each sum is commutative, so the mirrored
-//    march gives the same bits).
+//    tiles, and in the three- and four-step kernels odd x-chunks march
+//    right to left: two neighbouring chunks then reach the columns they
+//    share at the same time, which keeps the second read of those halo
+//    columns in the caches (the stencil is symmetric and each sum is
+//    commutative, so the mirrored march gives the same bits);
+//  * for one or a few shots, where the column marches are too few to fill
+//    the device and each one's length sets the launch time, the forward and
+//    reverse steps also run on 64 x 64-cell tiles held in LDS (fwdt_kernel,
+//    revt_kernel): one workgroup per tile, the z neighbours by DPP shifts,
+//    the x neighbours from LDS, one to four steps per launch.
 //
 // FP32 operation order: the Laplacian's nearest ring is summed as
 // differences from the centre cell (the farther rings as sums minus 4u) and
@@ -42,7 +47,7 @@
 // reference's recurrence with less rounding error (about 4-5x less than the
 // reference's order against an FP64 evaluation of the same recurrence, over
 // 2,000+ steps).  Every multiply-add that feeds a stored value is an explicit
-// fmaf, so all launch shapes (one to three steps per launch, any tiling)
+// fmaf, so all launch shapes (one to four steps per launch, any tiling)
 // produce bit-identical fields, images and gradients.
 //
 // Source injection is folded into the write of the field it enters:
