@@ -308,7 +308,7 @@ fwi::RevStep rev_args(fwi_ctx *c, int mode)
 }
 
 // Enqueues the launches of one pass over T steps -- the tuned number of
-// steps per launch (1-3), shorter launches for the remainder -- or
+// steps per launch (1-4), shorter launches for the remainder -- or
 // replays the captured graph of the same sequence.  `sq` is the source (or
 // forward) wavefield, `rq` the receiver wavefield of a reverse pass; both
 // are advanced to their final state.

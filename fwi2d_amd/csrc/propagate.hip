@@ -362,7 +362,7 @@ __device__ __forceinline__ void ldcol(float (&r)[VZ], const float *base, int col
     else vzero<VZ>(r);
 }
 
-// Loads for the unrolled three-step kernels, with no branch: the column is
+// Loads for the unrolled three- and four-step kernels, with no branch: the column is
 // clamped into the grid, so a column beyond the edge reads the edge column,
 // which is a zero rim column of every wavefield (sources and receivers sit
 // inside the rim).  Such values only feed rim columns.
