@@ -5,9 +5,13 @@
 #include <omp.h>
 #include <xmmintrin.h>
 #include "fwi_oracle.h"
-/* flush-to-zero and denormals-are-zero on every worker thread, as a tuned
- * FP32 CPU stencil runs (denormal tails otherwise slow the sweep). */
-#define OR_THREAD_SETUP() _mm_setcsr(_mm_getcsr() | 0x8040)
+/* flush-to-zero and denormals-are-zero on every worker thread for the pass,
+ * as a tuned FP32 CPU stencil runs (denormal tails otherwise slow the
+ * sweep); each thread's previous MXCSR is put back at the end of the pass,
+ * so the caller's later arithmetic keeps its subnormals. */
+static __thread unsigned or_csr_saved;
+#define OR_THREAD_SETUP() (or_csr_saved = _mm_getcsr(), _mm_setcsr(or_csr_saved | 0x8040))
+#define OR_THREAD_RESTORE() _mm_setcsr(or_csr_saved)
 #define REAL float
 #define SFX(name) name##_f32
 #include "oracle_core.inc"
