@@ -670,7 +670,7 @@ __device__ __forceinline__ bool tile_of(int nzseg, int nxchunk, int nslots, Tile
 
 // One time step: u_{t+1} = leapfrog(u_t, u_{t-1}) written over u_{t-1}
 // (the odd remainder of a pass, and a tuning alternative).
-template <int VZ, bool ABC, bool ST>
+template <int VZ, bool ABC, bool ST, bool REC = true>
 __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int t)
 {
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
     const int gshot = *a.shot0 + id.slot;
     const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
     const float wsrc = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
-    const bool rec_on = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    const bool rec_on = REC && (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
     float *seis = rec_on ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
     __shared__ FwdStage st;
     constexpr bool staged = ST;     // tables fit the LDS (the host checks)
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(64) void fwd1_kernel(FwdStep a, int nzseg, int nxch
 // It reads u_t, u_{t-1} and c once for both steps and writes u_{t+1} (out1)
 // and u_{t+2} (out2) for the cells it owns; the four buffers are distinct,
 // so no wave reads a value another wave of the same launch writes.
-template <int VZ, bool ABC, bool ST>
+template <int VZ, bool ABC, bool ST, bool REC = true>
 __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int t)
 {
@@ -786,8 +786,8 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
     const int siz = a.src_iz[gshot], six = a.src_ix[gshot];
     const float w1 = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
     const float w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
-    const bool rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
-    const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
+    const bool rec1 = REC && (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    const bool rec2 = REC && (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
     float *seis = (rec1 || rec2) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
     const int xa = x0 - 4;                           // window [x0 - 4, x1 + 4)
     __shared__ FwdStage st;
@@ -907,7 +907,7 @@ __global__ __launch_bounds__(64) void fwd2_kernel(FwdStep a, int nzseg, int nxch
 // chunk; the three fields of a column travel through three register queues.
 // It reads u_t, u_{t-1} and c once and writes only what the next launch
 // needs: u_{t+2} (out1) and u_{t+3} (out2) for the cells it owns.
-template <int VZ, bool ABC, bool ST>
+template <int VZ, bool ABC, bool ST, bool REC = true>
 __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int t)
 {
@@ -941,9 +941,9 @@ __global__ __launch_bounds__(64, 3) void fwd3_kernel(FwdStep a, int nzseg, int n
     const float w1 = (t + 1 < a.nt) ? a.wav[t + 1] : 0.f;
     const float w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
     const float w3 = (t + 3 < a.nt) ? a.wav[t + 3] : 0.f;
-    const bool rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
-    const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
-    const bool rec3 = (a.seis != nullptr) && (t + 2 >= a.tlag) && (t + 2 <= a.T - 2);
+    const bool rec1 = REC && (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    const bool rec2 = REC && (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
+    const bool rec3 = REC && (a.seis != nullptr) && (t + 2 >= a.tlag) && (t + 2 <= a.T - 2);
     float *seis = (rec1 || rec2 || rec3) ? a.seis + (long)id.slot * a.rec.nrec * a.nt : nullptr;
     const int xa = x0 - 8;                           // window [x0 - 8, x1 + 8)
     __shared__ FwdStage st;
@@ -1146,7 +1146,7 @@ __device__ __forceinline__ void sample2_receivers(const FwdStep &a, const float 
 // written when step 2 reaches its column (u_{t+1} is then still in the
 // second queue), those of steps 3 and 4 when step 4 does.  Same operations
 // in the same order as every other forward kernel, so the same bits.
-template <int VZ, bool ABC, bool ST>
+template <int VZ, bool ABC, bool ST, bool REC = true>
 __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxchunk, int xw,
                                                   int nslots, int t)
 {
@@ -1179,10 +1179,10 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
     const float w2 = (t + 2 < a.nt) ? a.wav[t + 2] : 0.f;
     const float w3 = (t + 3 < a.nt) ? a.wav[t + 3] : 0.f;
     const float w4 = (t + 4 < a.nt) ? a.wav[t + 4] : 0.f;
-    const bool rec1 = (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
-    const bool rec2 = (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
-    const bool rec3 = (a.seis != nullptr) && (t + 2 >= a.tlag) && (t + 2 <= a.T - 2);
-    const bool rec4 = (a.seis != nullptr) && (t + 3 >= a.tlag) && (t + 3 <= a.T - 2);
+    const bool rec1 = REC && (a.seis != nullptr) && (t >= a.tlag) && (t <= a.T - 2);
+    const bool rec2 = REC && (a.seis != nullptr) && (t + 1 >= a.tlag) && (t + 1 <= a.T - 2);
+    const bool rec3 = REC && (a.seis != nullptr) && (t + 2 >= a.tlag) && (t + 2 <= a.T - 2);
+    const bool rec4 = REC && (a.seis != nullptr) && (t + 3 >= a.tlag) && (t + 3 <= a.T - 2);
     float *seis = (rec1 || rec2 || rec3 || rec4) ? a.seis + (long)id.slot * a.rec.nrec * a.nt
                                                  : nullptr;
     const int xa = x0 - 12;                          // window [x0 - 12, x1 + 12)
@@ -2821,21 +2821,21 @@ inline int grid1d(long n, int block, int cap = 8192)
     return (int)b;
 }
 
-template <int VZ, bool ABC, bool ST>
+template <int VZ, bool ABC, bool ST, bool REC = true>
 void fwd_launch(const FwdStep &a, const Tiling &tl, int nslots, int t, int steps, int nblocks,
                 hipStream_t s)
 {
     if (steps == 4)
-        hipLaunchKernelGGL((fwd4_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
+        hipLaunchKernelGGL((fwd4_kernel<VZ, ABC, ST, REC>), dim3(nblocks), dim3(64), 0, s, a,
                            tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
     else if (steps == 3)
-        hipLaunchKernelGGL((fwd3_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
+        hipLaunchKernelGGL((fwd3_kernel<VZ, ABC, ST, REC>), dim3(nblocks), dim3(64), 0, s, a,
                            tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
     else if (steps == 2)
-        hipLaunchKernelGGL((fwd2_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
+        hipLaunchKernelGGL((fwd2_kernel<VZ, ABC, ST, REC>), dim3(nblocks), dim3(64), 0, s, a,
                            tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
     else
-        hipLaunchKernelGGL((fwd1_kernel<VZ, ABC, ST>), dim3(nblocks), dim3(64), 0, s, a,
+        hipLaunchKernelGGL((fwd1_kernel<VZ, ABC, ST, REC>), dim3(nblocks), dim3(64), 0, s, a,
                            tl.nzseg, tl.nxchunk, tl.xw, nslots, t);
 }
 
@@ -2848,6 +2848,11 @@ hipError_t fwd_dispatch(const FwdStep &a, const Tiling &tl, int nslots, int t, i
     if (a.abc) {
         if (tl.staged) fwd_launch<VZ, true, true>(a, tl, nslots, t, steps, nblocks, s);
         else fwd_launch<VZ, true, false>(a, tl, nslots, t, steps, nblocks, s);
+    } else if (a.seis == nullptr) {
+        // random-boundary passes that record nothing (migration, gradient):
+        // the build without the sampling code, which leaves the kernel
+        // fewer scalar registers to spill (the only staged table is unused)
+        fwd_launch<VZ, false, true, false>(a, tl, nslots, t, steps, nblocks, s);
     } else {
         if (tl.staged) fwd_launch<VZ, false, true>(a, tl, nslots, t, steps, nblocks, s);
         else fwd_launch<VZ, false, false>(a, tl, nslots, t, steps, nblocks, s);
@@ -3031,10 +3036,10 @@ template <int VZ>
 int resident_vz(int kind, int steps)
 {
     if (kind == 0)
-        return steps == 4   ? resident_of(fwd4_kernel<VZ, false, true>)
-               : steps == 3 ? resident_of(fwd3_kernel<VZ, false, true>)
-               : steps == 2 ? resident_of(fwd2_kernel<VZ, false, true>)
-                            : resident_of(fwd1_kernel<VZ, false, true>);
+        return steps == 4   ? resident_of(fwd4_kernel<VZ, false, true, false>)
+               : steps == 3 ? resident_of(fwd3_kernel<VZ, false, true, false>)
+               : steps == 2 ? resident_of(fwd2_kernel<VZ, false, true, false>)
+                            : resident_of(fwd1_kernel<VZ, false, true, false>);
     if (kind == 2)
         return steps == 4   ? resident_of(fwd4_kernel<VZ, true, true>)
                : steps == 3 ? resident_of(fwd3_kernel<VZ, true, true>)
