@@ -1362,9 +1362,10 @@ __global__ __launch_bounds__(64) void fwd4_kernel(FwdStep a, int nzseg, int nxch
 // ---------------------------------------------------------------- 2-D tiles
 //
 // Forward steps on 64 x 64-cell tiles held in LDS, one workgroup of NW
-// waves per tile (each thread: one row of every NW-th column): for batches of one or a few shots, where the column-marching
-// waves above are too few to fill the device and each wave's march over its
-// chunk plus halo sets the launch time.  A launch of NS steps loads u_t,
+// waves per tile (each thread: one row of every NW-th column): for batches
+// of one or a few shots, where the column-marching waves above are too few
+// to fill the device and each wave's march over its chunk plus halo sets the
+// launch time.  A launch of NS steps loads u_t,
 // u_{t-1} and c of the tile (out-of-grid cells as zero) and runs the steps
 // in LDS, each on a region 4 cells narrower on every side than the last; the
 // inner (64 - 8 NS)^2 cells are the tile's own.  Every cell is updated with
