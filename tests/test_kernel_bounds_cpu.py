@@ -17,9 +17,11 @@ The halo clamp is the case this pins: the first z segment starts at row -4,
 so its halo rows -8 .. -5 lie in the previous column, which exists only for
 columns >= 1 (clamped to >= 4 here).
 
-The forward kernel on 64 x 64-cell LDS tiles (`fwdt_kernel`) is modelled
-at the end: its stores cover the updated cells once, and every LDS read of
-every step is a cell the load phase filled or the previous step wrote."""
+The kernels on 64 x 64-cell LDS tiles (`fwdt_kernel`, `revt_kernel`, the
+same tiling and step regions) are modelled at the end: their stores cover
+the updated cells once, every LDS read of every step is a cell the load
+phase filled or the previous step wrote, and the NW waves of a tile (thread
+columns xg, xg + NW, ...) update each tile column once."""
 import itertools
 
 import numpy as np
@@ -199,3 +201,9 @@ def test_tile_kernel_reads_only_what_it_loaded_or_wrote(steps):
         if steps == 1:
             h = 4
             assert lo == h and hi == n - h
+
+
+@pytest.mark.parametrize("nw", [4, 8, 16])
+def test_tile_threads_cover_each_column_once(nw):
+    cols = [xg + nw * m for xg in range(nw) for m in range(TILE // nw)]
+    assert sorted(cols) == list(range(TILE))
