@@ -153,7 +153,8 @@ def test_inversion_resume_equals_straight_run(tmp_path):
 
 def test_prepared_configuration_end_to_end(tmp_path):
     """The data-preparation commands feed the executables: setup of the
-    one-shot 121 x 401 configuration, modeling, the direct-wave mute,
+    one-shot 121 x 401 configuration, modeling, the direct-wave mute (Python
+    and C++ tools, the same bytes),
     migration of the muted data (whose image equals the library's on the
     same muted data), a two-iteration inversion and the viewer's figures."""
     from fwi2d_amd import viewer
@@ -163,6 +164,15 @@ def test_prepared_configuration_end_to_end(tmp_path):
     run("modeling", mod)
     muted = C.main(["mute", mod])
     assert len(muted) == 1
+    # the C++ tool writes the same bytes from the executable's seismogram
+    with open(muted[0], "rb") as f:
+        py_bytes = f.read()
+    r = subprocess.run([os.path.join(BIN, "prep"), "mute", mod], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split() == muted
+    with open(muted[0], "rb") as f:
+        assert f.read() == py_bytes
     obs = np.fromfile(muted[0], np.float32).reshape(1, s.nrec, s.nt)
     raw = np.fromfile(muted[0].replace("input_RTM_", ""), np.float32).reshape(s.nrec, s.nt)
     # normalised by the shot's peak, then the direct wave (that peak) tapered
