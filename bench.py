@@ -61,7 +61,7 @@ def parse():
     p.add_argument("--no-traffic", action="store_true",
                    help="skip the rocprofv3 counter passes (roofline.traffic = null)")
     p.add_argument("--workload", default="rtm64",
-                   choices=("rtm64", "forward1", "fwi128", "grad256", "net"),
+                   choices=("rtm64", "forward1", "rtm1", "fwi128", "grad256", "net"),
                    help="rtm64 is the headline; forward1, fwi128 and grad256 show the rate's "
                         "dependence on shape; net times the learned preconditioner's Adam step")
     p.add_argument("--pmc-child", action="store_true",
@@ -191,6 +191,9 @@ def other_workload(a):
     the problem shape (same JSON keys; not the headline):
       forward1  1 shot, 121x401 + 100-cell random boundary, nt 2001, 1 ms,
                 15 Hz peak, 401 receivers (the field fits in the caches)
+      rtm1      the RTM of that shot (observed data modeled with a 100-cell
+                sponge on the true model, migrated on its smoothed-slowness
+                starting model)
       fwi128    one plain FWI iteration (gradient pass + two line-search
                 misfits), 128 shots, 301x1001 + 150 (random) / 100 (sponge),
                 nt 4001, 0.8 ms, 8 Hz peak, 1001 receivers
@@ -200,10 +203,10 @@ def other_workload(a):
     import fwi2d_amd as F
     from fwi2d_amd import configs as C
     copy_gbs = F.abi.stream_copy(1 << 30, reps=20)
-    if a.workload == "forward1":
+    if a.workload in ("forward1", "rtm1"):
         s = C.CONFIG1
         true_v = C.config1_model()
-        init_v = None
+        init_v = None if a.workload == "forward1" else C.smooth_slowness(true_v, 3.0)
         s = dataclasses.replace(s, seed=2024)
     elif a.workload == "fwi128":
         s = C.CONFIG3
@@ -240,6 +243,10 @@ def other_workload(a):
         if a.workload == "forward1":
             eng.modeling(F.RBC)
             return s.nsrc * nzz * nxx * T
+        if a.workload == "rtm1":
+            eng.reset(F.RESET_IMAGE | F.RESET_ILLUM)
+            eng.migration()
+            return s.nsrc * nzz * nxx * T * 3.0
         eng.reset(F.RESET_GRADIENT)
         eng.gradient(iteration=1)
         cells = s.nsrc * (azz * axx + nzz * nxx * 3.0) * T
